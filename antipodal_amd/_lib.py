@@ -1,0 +1,120 @@
+"""ctypes binding of libagwn.so (include/agwn.h).
+
+The library is the HIP product; this module only declares argument types, converts numpy / torch buffers to
+pointers and raises AgwnError on a nonzero status. There is no CPU fallback: if libagwn.so is missing, importing
+antipodal_amd fails with ImportError.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().with_name("libagwn.so")
+HEADER = Path(__file__).resolve().parent.parent / "include" / "agwn.h"
+
+_i64 = C.c_int64
+_p = C.c_void_p
+_int = C.c_int
+
+
+class AgwnError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"agwn status {code}: {msg}")
+        self.code = code
+
+
+# argument signatures; every entry point returns int except the three informational ones
+_SIG = {
+    "agwn_last_error": (C.c_char_p, []),
+    "agwn_version": (_int, []),
+    "agwn_device_available": (_int, []),
+    "agwn_frac_batch_f32": (_int, [_p, _i64, _p, _i64, _p, _p]),
+    "agwn_frac_batch_f64": (_int, [_p, _i64, _p, _i64, _p, _p]),
+    "agwn_frac_batch_f32_dev": (_int, [_p, _i64, _p, _i64, _p, _p, _p]),
+    "agwn_frac_batch_f64_dev": (_int, [_p, _i64, _p, _i64, _p, _p, _p]),
+    "agwn_mesh_create_f32": (_int, [_p, _i64, _p, _i64, C.POINTER(_p)]),
+    "agwn_mesh_create_f64": (_int, [_p, _i64, _p, _i64, C.POINTER(_p)]),
+    "agwn_mesh_destroy": (_int, [_p]),
+    "agwn_mesh_info": (_int, [_p, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_int)]),
+    "agwn_int_batch_f32": (_int, [_p, _p, _i64, _p, _p]),
+    "agwn_int_batch_f64": (_int, [_p, _p, _i64, _p, _p]),
+    "agwn_int_batch_f32_dev": (_int, [_p, _p, _i64, _p, _p, _p]),
+    "agwn_int_batch_f64_dev": (_int, [_p, _p, _i64, _p, _p, _p]),
+    "agwn_full_batch_f32": (_int, [_p, _p, _i64, _p, _p, _i64, _p]),
+    "agwn_full_batch_f64": (_int, [_p, _p, _i64, _p, _p, _i64, _p]),
+    "agwn_full_batch_f32_dev": (_int, [_p, _p, _i64, _p, _p, _i64, _p, _p]),
+    "agwn_full_batch_f64_dev": (_int, [_p, _p, _i64, _p, _p, _i64, _p, _p]),
+    "agwn_boundary_f32": (_int, [_p, _i64, _p, _i64, _p, _i64, C.POINTER(_i64)]),
+    "agwn_boundary_f64": (_int, [_p, _i64, _p, _i64, _p, _i64, C.POINTER(_i64)]),
+}
+
+
+def header_symbols() -> list[str]:
+    """Names of every function include/agwn.h declares."""
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(agwn_[a-z0-9_]+)\s*\(", text)))
+
+
+def _load() -> C.CDLL:
+    if not LIB_PATH.exists():
+        raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() (hipcc --offload-arch=gfx950)")
+    lib = C.CDLL(str(LIB_PATH))
+    for name, (res, args) in _SIG.items():
+        fn = getattr(lib, name, None)
+        if fn is None:
+            continue  # reported by the symbol test, not here
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def check(status: int) -> None:
+    if status != 0:
+        raise AgwnError(status, lib.agwn_last_error().decode(errors="replace"))
+
+
+def last_error() -> str:
+    return lib.agwn_last_error().decode(errors="replace")
+
+
+def device_available() -> bool:
+    return bool(lib.agwn_device_available())
+
+
+def ptr(a) -> int | None:
+    """Raw address of a C-contiguous numpy array or torch tensor (None for None)."""
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        if not a.flags["C_CONTIGUOUS"]:
+            raise ValueError("array must be C-contiguous")
+        return a.ctypes.data
+    if hasattr(a, "data_ptr"):
+        if not a.is_contiguous():
+            raise ValueError("tensor must be contiguous")
+        return a.data_ptr()
+    raise TypeError(f"unsupported buffer type {type(a)!r}")
+
+
+def sfx(dtype) -> str:
+    dt = np.dtype(dtype) if not hasattr(dtype, "is_floating_point") else None
+    if dt is not None:
+        if dt == np.float32:
+            return "f32"
+        if dt == np.float64:
+            return "f64"
+    else:  # torch dtype
+        import torch
+
+        if dtype == torch.float32:
+            return "f32"
+        if dtype == torch.float64:
+            return "f64"
+    raise TypeError(f"unsupported scalar type {dtype}")

@@ -1,0 +1,178 @@
+"""Host-side Python mirror of the reference's batch-GWN entry points, over the C ABI.
+
+  eval_gwnr_mesh_batch_fractional(positions, x0, boundary)   gwn_mesh.hh:67-83
+  eval_gwnr_mesh_batch_integer(mesh, positions, x0)          gwn_mesh.hh:127-143
+  eval_gwnr_mesh_batch(mesh, positions, x0, boundary)        gwn_mesh.hh:201-219
+  build_boundary_segments(vertices, indices)                 math/common.hh:123-174
+  HipMesh(vertices, indices)                                 the Intersector concept, intersector/intersector.hh:8-43
+
+Arrays are numpy (host; results come back as numpy) or torch CUDA tensors (device; the *_dev entry points run on the
+given stream and write into `out`). Layouts are the reference's: positions (n, 3), boundary (nseg, 7) with the weight
+last, vertices (nv, 3), indices (3*ntri,) int32. Size mismatches raise ValueError where the reference asserts
+(gwn_mesh.hh:74,134,209); a failing library call raises AgwnError.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib, ptr, sfx
+
+
+def _host(a, dtype, cols: int | None = None, name: str = "array") -> np.ndarray:
+    a = np.ascontiguousarray(np.asarray(a, dtype=dtype))
+    if cols is not None:
+        a = a.reshape(-1, cols)
+    return a
+
+
+def _is_dev(a) -> bool:
+    return hasattr(a, "is_cuda") and bool(a.is_cuda)
+
+
+def _x0(x0, dtype) -> np.ndarray:
+    x = np.ascontiguousarray(np.asarray(x0, dtype=dtype).reshape(3))
+    return x
+
+
+def _stream_ptr(stream) -> int | None:
+    if stream is None:
+        try:
+            import torch
+
+            return torch.cuda.current_stream().cuda_stream
+        except Exception:  # pragma: no cover - torch without a device
+            return None
+    return getattr(stream, "cuda_stream", stream)
+
+
+def build_boundary_segments(vertices, indices) -> np.ndarray:
+    """Open boundary as (nseg, 7) weighted oriented segments, sorted by (min index, max index)."""
+    v = np.ascontiguousarray(np.asarray(vertices))
+    if v.dtype not in (np.float32, np.float64):
+        v = v.astype(np.float64)
+    v = v.reshape(-1, 3)
+    idx = _host(indices, np.int32).reshape(-1)
+    if idx.size % 3:
+        raise ValueError("indices.size() must be a multiple of 3")
+    fn = getattr(lib, f"agwn_boundary_{sfx(v.dtype)}")
+    cap = idx.size
+    out = np.empty((max(cap, 1), 7), dtype=v.dtype)
+    n = C.c_int64(0)
+    check(fn(ptr(v), len(v), ptr(idx), idx.size, ptr(out), cap, C.byref(n)))
+    return out[: n.value].copy()
+
+
+def eval_gwnr_mesh_batch_fractional(positions, x0, boundary, out=None, stream=None):
+    """Fractional GWN of every position (gwn_mesh.hh:67-83). Host numpy in -> numpy out; torch CUDA in -> `out`."""
+    if _is_dev(positions):
+        s = sfx(positions.dtype)
+        n = positions.numel() // 3
+        if out is None or out.numel() != n:
+            raise ValueError("positions.size() != out_wnrs.size()")
+        nseg = 0 if boundary is None else boundary.numel() // 7
+        x = _x0(x0, np.float32 if s == "f32" else np.float64)
+        fn = getattr(lib, f"agwn_frac_batch_{s}_dev")
+        check(fn(ptr(positions), n, ptr(boundary) if nseg else None, nseg, ptr(x), ptr(out), _stream_ptr(stream)))
+        return out
+    pos = np.asarray(positions)
+    dt = np.float32 if pos.dtype == np.float32 else np.float64
+    pos = _host(pos, dt, 3)
+    seg = _host(boundary if boundary is not None else np.zeros((0, 7)), dt, 7)
+    res = np.empty(len(pos), dtype=dt) if out is None else out
+    if res.shape != (len(pos),) or res.dtype != dt:
+        raise ValueError("positions.size() != out_wnrs.size()")
+    fn = getattr(lib, f"agwn_frac_batch_{sfx(dt)}")
+    check(fn(ptr(pos), len(pos), ptr(seg) if len(seg) else None, len(seg), ptr(_x0(x0, dt)), ptr(res)))
+    return res
+
+
+class HipMesh:
+    """Prepared triangle mesh on the GPU (LBVH). Satisfies the reference's Intersector concept in batch form:
+    signed_intersection_count(positions, dir) returns, per position, sum sign(N . dir) over every triangle the ray
+    (p, dir) hits at t >= 0 -- NaiveIntersector's Moeller-Trumbore arithmetic, bit for bit (intersector.hh:61-100)."""
+
+    def __init__(self, vertices, indices, dtype=None):
+        v = np.asarray(vertices)
+        dt = np.dtype(dtype) if dtype is not None else (np.float32 if v.dtype == np.float32 else np.float64)
+        self.dtype = dt
+        self._v = _host(v, dt, 3)
+        self._i = _host(indices, np.int32).reshape(-1)
+        if self._i.size % 3:
+            raise ValueError("indices.size() must be a multiple of 3")
+        h = C.c_void_p()
+        fn = getattr(lib, f"agwn_mesh_create_{sfx(dt)}")
+        check(fn(ptr(self._v), len(self._v), ptr(self._i), self._i.size, C.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def info(self) -> dict:
+        ntri, nodes, sb = C.c_int64(), C.c_int64(), C.c_int()
+        check(lib.agwn_mesh_info(self._h, C.byref(ntri), C.byref(nodes), C.byref(sb)))
+        return {"ntri": ntri.value, "nodes": nodes.value, "scalar_bytes": sb.value}
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib.agwn_mesh_destroy(self._h)
+            self._h = None
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def signed_intersection_count(self, positions, direction, out=None, stream=None):
+        s = sfx(self.dtype)
+        if _is_dev(positions):
+            n = positions.numel() // 3
+            if out is None or out.numel() != n:
+                raise ValueError("positions.size() != out_counts.size()")
+            fn = getattr(lib, f"agwn_int_batch_{s}_dev")
+            check(fn(self._h, ptr(positions), n, ptr(_x0(direction, self.dtype)), ptr(out), _stream_ptr(stream)))
+            return out
+        pos = _host(positions, self.dtype, 3)
+        res = np.empty(len(pos), dtype=np.int32)
+        fn = getattr(lib, f"agwn_int_batch_{s}")
+        check(fn(self._h, ptr(pos), len(pos), ptr(_x0(direction, self.dtype)), ptr(res)))
+        return res
+
+    def full(self, positions, x0, boundary, out=None, stream=None):
+        s = sfx(self.dtype)
+        if _is_dev(positions):
+            n = positions.numel() // 3
+            if out is None or out.numel() != n:
+                raise ValueError("positions.size() != out_wnrs.size()")
+            nseg = 0 if boundary is None else boundary.numel() // 7
+            fn = getattr(lib, f"agwn_full_batch_{s}_dev")
+            check(fn(self._h, ptr(positions), n, ptr(_x0(x0, self.dtype)), ptr(boundary) if nseg else None, nseg,
+                     ptr(out), _stream_ptr(stream)))
+            return out
+        pos = _host(positions, self.dtype, 3)
+        seg = _host(boundary if boundary is not None else np.zeros((0, 7)), self.dtype, 7)
+        res = np.empty(len(pos), dtype=self.dtype)
+        fn = getattr(lib, f"agwn_full_batch_{s}")
+        check(fn(self._h, ptr(pos), len(pos), ptr(_x0(x0, self.dtype)), ptr(seg) if len(seg) else None, len(seg),
+                 ptr(res)))
+        return res
+
+
+def eval_gwnr_mesh_batch_integer(mesh: HipMesh, positions, x0, out=None, stream=None):
+    """Integer GWN term of every position (gwn_mesh.hh:127-143)."""
+    return mesh.signed_intersection_count(positions, x0, out=out, stream=stream)
+
+
+def eval_gwnr_mesh_batch(mesh: HipMesh, positions, x0, boundary, out=None, stream=None):
+    """Full GWN = fractional + integer of every position (gwn_mesh.hh:201-219)."""
+    return mesh.full(positions, x0, boundary, out=out, stream=stream)

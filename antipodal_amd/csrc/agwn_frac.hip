@@ -1,0 +1,262 @@
+// Fractional GWN term on gfx950: the boundary sum of gwn_mesh.hh:34-46 / math/common.hh:181-189, batched.
+//
+// Work decomposition. One lane owns Q query points; a 256-lane workgroup owns 256*Q consecutive queries and walks the
+// whole boundary. The boundary is pre-rotated once per call into the pole frame (row 0 = x1 = -x0), packed 32 B per
+// segment, and read with wave-uniform addresses, so each segment arrives in SGPRs through one scalar load and feeds
+// the VALU of all 64 lanes: no LDS traffic and no bank conflicts in the inner loop.
+//
+// Per (query, segment) pair in the pole frame, with v = endpoint - query:
+//   num = v0.y v1.z - v0.z v1.y                         (= x1 . (v0 x v1))
+//   den = (l0 + v0.x)(l1 + v1.x) + v0.y v1.y + v0.z v1.z (= l0 l1 + l1 x1.v0 + l0 x1.v1 + v0.v1)
+//   area += w * atan2(num, den)
+// FP32 kernel: the pair is evaluated in FP32 and accumulated in FP64 (one convert + one FP64 FMA per pair; this is
+// cheaper than FP32 two-sum compensation and strictly more accurate). A pair whose (num, den) is small against
+// l0*l1 is ill-conditioned -- the query is near the segment, or an endpoint is seen almost along the pole -- and is
+// re-evaluated in FP64 from the FP64 copy of the segment with the cancellation-free l + v.x = (v.y^2+v.z^2)/(l - v.x).
+// FP64 kernel: every pair in FP64 with the cancellation-free form and two-sum compensated accumulation.
+
+#include "agwn_internal.hpp"
+
+#include <algorithm>
+
+namespace agwn
+{
+namespace
+{
+constexpr int kBlock = 256;
+constexpr double kTwoPi = 6.283185307179586476925286766559;
+
+frame make_frame_impl(double x0x, double x0y, double x0z)
+{
+    double ex = -x0x, ey = -x0y, ez = -x0z;
+    double const l = std::sqrt(ex * ex + ey * ey + ez * ez);
+    if (l > 0.0)
+    {
+        ex /= l;
+        ey /= l;
+        ez /= l;
+    }
+    else
+    {
+        ex = 1.0;
+        ey = 0.0;
+        ez = 0.0;
+    }
+    // helper axis least aligned with e1
+    double hx = 0.0, hy = 0.0, hz = 0.0;
+    if (std::fabs(ex) < 0.9)
+        hx = 1.0;
+    else
+        hy = 1.0;
+    // e2 = normalize(e1 x h), e3 = e1 x e2  -> rows (e1, e2, e3) are right-handed and orthonormal
+    double fx = ey * hz - ez * hy, fy = ez * hx - ex * hz, fz = ex * hy - ey * hx;
+    double const lf = std::sqrt(fx * fx + fy * fy + fz * fz);
+    fx /= lf;
+    fy /= lf;
+    fz /= lf;
+    double const gx = ey * fz - ez * fy, gy = ez * fx - ex * fz, gz = ex * fy - ey * fx;
+    return frame{{ex, ey, ez, fx, fy, fz, gx, gy, gz}};
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_prep_segments(T const* __restrict__ seg7, int64_t nseg, frame R,
+                                                          seg_f32* __restrict__ s32, seg_f64* __restrict__ s64)
+{
+    int64_t const i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (i >= nseg)
+        return;
+    T const* g = seg7 + 7 * i;
+    double const ax = double(g[0]), ay = double(g[1]), az = double(g[2]);
+    double const bx = double(g[3]), by = double(g[4]), bz = double(g[5]);
+    double const w = double(g[6]);
+    double const rax = R.r[0] * ax + R.r[1] * ay + R.r[2] * az;
+    double const ray = R.r[3] * ax + R.r[4] * ay + R.r[5] * az;
+    double const raz = R.r[6] * ax + R.r[7] * ay + R.r[8] * az;
+    double const rbx = R.r[0] * bx + R.r[1] * by + R.r[2] * bz;
+    double const rby = R.r[3] * bx + R.r[4] * by + R.r[5] * bz;
+    double const rbz = R.r[6] * bx + R.r[7] * by + R.r[8] * bz;
+    if (s32)
+        s32[i] = seg_f32{float(rax), float(ray), float(raz), float(rbx), float(rby), float(rbz), float(w), 0.0f};
+    s64[i] = seg_f64{rax, ray, raz, rbx, rby, rbz, w, 0.0};
+}
+
+// One pair in FP64, cancellation-free denominator. v0 = a - p, v1 = b - p in the pole frame.
+__device__ __forceinline__ double pair_f64(double v0x, double v0y, double v0z, double v1x, double v1y, double v1z)
+{
+    double const yz0 = v0y * v0y + v0z * v0z;
+    double const yz1 = v1y * v1y + v1z * v1z;
+    double const l0 = sqrt(v0x * v0x + yz0);
+    double const l1 = sqrt(v1x * v1x + yz1);
+    double const lp0 = v0x >= 0.0 ? l0 + v0x : yz0 / (l0 - v0x);
+    double const lp1 = v1x >= 0.0 ? l1 + v1x : yz1 / (l1 - v1x);
+    double const num = v0y * v1z - v0z * v1y;
+    double const den = lp0 * lp1 + (v0y * v1y + v0z * v1z);
+    return atan2(num, den);
+}
+
+template <int Q>
+__global__ void __launch_bounds__(kBlock)
+    k_frac_f32(float const* __restrict__ pos, int64_t n, seg_f32 const* __restrict__ seg,
+               seg_f64 const* __restrict__ seg64, int64_t nseg, frame R, float tau, int32_t const* __restrict__ counts,
+               float* __restrict__ out)
+{
+    int64_t const base = int64_t(blockIdx.x) * (kBlock * Q) + threadIdx.x;
+    float px[Q], py[Q], pz[Q];
+    double dx[Q], dy[Q], dz[Q], acc[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+    {
+        int64_t const i = base + int64_t(q) * kBlock;
+        double x = 0.0, y = 0.0, z = 0.0;
+        if (i < n)
+        {
+            x = double(pos[3 * i + 0]);
+            y = double(pos[3 * i + 1]);
+            z = double(pos[3 * i + 2]);
+        }
+        dx[q] = R.r[0] * x + R.r[1] * y + R.r[2] * z;
+        dy[q] = R.r[3] * x + R.r[4] * y + R.r[5] * z;
+        dz[q] = R.r[6] * x + R.r[7] * y + R.r[8] * z;
+        px[q] = float(dx[q]);
+        py[q] = float(dy[q]);
+        pz[q] = float(dz[q]);
+        acc[q] = 0.0;
+    }
+
+#pragma unroll 2
+    for (int64_t s = 0; s < nseg; ++s)
+    {
+        seg_f32 const g = seg[s];
+        double const w = double(g.w);
+#pragma unroll
+        for (int q = 0; q < Q; ++q)
+        {
+            float const v0x = g.ax - px[q], v0y = g.ay - py[q], v0z = g.az - pz[q];
+            float const v1x = g.bx - px[q], v1y = g.by - py[q], v1z = g.bz - pz[q];
+            float const l0 = __builtin_sqrtf(v0x * v0x + v0y * v0y + v0z * v0z);
+            float const l1 = __builtin_sqrtf(v1x * v1x + v1y * v1y + v1z * v1z);
+            float const num = v0y * v1z - v0z * v1y;
+            float const den = (l0 + v0x) * (l1 + v1x) + (v0y * v1y + v0z * v1z);
+            double a;
+            if (__builtin_expect(__builtin_fabsf(num) + __builtin_fabsf(den) < tau * (l0 * l1), 0))
+            {
+                seg_f64 const h = seg64[s];
+                a = pair_f64(h.ax - dx[q], h.ay - dy[q], h.az - dz[q], h.bx - dx[q], h.by - dy[q], h.bz - dz[q]);
+            }
+            else
+            {
+                a = double(atan2f(num, den));
+            }
+            acc[q] = fma(w, a, acc[q]);
+        }
+    }
+
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+    {
+        int64_t const i = base + int64_t(q) * kBlock;
+        if (i < n)
+        {
+            double r = acc[q] / kTwoPi;
+            if (counts)
+                r += double(counts[i]);
+            out[i] = float(r);
+        }
+    }
+}
+
+__device__ __forceinline__ void two_sum(double a, double b, double& s, double& e)
+{
+    s = a + b;
+    double const bv = s - a;
+    double const av = s - bv;
+    e = (a - av) + (b - bv);
+}
+
+__global__ void __launch_bounds__(kBlock)
+    k_frac_f64(double const* __restrict__ pos, int64_t n, seg_f64 const* __restrict__ seg, int64_t nseg, frame R,
+               int32_t const* __restrict__ counts, double* __restrict__ out)
+{
+    int64_t const i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    double x = 0.0, y = 0.0, z = 0.0;
+    if (i < n)
+    {
+        x = pos[3 * i + 0];
+        y = pos[3 * i + 1];
+        z = pos[3 * i + 2];
+    }
+    double const px = R.r[0] * x + R.r[1] * y + R.r[2] * z;
+    double const py = R.r[3] * x + R.r[4] * y + R.r[5] * z;
+    double const pz = R.r[6] * x + R.r[7] * y + R.r[8] * z;
+    double sum = 0.0, comp = 0.0;
+#pragma unroll 2
+    for (int64_t s = 0; s < nseg; ++s)
+    {
+        seg_f64 const g = seg[s];
+        double const a = pair_f64(g.ax - px, g.ay - py, g.az - pz, g.bx - px, g.by - py, g.bz - pz);
+        double t, e;
+        two_sum(sum, g.w * a, t, e);
+        sum = t;
+        comp += e;
+    }
+    if (i < n)
+    {
+        double r = (sum + comp) / kTwoPi;
+        if (counts)
+            r += double(counts[i]);
+        out[i] = r;
+    }
+}
+
+// ill-conditioning threshold of the FP32 kernel: pairs with |num| + |den| < tau * l0 * l1 go to FP64
+constexpr float kTau = 1.0f / 32.0f;
+constexpr int kQ = 2;
+} // namespace
+
+frame make_pole_frame(double x0x, double x0y, double x0z) { return make_frame_impl(x0x, x0y, x0z); }
+
+int frac_f32(float const* pos, int64_t n, float const* seg7, int64_t nseg, double const x0[3], int32_t const* counts,
+             float* out, hipStream_t stream)
+{
+    if (n == 0)
+        return ok;
+    frame const R = make_frame_impl(x0[0], x0[1], x0[2]);
+    dbuf s32, s64;
+    AGWN_TRY(s32.alloc(size_t(std::max<int64_t>(nseg, 1)) * sizeof(seg_f32), stream));
+    AGWN_TRY(s64.alloc(size_t(std::max<int64_t>(nseg, 1)) * sizeof(seg_f64), stream));
+    if (nseg > 0)
+    {
+        k_prep_segments<float><<<dim3(unsigned((nseg + kBlock - 1) / kBlock)), dim3(kBlock), 0, stream>>>(
+            seg7, nseg, R, s32.as<seg_f32>(), s64.as<seg_f64>());
+        AGWN_HIP(hipGetLastError());
+    }
+    int64_t const per_block = int64_t(kBlock) * kQ;
+    int64_t const blocks = (n + per_block - 1) / per_block;
+    AGWN_CHECK(blocks < (int64_t(1) << 31), "frac_f32: too many queries (%lld)", (long long)n);
+    k_frac_f32<kQ><<<dim3(unsigned(blocks)), dim3(kBlock), 0, stream>>>(pos, n, s32.as<seg_f32>(), s64.as<seg_f64>(),
+                                                                      nseg, R, kTau, counts, out);
+    AGWN_HIP(hipGetLastError());
+    return ok;
+}
+
+int frac_f64(double const* pos, int64_t n, double const* seg7, int64_t nseg, double const x0[3],
+             int32_t const* counts, double* out, hipStream_t stream)
+{
+    if (n == 0)
+        return ok;
+    frame const R = make_frame_impl(x0[0], x0[1], x0[2]);
+    dbuf s64;
+    AGWN_TRY(s64.alloc(size_t(std::max<int64_t>(nseg, 1)) * sizeof(seg_f64), stream));
+    if (nseg > 0)
+    {
+        k_prep_segments<double><<<dim3(unsigned((nseg + kBlock - 1) / kBlock)), dim3(kBlock), 0, stream>>>(
+            seg7, nseg, R, nullptr, s64.as<seg_f64>());
+        AGWN_HIP(hipGetLastError());
+    }
+    int64_t const blocks = (n + kBlock - 1) / kBlock;
+    AGWN_CHECK(blocks < (int64_t(1) << 31), "frac_f64: too many queries (%lld)", (long long)n);
+    k_frac_f64<<<dim3(unsigned(blocks)), dim3(kBlock), 0, stream>>>(pos, n, s64.as<seg_f64>(), nseg, R, counts, out);
+    AGWN_HIP(hipGetLastError());
+    return ok;
+}
+} // namespace agwn

@@ -1,0 +1,122 @@
+// Internal declarations shared by the HIP translation units of libagwn.so (not installed).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+namespace agwn
+{
+// ---------------------------------------------------------------------------------------------------------------------
+// error plumbing: every C entry point returns an int status; the message is kept per host thread.
+// ---------------------------------------------------------------------------------------------------------------------
+enum status : int
+{
+    ok = 0,
+    err_invalid = 1,
+    err_hip = 2,
+    err_nodevice = 3,
+    err_alloc = 4,
+    err_unsupported = 5,
+};
+
+void set_error(char const* fmt, ...);
+void clear_error();
+
+struct failure
+{
+    int code;
+};
+
+// Throw-free inside kernels' launchers: these helpers record the message and return the status to the caller.
+#define AGWN_HIP(call)                                                                                                 \
+    do                                                                                                                 \
+    {                                                                                                                  \
+        hipError_t const e__ = (call);                                                                                 \
+        if (e__ != hipSuccess)                                                                                         \
+        {                                                                                                              \
+            ::agwn::set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e__), __FILE__, __LINE__);              \
+            return (e__ == hipErrorOutOfMemory || e__ == hipErrorMemoryAllocation) ? ::agwn::err_alloc                 \
+                                                                                   : ::agwn::err_hip;                  \
+        }                                                                                                              \
+    } while (0)
+
+#define AGWN_CHECK(cond, ...)                                                                                          \
+    do                                                                                                                 \
+    {                                                                                                                  \
+        if (!(cond))                                                                                                   \
+        {                                                                                                              \
+            ::agwn::set_error(__VA_ARGS__);                                                                            \
+            return ::agwn::err_invalid;                                                                                \
+        }                                                                                                              \
+    } while (0)
+
+#define AGWN_TRY(expr)                                                                                                 \
+    do                                                                                                                 \
+    {                                                                                                                  \
+        int const s__ = (expr);                                                                                        \
+        if (s__ != ::agwn::ok)                                                                                         \
+            return s__;                                                                                                \
+    } while (0)
+
+// Stream-ordered device buffer (hipMallocAsync / hipFreeAsync on the owning stream).
+struct dbuf
+{
+    void* p = nullptr;
+    hipStream_t s = nullptr;
+    dbuf() = default;
+    dbuf(dbuf const&) = delete;
+    dbuf& operator=(dbuf const&) = delete;
+    ~dbuf()
+    {
+        if (p)
+            (void)hipFreeAsync(p, s);
+    }
+    int alloc(size_t bytes, hipStream_t stream)
+    {
+        s = stream;
+        if (bytes == 0)
+            bytes = 256;
+        AGWN_HIP(hipMallocAsync(&p, bytes, stream));
+        return ok;
+    }
+    template <class U>
+    U* as() const
+    {
+        return static_cast<U*>(p);
+    }
+};
+
+int ensure_device(); // selects nothing; checks a device is present (cached)
+
+// ---------------------------------------------------------------------------------------------------------------------
+// Rotated frame of the fractional term: row 0 is x1 = -x0 (the antipodal pole), rows 1, 2 complete a right-handed
+// orthonormal frame. In that frame x1 . v = v.x and x1 . (v0 x v1) = v0.y v1.z - v0.z v1.y.
+// ---------------------------------------------------------------------------------------------------------------------
+struct frame
+{
+    double r[9];
+};
+frame make_pole_frame(double x0x, double x0y, double x0z);
+
+// Packed, rotated boundary segment: endpoints a, b in the pole frame, weight w. 32 B (one scalar dwordx8 load).
+struct alignas(32) seg_f32
+{
+    float ax, ay, az, bx, by, bz, w, pad;
+};
+// The same segment in double, for the FP64 path and the FP32 kernel's ill-conditioned-pair fallback. 64 B.
+struct alignas(64) seg_f64
+{
+    double ax, ay, az, bx, by, bz, w, pad;
+};
+
+// Fractional launchers (agwn_frac.hip). pos/seg7/out/counts are device pointers. counts may be null (fractional
+// only); otherwise out[i] = fractional + counts[i] (the full GWN of gwn_mesh.hh:213-218).
+int frac_f32(float const* pos, int64_t n, float const* seg7, int64_t nseg, double const x0[3], int32_t const* counts,
+             float* out, hipStream_t stream);
+int frac_f64(double const* pos, int64_t n, double const* seg7, int64_t nseg, double const x0[3], int32_t const* counts,
+             double* out, hipStream_t stream);
+} // namespace agwn

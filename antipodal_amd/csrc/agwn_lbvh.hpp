@@ -1,0 +1,44 @@
+// LBVH over triangles (shared by the plain and the robust integer terms).
+//
+// Build (once per mesh): Morton codes of the triangle centroids -> LSD radix sort (8-bit digits; per-tile LDS
+// histograms, stable in-tile ranking with 64-lane ballots) -> Karras (2012) binary radix tree over the sorted codes
+// (ties broken by sorted position) -> tree depth. Refit (per box set): bottom-up with one atomic arrival counter per
+// internal node; the second child to arrive writes the parent box.
+//
+// Node encoding: internal nodes 0..n-2 (root 0), leaves 0..n-1 in Morton order; a child index c >= 0 is internal
+// node c, c < 0 is leaf ~c.
+#pragma once
+
+#include "agwn_internal.hpp"
+
+namespace agwn
+{
+struct lbvh_topology
+{
+    int64_t n = 0;          // triangles (leaves)
+    int32_t* order = nullptr;   // [n] sorted position -> original triangle index
+    int32_t* child = nullptr;   // [2*(n-1)] children of internal nodes
+    int32_t* parent = nullptr;  // [n-1 + n]: parent of internal i at [i], of leaf k at [n-1+k]; root's parent -1
+    int max_depth = 0;          // edges from the root to the deepest leaf
+};
+
+// Device-owned topology (freed with hipFree by release()).
+int lbvh_build_topology(float const* centroids, int64_t n, lbvh_topology& topo, hipStream_t s);
+void lbvh_release(lbvh_topology& topo);
+
+// Traversal node: the two child boxes and child indices. B = float or double box coordinates.
+template <class B>
+struct alignas(sizeof(B) == 4 ? 64 : 128) bvh_node
+{
+    B lo0[3], hi0[3], lo1[3], hi1[3];
+    int32_t c0, c1;
+};
+
+// Bottom-up refit. leaf_lo/leaf_hi: [n][3] per sorted leaf (an empty box has lo > hi). Writes nodes[n-1].
+// scratch_flags: n-1 ints (zeroed here).
+template <class B>
+int lbvh_refit(lbvh_topology const& topo, B const* leaf_lo, B const* leaf_hi, bvh_node<B>* nodes, B* node_lo,
+               B* node_hi, int32_t* scratch_flags, hipStream_t s);
+
+constexpr int kMaxStack = 64;
+} // namespace agwn

@@ -1,0 +1,689 @@
+// Prepared mesh (agwn_mesh) and the plain integer term: signed ray-crossing counts along an arbitrary direction,
+// bit-identical to NaiveIntersector<T>::signed_intersection_count (intersector/intersector.hh:61-100).
+//
+// Exactness argument. The reference tests every triangle with Moeller-Trumbore in T. Here the per-direction part of
+// that test (e1 = b - a, e2 = c - a, h = d x e2, det = e1 . h, inv = 1/det, sign(d . (e1 x e2))) is computed once per
+// call per triangle with the same operations (no contraction, IEEE division), and the per-query part (s = p - a,
+// u, q = s x e1, v, t) per candidate with the same operations -- so every tested triangle gives the reference's
+// verdict bit for bit. The LBVH only decides which triangles are tested. It must never cull a triangle the FP test
+// would accept: each leaf box is the triangle's AABB grown by pad = 32 eps_T (max|p| + max|v|) cond, with
+// cond = |e1||e2||d| / |det|, which bounds how far from the triangle the rounded test can accept a ray (the rounding of
+// s, of the three dot/cross products and of the multiplications by inv, each O(eps |s| |e| / |det|) in barycentric
+// units), rounded outward to float; the float slab test is made conservative by scaling t_far by 1 + 2 gamma_3 (Ize,
+// "Robust BVH Ray Traversal", JCGT 2013). Triangles with det == 0 are never accepted by the reference and are dropped;
+// triangles with cond above 2^13 (f32) / 2^40 (f64) -- nearly parallel to d -- would need boxes so large that they
+// are tested exhaustively by every query instead (the "brute" list).
+
+#include "../../include/agwn.h"
+#include "agwn_lbvh.hpp"
+
+#include <algorithm>
+#include <cfloat>
+#include <cstring>
+#include <memory>
+#include <mutex>
+
+namespace agwn
+{
+namespace
+{
+constexpr int kB = 256;
+constexpr int kTB = 128; // traversal block: 128 lanes x 64-entry LDS stack x 4 B = 32 KiB
+
+template <class T>
+struct alignas(sizeof(T) == 4 ? 64 : 128) leaf_rec
+{
+    T ax, ay, az, e1x, e1y, e1z, e2x, e2y, e2z, hx, hy, hz, inv;
+    int32_t sign;
+    int32_t live; // 1: tested through the BVH; 0: never hit (det == 0) or in the brute list
+};
+
+template <class T>
+__global__ void __launch_bounds__(kB) k_centroids(T const* __restrict__ v, int32_t const* __restrict__ idx, int64_t ntri,
+                                                  float* __restrict__ c)
+{
+    int64_t const t = int64_t(blockIdx.x) * kB + threadIdx.x;
+    if (t >= ntri)
+        return;
+    for (int k = 0; k < 3; ++k)
+    {
+        double const s = double(v[3 * int64_t(idx[3 * t]) + k]) + double(v[3 * int64_t(idx[3 * t + 1]) + k]) +
+                         double(v[3 * int64_t(idx[3 * t + 2]) + k]);
+        c[3 * t + k] = float(s / 3.0);
+    }
+}
+
+// sorted triangle vertices: tri[9*k..] = a, b, c of the triangle at sorted position k
+template <class T>
+__global__ void __launch_bounds__(kB) k_gather_tris(T const* __restrict__ v, int32_t const* __restrict__ idx,
+                                                    int32_t const* __restrict__ order, int64_t ntri, T* __restrict__ tri)
+{
+    int64_t const k = int64_t(blockIdx.x) * kB + threadIdx.x;
+    if (k >= ntri)
+        return;
+    int64_t const t = order[k];
+    for (int j = 0; j < 3; ++j)
+        for (int c = 0; c < 3; ++c)
+            tri[9 * k + 3 * j + c] = v[3 * int64_t(idx[3 * t + j]) + c];
+}
+
+// max over queries of |p|_2 (as double bits via atomicMax on the non-negative encoding)
+template <class T>
+__global__ void __launch_bounds__(kB) k_max_norm(T const* __restrict__ p, int64_t n, unsigned long long* __restrict__ out)
+{
+    __shared__ double red[kB];
+    double m = 0.0;
+    for (int64_t i = int64_t(blockIdx.x) * kB + threadIdx.x; i < n; i += int64_t(gridDim.x) * kB)
+    {
+        double const x = double(p[3 * i]), y = double(p[3 * i + 1]), z = double(p[3 * i + 2]);
+        m = fmax(m, sqrt(x * x + y * y + z * z));
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = kB / 2; w > 0; w >>= 1)
+    {
+        if (int(threadIdx.x) < w)
+            red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        atomicMax(out, (unsigned long long)__double_as_longlong(red[0]));
+}
+
+__device__ __forceinline__ float round_down(double x)
+{
+    float f = float(x);
+    if (double(f) > x)
+        f = nextafterf(f, -INFINITY);
+    return f;
+}
+__device__ __forceinline__ float round_up(double x)
+{
+    float f = float(x);
+    if (double(f) < x)
+        f = nextafterf(f, INFINITY);
+    return f;
+}
+
+// Per-call, per-direction leaf preparation.
+template <class T>
+__global__ void __launch_bounds__(kB)
+    k_prep_leaves(T const* __restrict__ tri, int64_t ntri, T dx, T dy, T dz, double dsum, double eps, double float_pad,
+                  double cond_max, int force_brute, leaf_rec<T>* __restrict__ rec, float* __restrict__ lo,
+                  float* __restrict__ hi, int32_t* __restrict__ brute, int32_t* __restrict__ nbrute)
+{
+#pragma clang fp contract(off)
+    int64_t const k = int64_t(blockIdx.x) * kB + threadIdx.x;
+    if (k >= ntri)
+        return;
+    T const* g = tri + 9 * k;
+    T const ax = g[0], ay = g[1], az = g[2];
+    T const bx = g[3], by = g[4], bz = g[5];
+    T const cx = g[6], cy = g[7], cz = g[8];
+    // intersector.hh:73-80, same operations in the same order
+    T const e1x = bx - ax, e1y = by - ay, e1z = bz - az;
+    T const e2x = cx - ax, e2y = cy - ay, e2z = cz - az;
+    T const hx = dy * e2z - dz * e2y;
+    T const hy = dz * e2x - dx * e2z;
+    T const hz = dx * e2y - dy * e2x;
+    T const det = e1x * hx + e1y * hy + e1z * hz;
+    // intersector.hh:95-97: sign of (e1 x e2) . dir, ties to -1
+    T const nx = e1y * e2z - e1z * e2y;
+    T const ny = e1z * e2x - e1x * e2z;
+    T const nz = e1x * e2y - e1y * e2x;
+    T const nd = nx * dx + ny * dy + nz * dz;
+    leaf_rec<T> r;
+    r.ax = ax;
+    r.ay = ay;
+    r.az = az;
+    r.e1x = e1x;
+    r.e1y = e1y;
+    r.e1z = e1z;
+    r.e2x = e2x;
+    r.e2y = e2y;
+    r.e2z = e2z;
+    r.hx = hx;
+    r.hy = hy;
+    r.hz = hz;
+    r.inv = det != T(0) ? T(1) / det : T(0);
+    r.sign = nd > T(0) ? 1 : -1;
+    r.live = 0;
+    float l[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, h[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX}; // empty
+    if (det != T(0))
+    {
+        double const n1 = sqrt(double(e1x) * e1x + double(e1y) * e1y + double(e1z) * e1z);
+        double const n2 = sqrt(double(e2x) * e2x + double(e2y) * e2y + double(e2z) * e2z);
+        double const nd_ = sqrt(double(dx) * dx + double(dy) * dy + double(dz) * dz);
+        double const cond = n1 * n2 * nd_ / fabs(double(det));
+        if (cond > cond_max || force_brute)
+        {
+            brute[atomicAdd(nbrute, 1)] = int32_t(k);
+        }
+        else
+        {
+            r.live = 1;
+            double const pad = 32.0 * eps * dsum * cond + float_pad;
+            double const vx[3] = {double(ax), double(bx), double(cx)};
+            double const vy[3] = {double(ay), double(by), double(cy)};
+            double const vz[3] = {double(az), double(bz), double(cz)};
+            double const mn[3] = {fmin(fmin(vx[0], vx[1]), vx[2]), fmin(fmin(vy[0], vy[1]), vy[2]),
+                                  fmin(fmin(vz[0], vz[1]), vz[2])};
+            double const mx[3] = {fmax(fmax(vx[0], vx[1]), vx[2]), fmax(fmax(vy[0], vy[1]), vy[2]),
+                                  fmax(fmax(vz[0], vz[1]), vz[2])};
+            for (int a = 0; a < 3; ++a)
+            {
+                l[a] = round_down(mn[a] - pad);
+                h[a] = round_up(mx[a] + pad);
+            }
+        }
+    }
+    rec[k] = r;
+    for (int a = 0; a < 3; ++a)
+    {
+        lo[3 * k + a] = l[a];
+        hi[3 * k + a] = h[a];
+    }
+}
+
+// Moeller-Trumbore, query part (intersector.hh:81-97), reference operation order, no contraction.
+template <class T>
+__device__ __forceinline__ int mt_test(leaf_rec<T> const& r, T px, T py, T pz, T dx, T dy, T dz)
+{
+#pragma clang fp contract(off)
+    T const sx = px - r.ax, sy = py - r.ay, sz = pz - r.az;
+    T const u = r.inv * (sx * r.hx + sy * r.hy + sz * r.hz);
+    if (u < T(0) || u > T(1))
+        return 0;
+    T const qx = sy * r.e1z - sz * r.e1y;
+    T const qy = sz * r.e1x - sx * r.e1z;
+    T const qz = sx * r.e1y - sy * r.e1x;
+    T const v = r.inv * (dx * qx + dy * qy + dz * qz);
+    if (v < T(0) || u + v > T(1))
+        return 0;
+    T const t = r.inv * (r.e2x * qx + r.e2y * qy + r.e2z * qz);
+    if (t < T(0))
+        return 0;
+    return r.sign;
+}
+
+// conservative float slab test of the ray (o, 1/d) against [lo, hi] for t >= 0
+__device__ __forceinline__ bool slab(float const* lo, float const* hi, float ox, float oy, float oz, float ix, float iy,
+                                     float iz)
+{
+    if (!(lo[0] <= hi[0]))
+        return false; // empty box
+    float const tx0 = (lo[0] - ox) * ix, tx1 = (hi[0] - ox) * ix;
+    float const ty0 = (lo[1] - oy) * iy, ty1 = (hi[1] - oy) * iy;
+    float const tz0 = (lo[2] - oz) * iz, tz1 = (hi[2] - oz) * iz;
+    float const tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
+    float const tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1));
+    return tn <= tf * 1.0000004f; // 1 + 2 gamma_3, gamma_3 = 3u/(1-3u), u = 2^-24
+}
+
+__device__ __forceinline__ float inv_dir(double d)
+{
+    float const f = float(d);
+    return f != 0.0f ? 1.0f / f : (signbit(f) ? -1e30f : 1e30f);
+}
+
+template <class T>
+__global__ void __launch_bounds__(kTB)
+    k_count(T const* __restrict__ pos, int64_t n, T dx, T dy, T dz, bvh_node<float> const* __restrict__ nodes,
+            leaf_rec<T> const* __restrict__ rec, int32_t const* __restrict__ brute, int32_t const* __restrict__ nbrute_p,
+            int use_bvh, int32_t* __restrict__ out)
+{
+    __shared__ int32_t stack[kMaxStack][kTB];
+    int64_t const i = int64_t(blockIdx.x) * kTB + threadIdx.x;
+    if (i >= n)
+        return;
+    T const px = pos[3 * i], py = pos[3 * i + 1], pz = pos[3 * i + 2];
+    int sum = 0;
+    if (use_bvh)
+    {
+        float const ox = float(px), oy = float(py), oz = float(pz);
+        float const ix = inv_dir(double(dx)), iy = inv_dir(double(dy)), iz = inv_dir(double(dz));
+        int sp = 0;
+        int32_t node = 0;
+        while (true)
+        {
+            bvh_node<float> const nd = nodes[node];
+            bool const h0 = slab(nd.lo0, nd.hi0, ox, oy, oz, ix, iy, iz);
+            bool const h1 = slab(nd.lo1, nd.hi1, ox, oy, oz, ix, iy, iz);
+            int32_t next = -1;
+            bool have = false;
+            if (h0)
+            {
+                if (nd.c0 < 0)
+                {
+                    leaf_rec<T> const r = rec[~nd.c0];
+                    if (r.live)
+                        sum += mt_test(r, px, py, pz, dx, dy, dz);
+                }
+                else
+                {
+                    next = nd.c0;
+                    have = true;
+                }
+            }
+            if (h1)
+            {
+                if (nd.c1 < 0)
+                {
+                    leaf_rec<T> const r = rec[~nd.c1];
+                    if (r.live)
+                        sum += mt_test(r, px, py, pz, dx, dy, dz);
+                }
+                else if (have)
+                {
+                    if (sp < kMaxStack) // depth <= kMaxStack - 2 is checked at build
+                        stack[sp++][threadIdx.x] = nd.c1;
+                }
+                else
+                {
+                    next = nd.c1;
+                    have = true;
+                }
+            }
+            if (have)
+            {
+                node = next;
+            }
+            else
+            {
+                if (sp == 0)
+                    break;
+                node = stack[--sp][threadIdx.x];
+            }
+        }
+    }
+    int32_t const nb = *nbrute_p;
+    for (int32_t b = 0; b < nb; ++b)
+        sum += mt_test(rec[brute[b]], px, py, pz, dx, dy, dz);
+    out[i] = sum;
+}
+
+// Per-direction prepared state, cached on the mesh (rebuilt when the direction changes or the queries reach farther).
+template <class T>
+struct dir_state
+{
+    T d[3] = {0, 0, 0};
+    double dsum = -1.0;
+    leaf_rec<T>* rec = nullptr;
+    float* lo = nullptr;
+    float* hi = nullptr;
+    float* nlo = nullptr;
+    float* nhi = nullptr;
+    bvh_node<float>* nodes = nullptr;
+    int32_t* flags = nullptr;
+    int32_t* brute = nullptr;
+    int32_t* nbrute = nullptr;
+    ~dir_state()
+    {
+        for (void* p : {(void*)rec, (void*)lo, (void*)hi, (void*)nlo, (void*)nhi, (void*)nodes, (void*)flags,
+                        (void*)brute, (void*)nbrute})
+            if (p)
+                (void)hipFree(p);
+    }
+};
+} // namespace
+
+// ---------------------------------------------------------------------------------------------------------------------
+// the mesh handle
+// ---------------------------------------------------------------------------------------------------------------------
+struct mesh_impl
+{
+    virtual ~mesh_impl() = default;
+    virtual int scalar_bytes() const = 0;
+    int64_t ntri = 0;
+    int64_t nv = 0;
+    lbvh_topology topo;
+};
+
+template <class T>
+struct mesh_t final : mesh_impl
+{
+    T* tri = nullptr; // sorted triangle vertices, 9 per triangle
+    double vmax = 0.0; // max |v|_2 over the vertices
+    std::mutex mu;
+    std::shared_ptr<dir_state<T>> cache;
+
+    int scalar_bytes() const override { return int(sizeof(T)); }
+    ~mesh_t() override
+    {
+        cache.reset();
+        if (tri)
+            (void)hipFree(tri);
+        lbvh_release(topo);
+    }
+
+    int create(T const* hv, int64_t nverts, int32_t const* hidx, int64_t nidx)
+    {
+        nv = nverts;
+        ntri = nidx / 3;
+        for (int64_t i = 0; i < nverts; ++i)
+        {
+            double const x = double(hv[3 * i]), y = double(hv[3 * i + 1]), z = double(hv[3 * i + 2]);
+            vmax = std::max(vmax, std::sqrt(x * x + y * y + z * z));
+        }
+        if (ntri == 0)
+            return ok;
+        hipStream_t s = nullptr;
+        AGWN_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        struct sguard
+        {
+            hipStream_t s;
+            ~sguard() { (void)hipStreamDestroy(s); }
+        } sg{s};
+        dbuf dv, di, dc;
+        AGWN_TRY(dv.alloc(size_t(std::max<int64_t>(nverts, 1)) * 3 * sizeof(T), s));
+        AGWN_TRY(di.alloc(size_t(nidx) * sizeof(int32_t), s));
+        AGWN_TRY(dc.alloc(size_t(ntri) * 3 * sizeof(float), s));
+        AGWN_HIP(hipMemcpyAsync(dv.p, hv, size_t(nverts) * 3 * sizeof(T), hipMemcpyHostToDevice, s));
+        AGWN_HIP(hipMemcpyAsync(di.p, hidx, size_t(nidx) * sizeof(int32_t), hipMemcpyHostToDevice, s));
+        unsigned const g = unsigned((ntri + kB - 1) / kB);
+        k_centroids<T><<<dim3(g), dim3(kB), 0, s>>>(dv.as<T>(), di.as<int32_t>(), ntri, dc.as<float>());
+        AGWN_HIP(hipGetLastError());
+        AGWN_TRY(lbvh_build_topology(dc.as<float>(), ntri, topo, s));
+        AGWN_HIP(hipMalloc(&tri, size_t(ntri) * 9 * sizeof(T)));
+        k_gather_tris<T><<<dim3(g), dim3(kB), 0, s>>>(dv.as<T>(), di.as<int32_t>(), topo.order, ntri, tri);
+        AGWN_HIP(hipGetLastError());
+        AGWN_HIP(hipStreamSynchronize(s));
+        return ok;
+    }
+
+    // Returns the prepared state for direction d valid for queries with |p| <= qmax.
+    int prepare(T const d[3], double qmax, hipStream_t s, std::shared_ptr<dir_state<T>>& out)
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        double const dsum = qmax + vmax;
+        if (cache && cache->d[0] == d[0] && cache->d[1] == d[1] && cache->d[2] == d[2] && dsum <= cache->dsum)
+        {
+            out = cache;
+            return ok;
+        }
+        auto st = std::make_shared<dir_state<T>>();
+        st->d[0] = d[0];
+        st->d[1] = d[1];
+        st->d[2] = d[2];
+        // grow the valid range a little so nearby query sets reuse the state
+        st->dsum = dsum * 1.25 + 1e-30;
+        size_t const nt = size_t(ntri);
+        AGWN_HIP(hipMalloc(&st->rec, nt * sizeof(leaf_rec<T>)));
+        AGWN_HIP(hipMalloc(&st->lo, nt * 3 * sizeof(float)));
+        AGWN_HIP(hipMalloc(&st->hi, nt * 3 * sizeof(float)));
+        AGWN_HIP(hipMalloc(&st->nlo, std::max<size_t>(nt, 2) * 3 * sizeof(float)));
+        AGWN_HIP(hipMalloc(&st->nhi, std::max<size_t>(nt, 2) * 3 * sizeof(float)));
+        AGWN_HIP(hipMalloc(&st->nodes, std::max<size_t>(nt, 2) * sizeof(bvh_node<float>)));
+        AGWN_HIP(hipMalloc(&st->flags, std::max<size_t>(nt, 2) * sizeof(int32_t)));
+        AGWN_HIP(hipMalloc(&st->brute, std::max<size_t>(nt, 1) * sizeof(int32_t)));
+        AGWN_HIP(hipMalloc(&st->nbrute, sizeof(int32_t)));
+        AGWN_HIP(hipMemsetAsync(st->nbrute, 0, sizeof(int32_t), s));
+        double const eps = sizeof(T) == 4 ? 0x1p-24 : 0x1p-53;
+        double const float_pad = sizeof(T) == 4 ? 0.0 : 4.0 * 0x1p-24 * st->dsum; // float(p) of f64 queries
+        double const cond_max = sizeof(T) == 4 ? 0x1p13 : 0x1p40;
+        unsigned const g = unsigned((ntri + kB - 1) / kB);
+        // a single triangle has no internal node: it is tested exhaustively (unless det == 0: never hit)
+        k_prep_leaves<T><<<dim3(g), dim3(kB), 0, s>>>(tri, ntri, d[0], d[1], d[2], st->dsum, eps, float_pad,
+                                                      cond_max, ntri == 1 ? 1 : 0, st->rec, st->lo, st->hi, st->brute,
+                                                      st->nbrute);
+        AGWN_HIP(hipGetLastError());
+        if (ntri > 1)
+        {
+            AGWN_TRY(lbvh_refit<float>(topo, st->lo, st->hi, st->nodes, st->nlo, st->nhi, st->flags, s));
+        }
+        AGWN_HIP(hipStreamSynchronize(s));
+        cache = st;
+        out = st;
+        return ok;
+    }
+
+    int count(T const* pos, int64_t n, T const d[3], int32_t* out, hipStream_t s)
+    {
+        if (n == 0)
+            return ok;
+        if (ntri == 0)
+        {
+            AGWN_HIP(hipMemsetAsync(out, 0, size_t(n) * sizeof(int32_t), s));
+            return ok;
+        }
+        dbuf qm;
+        AGWN_TRY(qm.alloc(sizeof(unsigned long long), s));
+        AGWN_HIP(hipMemsetAsync(qm.p, 0, sizeof(unsigned long long), s));
+        unsigned const g = unsigned(std::min<int64_t>((n + kB - 1) / kB, 2048));
+        k_max_norm<T><<<dim3(g), dim3(kB), 0, s>>>(pos, n, qm.as<unsigned long long>());
+        AGWN_HIP(hipGetLastError());
+        unsigned long long bits = 0;
+        AGWN_HIP(hipMemcpyAsync(&bits, qm.p, sizeof(bits), hipMemcpyDeviceToHost, s));
+        AGWN_HIP(hipStreamSynchronize(s));
+        double qmax;
+        std::memcpy(&qmax, &bits, sizeof(qmax));
+        std::shared_ptr<dir_state<T>> st;
+        AGWN_TRY(prepare(d, qmax, s, st));
+        int64_t const blocks = (n + kTB - 1) / kTB;
+        AGWN_CHECK(blocks < (int64_t(1) << 31), "too many queries (%lld)", (long long)n);
+        k_count<T><<<dim3(unsigned(blocks)), dim3(kTB), 0, s>>>(pos, n, d[0], d[1], d[2], st->nodes, st->rec,
+                                                                st->brute, st->nbrute, ntri > 1 ? 1 : 0, out);
+        AGWN_HIP(hipGetLastError());
+        // keep the state alive until the kernel has consumed it (it may be replaced by another thread's call)
+        AGWN_HIP(hipStreamSynchronize(s));
+        return ok;
+    }
+};
+} // namespace agwn
+
+struct agwn_mesh
+{
+    std::unique_ptr<agwn::mesh_impl> impl;
+};
+
+namespace agwn
+{
+namespace
+{
+struct call_stream2
+{
+    hipStream_t s = nullptr;
+    ~call_stream2()
+    {
+        if (s)
+            (void)hipStreamDestroy(s);
+    }
+};
+
+template <class F>
+int guarded2(F&& f)
+{
+    clear_error();
+    try
+    {
+        return f();
+    }
+    catch (std::bad_alloc const&)
+    {
+        set_error("host allocation failed");
+        return err_alloc;
+    }
+    catch (std::exception const& e)
+    {
+        set_error("%s", e.what());
+        return err_invalid;
+    }
+}
+
+template <class T>
+int mesh_create(T const* verts, int64_t nv, int32_t const* idx, int64_t nidx, agwn_mesh** out)
+{
+    AGWN_CHECK(out != nullptr, "out is null");
+    *out = nullptr;
+    AGWN_CHECK(nv >= 0 && nidx >= 0, "negative count");
+    AGWN_CHECK(nidx % 3 == 0, "indices.size() = %lld is not a multiple of 3", (long long)nidx);
+    AGWN_CHECK(nidx == 0 || (verts && idx), "null mesh arrays");
+    for (int64_t i = 0; i < nidx; ++i)
+        AGWN_CHECK(idx[i] >= 0 && idx[i] < nv, "index %lld = %d out of range [0, %lld)", (long long)i, idx[i],
+                   (long long)nv);
+    AGWN_TRY(ensure_device());
+    auto m = std::make_unique<mesh_t<T>>();
+    AGWN_TRY(m->create(verts, nv, idx, nidx));
+    auto* h = new agwn_mesh;
+    h->impl = std::move(m);
+    *out = h;
+    return ok;
+}
+
+template <class T>
+mesh_t<T>* typed(agwn_mesh const* m)
+{
+    if (!m || !m->impl || m->impl->scalar_bytes() != int(sizeof(T)))
+        return nullptr;
+    return static_cast<mesh_t<T>*>(m->impl.get());
+}
+
+template <class T>
+int int_dev(agwn_mesh const* m, T const* pos, int64_t n, T const* dir, int32_t* out, hipStream_t s)
+{
+    mesh_t<T>* mt = typed<T>(m);
+    AGWN_CHECK(mt != nullptr, "mesh handle is null or of the other scalar type");
+    AGWN_CHECK(n >= 0, "negative count");
+    AGWN_CHECK(dir != nullptr, "direction is null");
+    AGWN_CHECK(n == 0 || (pos && out), "positions/out are null with n=%lld", (long long)n);
+    T const d[3] = {dir[0], dir[1], dir[2]};
+    return mt->count(pos, n, d, out, s);
+}
+
+template <class T>
+int full_dev(agwn_mesh const* m, T const* pos, int64_t n, T const* x0, T const* seg7, int64_t nseg, T* out,
+             hipStream_t s)
+{
+    mesh_t<T>* mt = typed<T>(m);
+    AGWN_CHECK(mt != nullptr, "mesh handle is null or of the other scalar type");
+    AGWN_CHECK(n >= 0 && nseg >= 0, "negative count");
+    AGWN_CHECK(x0 != nullptr, "x0 is null");
+    AGWN_CHECK(n == 0 || (pos && out), "positions/out are null with n=%lld", (long long)n);
+    AGWN_CHECK(nseg == 0 || seg7, "boundary is null with nseg=%lld", (long long)nseg);
+    if (n == 0)
+        return ok;
+    dbuf cnt;
+    AGWN_TRY(cnt.alloc(size_t(n) * sizeof(int32_t), s));
+    T const d[3] = {x0[0], x0[1], x0[2]};
+    AGWN_TRY(mt->count(pos, n, d, cnt.as<int32_t>(), s));
+    double const x[3] = {double(x0[0]), double(x0[1]), double(x0[2])};
+    if constexpr (sizeof(T) == 4)
+        return frac_f32(pos, n, seg7, nseg, x, cnt.as<int32_t>(), out, s);
+    else
+        return frac_f64(pos, n, seg7, nseg, x, cnt.as<int32_t>(), out, s);
+}
+
+template <class T>
+int int_host(agwn_mesh const* m, T const* pos, int64_t n, T const* dir, int32_t* out)
+{
+    AGWN_CHECK(typed<T>(m) != nullptr, "mesh handle is null or of the other scalar type");
+    AGWN_CHECK(n >= 0, "negative count");
+    AGWN_CHECK(n == 0 || (pos && out), "positions/out are null with n=%lld", (long long)n);
+    if (n == 0)
+        return ok;
+    call_stream2 cs;
+    AGWN_HIP(hipStreamCreateWithFlags(&cs.s, hipStreamNonBlocking));
+    dbuf dp, dout;
+    AGWN_TRY(dp.alloc(size_t(n) * 3 * sizeof(T), cs.s));
+    AGWN_TRY(dout.alloc(size_t(n) * sizeof(int32_t), cs.s));
+    AGWN_HIP(hipMemcpyAsync(dp.p, pos, size_t(n) * 3 * sizeof(T), hipMemcpyHostToDevice, cs.s));
+    AGWN_TRY(int_dev<T>(m, dp.as<T>(), n, dir, dout.as<int32_t>(), cs.s));
+    AGWN_HIP(hipMemcpyAsync(out, dout.p, size_t(n) * sizeof(int32_t), hipMemcpyDeviceToHost, cs.s));
+    AGWN_HIP(hipStreamSynchronize(cs.s));
+    return ok;
+}
+
+template <class T>
+int full_host(agwn_mesh const* m, T const* pos, int64_t n, T const* x0, T const* seg7, int64_t nseg, T* out)
+{
+    AGWN_CHECK(typed<T>(m) != nullptr, "mesh handle is null or of the other scalar type");
+    AGWN_CHECK(n >= 0 && nseg >= 0, "negative count");
+    AGWN_CHECK(n == 0 || (pos && out), "positions/out are null with n=%lld", (long long)n);
+    AGWN_CHECK(nseg == 0 || seg7, "boundary is null with nseg=%lld", (long long)nseg);
+    if (n == 0)
+        return ok;
+    call_stream2 cs;
+    AGWN_HIP(hipStreamCreateWithFlags(&cs.s, hipStreamNonBlocking));
+    dbuf dp, ds, dout;
+    AGWN_TRY(dp.alloc(size_t(n) * 3 * sizeof(T), cs.s));
+    AGWN_TRY(ds.alloc(size_t(std::max<int64_t>(nseg, 1)) * 7 * sizeof(T), cs.s));
+    AGWN_TRY(dout.alloc(size_t(n) * sizeof(T), cs.s));
+    AGWN_HIP(hipMemcpyAsync(dp.p, pos, size_t(n) * 3 * sizeof(T), hipMemcpyHostToDevice, cs.s));
+    if (nseg > 0)
+        AGWN_HIP(hipMemcpyAsync(ds.p, seg7, size_t(nseg) * 7 * sizeof(T), hipMemcpyHostToDevice, cs.s));
+    AGWN_TRY(full_dev<T>(m, dp.as<T>(), n, x0, nseg ? ds.as<T>() : nullptr, nseg, dout.as<T>(), cs.s));
+    AGWN_HIP(hipMemcpyAsync(out, dout.p, size_t(n) * sizeof(T), hipMemcpyDeviceToHost, cs.s));
+    AGWN_HIP(hipStreamSynchronize(cs.s));
+    return ok;
+}
+} // namespace
+} // namespace agwn
+
+using namespace agwn;
+
+extern "C" {
+int agwn_mesh_create_f32(float const* v, int64_t nv, int32_t const* idx, int64_t nidx, agwn_mesh** out)
+{
+    return guarded2([&] { return mesh_create(v, nv, idx, nidx, out); });
+}
+int agwn_mesh_create_f64(double const* v, int64_t nv, int32_t const* idx, int64_t nidx, agwn_mesh** out)
+{
+    return guarded2([&] { return mesh_create(v, nv, idx, nidx, out); });
+}
+int agwn_mesh_destroy(agwn_mesh* m)
+{
+    clear_error();
+    delete m;
+    return ok;
+}
+int agwn_mesh_info(agwn_mesh const* m, int64_t* ntri, int64_t* nnodes, int* scalar_bytes)
+{
+    clear_error();
+    AGWN_CHECK(m && m->impl, "mesh handle is null");
+    if (ntri)
+        *ntri = m->impl->ntri;
+    if (nnodes)
+        *nnodes = m->impl->ntri > 1 ? m->impl->ntri - 1 : 0;
+    if (scalar_bytes)
+        *scalar_bytes = m->impl->scalar_bytes();
+    return ok;
+}
+int agwn_int_batch_f32(agwn_mesh const* m, float const* pos, int64_t n, float const* dir, int32_t* out)
+{
+    return guarded2([&] { return int_host(m, pos, n, dir, out); });
+}
+int agwn_int_batch_f64(agwn_mesh const* m, double const* pos, int64_t n, double const* dir, int32_t* out)
+{
+    return guarded2([&] { return int_host(m, pos, n, dir, out); });
+}
+int agwn_int_batch_f32_dev(agwn_mesh const* m, float const* pos, int64_t n, float const* dir, int32_t* out,
+                           agwn_stream s)
+{
+    return guarded2([&] { return int_dev(m, pos, n, dir, out, hipStream_t(s)); });
+}
+int agwn_int_batch_f64_dev(agwn_mesh const* m, double const* pos, int64_t n, double const* dir, int32_t* out,
+                           agwn_stream s)
+{
+    return guarded2([&] { return int_dev(m, pos, n, dir, out, hipStream_t(s)); });
+}
+int agwn_full_batch_f32(agwn_mesh const* m, float const* pos, int64_t n, float const* x0, float const* seg7,
+                        int64_t nseg, float* out)
+{
+    return guarded2([&] { return full_host(m, pos, n, x0, seg7, nseg, out); });
+}
+int agwn_full_batch_f64(agwn_mesh const* m, double const* pos, int64_t n, double const* x0, double const* seg7,
+                        int64_t nseg, double* out)
+{
+    return guarded2([&] { return full_host(m, pos, n, x0, seg7, nseg, out); });
+}
+int agwn_full_batch_f32_dev(agwn_mesh const* m, float const* pos, int64_t n, float const* x0, float const* seg7,
+                            int64_t nseg, float* out, agwn_stream s)
+{
+    return guarded2([&] { return full_dev(m, pos, n, x0, seg7, nseg, out, hipStream_t(s)); });
+}
+int agwn_full_batch_f64_dev(agwn_mesh const* m, double const* pos, int64_t n, double const* x0, double const* seg7,
+                            int64_t nseg, double* out, agwn_stream s)
+{
+    return guarded2([&] { return full_dev(m, pos, n, x0, seg7, nseg, out, hipStream_t(s)); });
+}
+} // extern "C"

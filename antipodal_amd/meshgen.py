@@ -1,0 +1,191 @@
+"""Seeded, bit-reproducible synthetic inputs for the batch-GWN workloads.
+
+Every random number comes from a counter-based SplitMix64 stream (Steele, Lea & Flood 2014: state_i = seed' + i *
+0x9E3779B97F4A7C15, output = the SplitMix64 finaliser of state_i), converted to doubles as (z >> 11) * 2^-53. Gaussians
+use Box-Muller on two such streams. The same (seed, stream) gives the same bits on every machine with this numpy, so
+CPU and GPU runs see identical inputs; FP32 inputs are float32(FP64 inputs).
+
+Workloads (sizes as named in the benchmark plan):
+  icosphere_cap_mesh()  icosphere level 3 (1280 triangles, radius 1) minus every triangle whose centroid has z > 0.6
+  holed_torus_mesh()    torus R=1, r=0.35, nu x nv quads (2 triangles each), vertices perturbed by isotropic Gaussian
+                        noise sigma = 0.01 r, then every triangle whose centroid lies within `hole_radius` of one of
+                        `holes` seeded surface points removed. Defaults (1353 x 473, 400 holes of radius 0.05, seed 1)
+                        leave 1,004,810 triangles and 31,376 unit-weight boundary segments: the 1M-triangle open mesh.
+                        (400 disks on this torus cannot give much more boundary at 1M triangles: the total rim length
+                        peaks near hole radius 0.074, at about 31k segments.)
+  grid_points()         a dense n^3 lattice over a bounding box expanded by 10 % (5 % each side)
+  uniform_points()      uniform points in a box
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+_GOLDEN = np.uint64(0x9E3779B97F4A7C15)
+_M1 = np.uint64(0xBF58476D1CE4E5B9)
+_M2 = np.uint64(0x94D049BB133111EB)
+
+
+def _mix(z: np.ndarray) -> np.ndarray:
+    z = (z ^ (z >> np.uint64(30))) * _M1
+    z = (z ^ (z >> np.uint64(27))) * _M2
+    return z ^ (z >> np.uint64(31))
+
+
+def splitmix64(seed: int, stream: int, n: int) -> np.ndarray:
+    """n uint64 outputs of the counter-based SplitMix64 stream (seed, stream)."""
+    with np.errstate(over="ignore"):
+        base = _mix(np.array([(seed * 0x9E3779B97F4A7C15 + stream * 0xD1B54A32D192ED03) & (2**64 - 1)], dtype=np.uint64))[0]
+        ctr = np.arange(1, n + 1, dtype=np.uint64)
+        return _mix(base + ctr * _GOLDEN)
+
+
+def uniform01(seed: int, stream: int, n: int) -> np.ndarray:
+    """n doubles in [0, 1): (z >> 11) * 2^-53."""
+    return (splitmix64(seed, stream, n) >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+def normal(seed: int, stream: int, n: int) -> np.ndarray:
+    """n standard normal doubles (Box-Muller, cosine branch)."""
+    u1 = 1.0 - uniform01(seed, 2 * stream + 1, n)  # (0, 1]
+    u2 = uniform01(seed, 2 * stream + 2, n)
+    return np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * math.pi * u2)
+
+
+def uniform_points(seed: int, n: int, lo, hi, stream: int = 0) -> np.ndarray:
+    lo = np.asarray(lo, dtype=np.float64)
+    hi = np.asarray(hi, dtype=np.float64)
+    u = uniform01(seed, stream, 3 * n).reshape(n, 3)
+    return lo + u * (hi - lo)
+
+
+# ---------------------------------------------------------------------------------------------------------------------
+# meshes
+# ---------------------------------------------------------------------------------------------------------------------
+def icosphere(level: int = 3, radius: float = 1.0):
+    """Icosahedron subdivided `level` times, vertices projected to the sphere; outward (CCW-from-outside) triangles."""
+    t = (1.0 + math.sqrt(5.0)) / 2.0
+    verts = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+             (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    verts = [np.array(v, dtype=np.float64) / np.linalg.norm(v) for v in verts]
+    faces = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+             (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11),
+             (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    for _ in range(level):
+        cache: dict[tuple[int, int], int] = {}
+
+        def mid(a: int, b: int) -> int:
+            k = (min(a, b), max(a, b))
+            if k not in cache:
+                m = verts[a] + verts[b]
+                verts.append(m / np.linalg.norm(m))
+                cache[k] = len(verts) - 1
+            return cache[k]
+
+        nf = []
+        for a, b, c in faces:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = nf
+    v = np.array(verts, dtype=np.float64) * radius
+    f = np.array(faces, dtype=np.int32)
+    # orient outward: the icosahedron table above is CCW from outside; verify and fix per face
+    c = v[f].mean(axis=1)
+    n = np.cross(v[f[:, 1]] - v[f[:, 0]], v[f[:, 2]] - v[f[:, 0]])
+    flip = np.einsum("ij,ij->i", n, c) < 0
+    f[flip] = f[flip][:, [0, 2, 1]]
+    return v, f.reshape(-1)
+
+
+def _compact(verts: np.ndarray, idx: np.ndarray):
+    used = np.unique(idx)
+    remap = np.full(len(verts), -1, dtype=np.int64)
+    remap[used] = np.arange(len(used))
+    return np.ascontiguousarray(verts[used]), remap[idx].astype(np.int32)
+
+
+def icosphere_cap_mesh(level: int = 3, radius: float = 1.0, zcut: float = 0.6):
+    """The small open mesh: icosphere minus the triangles whose centroid z exceeds `zcut` (one boundary loop)."""
+    v, idx = icosphere(level, radius)
+    tri = idx.reshape(-1, 3)
+    keep = v[tri].mean(axis=1)[:, 2] <= zcut
+    return _compact(v, tri[keep].reshape(-1))
+
+
+def torus_mesh(nu: int, nv: int, R: float = 1.0, r: float = 0.35):
+    i = np.arange(nu)
+    j = np.arange(nv)
+    u = 2.0 * math.pi * i / nu
+    w = 2.0 * math.pi * j / nv
+    uu, ww = np.meshgrid(u, w, indexing="ij")
+    rad = R + r * np.cos(ww)
+    verts = np.stack([rad * np.cos(uu), rad * np.sin(uu), r * np.sin(ww)], axis=-1).reshape(-1, 3)
+    ii, jj = np.meshgrid(i, j, indexing="ij")
+    a = ii * nv + jj
+    b = ((ii + 1) % nu) * nv + jj
+    c = ((ii + 1) % nu) * nv + (jj + 1) % nv
+    d = ii * nv + (jj + 1) % nv
+    tris = np.stack([np.stack([a, b, c], -1), np.stack([a, c, d], -1)], axis=2).reshape(-1, 3)
+    return verts, tris.astype(np.int32).reshape(-1)
+
+
+def holed_torus_mesh(seed: int = 1, nu: int = 1353, nv: int = 473, R: float = 1.0, r: float = 0.35,
+                     noise: float = 0.01, holes: int = 400, hole_radius: float = 0.05):
+    """Noisy torus with `holes` disk-shaped holes (triangles whose centroid is within hole_radius of a hole centre).
+
+    Hole centres are the centroids of seeded uniformly chosen triangles of the noisy torus.
+    """
+    verts, idx = torus_mesh(nu, nv, R, r)
+    nvtx = len(verts)
+    verts = verts + (noise * r) * normal(seed, 1, 3 * nvtx).reshape(nvtx, 3)
+    tri = idx.reshape(-1, 3)
+    cent = verts[tri].mean(axis=1)
+    if holes > 0:
+        pick = (uniform01(seed, 2, holes) * len(tri)).astype(np.int64)
+        centres = cent[pick]
+        from scipy.spatial import cKDTree
+
+        d, _ = cKDTree(centres).query(cent, k=1)
+        tri = tri[d > hole_radius]
+    return _compact(verts, tri.reshape(-1))
+
+
+def expanded_bbox(verts: np.ndarray, frac: float = 0.10):
+    lo = verts.min(axis=0)
+    hi = verts.max(axis=0)
+    pad = 0.5 * frac * (hi - lo)
+    return lo - pad, hi + pad
+
+
+def grid_axes(lo, hi, n: int):
+    """start, dx, dy, dz of an n^3 lattice p(i,j,k) = start + i dx + j dy + k dz spanning [lo, hi]."""
+    lo = np.asarray(lo, dtype=np.float64)
+    hi = np.asarray(hi, dtype=np.float64)
+    step = (hi - lo) / (n - 1)
+    return lo, np.array([step[0], 0, 0]), np.array([0, step[1], 0]), np.array([0, 0, step[2]])
+
+
+def grid_points(lo, hi, n: int, dtype=np.float64) -> np.ndarray:
+    """n^3 lattice points in ((k*n)+j)*n+i order (x fastest), computed in `dtype` as start + i dx + j dy + k dz."""
+    start, dx, dy, dz = (np.asarray(a, dtype=dtype) for a in grid_axes(lo, hi, n))
+    r = np.arange(n, dtype=dtype)
+    pts = np.empty((n, n, n, 3), dtype=dtype)
+    # left-to-right evaluation order of start + dx*i + dy*j + dz*k, per component
+    pts[...] = start
+    pts += (dx[None, None, None, :] * r[None, None, :, None])
+    pts += (dy[None, None, None, :] * r[None, :, None, None])
+    pts += (dz[None, None, None, :] * r[:, None, None, None])
+    return pts.reshape(-1, 3)
+
+
+EXAMPLE_X0 = (0.31, 0.57, 0.76)  # examples/main.cc:205 (normalised by the caller)
+
+
+def example_x0(dtype=np.float64) -> np.ndarray:
+    """normalize(0.31, 0.57, 0.76) evaluated in `dtype` the way the reference's normalize does (a / sqrt(dot(a,a)))."""
+    a = np.array(EXAMPLE_X0, dtype=dtype)
+    d = dtype(dtype(a[0] * a[0]) + dtype(a[1] * a[1]))
+    d = dtype(d + dtype(a[2] * a[2]))
+    l = dtype(np.sqrt(d))
+    return np.array([a[0] / l, a[1] / l, a[2] / l], dtype=dtype)

@@ -1,0 +1,116 @@
+/*
+ * agwn.h -- C ABI of the MI355X (gfx950) batch generalized-winding-number library, libagwn.so.
+ *
+ * This is the boundary between host code and the hand-written HIP kernels. Every entry point takes plain pointers
+ * and sizes, returns an int status (AGWN_OK == 0) and never throws; on failure agwn_last_error() returns a message
+ * for the calling thread. No C++ or torch types cross it. The C++20 plugin headers in include/antipodal_hip/ wrap it
+ * behind the reference's own Dispatcher / Intersector / RobustMeshGwn shapes (references below are to the
+ * MartensCedric/antipodal sources, relative to its repository root).
+ *
+ * Memory layouts are the reference's in-memory layouts, so std::span data passes zero-copy:
+ *   positions  n   x {x,y,z}                      (vec3<T>, math/common.hh:32-45)
+ *   seg7       nseg x {p0.xyz, p1.xyz, weight}    (weighted_segment3<T>, weight LAST, math/common.hh:50-65)
+ *   verts      nv  x {x,y,z};  idx  3*ntri int32  (the spans NaiveIntersector holds, intersector/intersector.hh:58-59)
+ *
+ * Entry points without a suffix take HOST pointers (inputs are copied to the device, outputs back; the call is
+ * synchronous). Entry points with the _dev suffix take DEVICE pointers and a stream (hipStream_t, NULL = default
+ * stream) and are asynchronous with respect to the host. Counts are int64; the reference casts them to int
+ * (gwn_mesh.hh:75), so any count it accepts is accepted here.
+ *
+ * Threading: handles are immutable after creation; concurrent calls on one handle from several host threads are
+ * safe (the reference requires this of intersectors, intersector/intersector.hh:28-30, and of RobustMeshGwn,
+ * gwn_mesh_robust.hh:161-162).
+ */
+#ifndef AGWN_H
+#define AGWN_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AGWN_OK 0
+#define AGWN_ERR_INVALID 1     /* null pointer, negative count, size mismatch, unknown enum */
+#define AGWN_ERR_HIP 2         /* a HIP runtime call or kernel launch failed */
+#define AGWN_ERR_NODEVICE 3    /* no usable HIP device */
+#define AGWN_ERR_ALLOC 4       /* device allocation failed */
+#define AGWN_ERR_UNSUPPORTED 5 /* argument combination not supported by this build */
+
+/* Precision policies of the robust evaluator: robust::int64_grid / int128_grid / exact_float
+ * (math/robust_predicates.hh:284-317). */
+#define AGWN_INT64_GRID 0
+#define AGWN_INT128_GRID 1
+#define AGWN_EXACT_FLOAT 2
+
+typedef void* agwn_stream; /* hipStream_t */
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* agwn_last_error(void);
+/* Library version, major*10000 + minor*100 + patch. */
+int agwn_version(void);
+/* 1 if a HIP device is usable from this process, else 0 (sets the last error). No kernel is launched. */
+int agwn_device_available(void);
+
+/* ---------------------------------------------------------------------------------------------------------------
+ * Fractional term. Replaces eval_gwnr_mesh_batch_fractional (gwn_mesh.hh:67-83) and the per-pair kernel
+ * signed_spherical_tri_area_half_unorm (math/common.hh:181-189): out[i] = sum_s w_s * area(-x0, p0_s - p_i,
+ * p1_s - p_i) / (2 pi). x0 must be unit length, as in the reference (gwn_mesh.hh:30-31).
+ * --------------------------------------------------------------------------------------------------------------- */
+int agwn_frac_batch_f32(const float* pos, int64_t n, const float* seg7, int64_t nseg, const float* x0, float* out);
+int agwn_frac_batch_f64(const double* pos, int64_t n, const double* seg7, int64_t nseg, const double* x0, double* out);
+int agwn_frac_batch_f32_dev(const float* pos, int64_t n, const float* seg7, int64_t nseg, const float* x0, float* out,
+                            agwn_stream stream);
+int agwn_frac_batch_f64_dev(const double* pos, int64_t n, const double* seg7, int64_t nseg, const double* x0,
+                            double* out, agwn_stream stream);
+
+/* ---------------------------------------------------------------------------------------------------------------
+ * Prepared triangle mesh: a GPU-built LBVH (Morton codes, LDS radix sort, Karras hierarchy) over the triangles.
+ * Replaces the Intersector concept's implementations NaiveIntersector<T> (intersector/intersector.hh:55-101) and
+ * EmbreeIntersector (intersector/intersector_embree.hh:56-143). The handle owns device copies (the reference's
+ * intersectors hold non-owning spans, intersector.hh:31-35; owning is strictly more permissive).
+ * verts/idx are host pointers. scalar type is fixed at creation (f32 or f64).
+ * --------------------------------------------------------------------------------------------------------------- */
+typedef struct agwn_mesh agwn_mesh;
+int agwn_mesh_create_f32(const float* verts, int64_t nv, const int32_t* idx, int64_t nidx, agwn_mesh** out);
+int agwn_mesh_create_f64(const double* verts, int64_t nv, const int32_t* idx, int64_t nidx, agwn_mesh** out);
+int agwn_mesh_destroy(agwn_mesh* mesh);
+/* ntri, number of BVH nodes, scalar bytes (4 or 8). Any output may be NULL. */
+int agwn_mesh_info(const agwn_mesh* mesh, int64_t* ntri, int64_t* nnodes, int* scalar_bytes);
+
+/* Integer term: out[i] = signed ray-crossing count of the ray (pos[i], dir) -- NaiveIntersector::
+ * signed_intersection_count (intersector.hh:61-100), bit-exact: same Moeller-Trumbore arithmetic per triangle, the
+ * BVH only culls triangles that test cannot accept. Replaces eval_gwnr_mesh_batch_integer (gwn_mesh.hh:127-143). */
+int agwn_int_batch_f32(const agwn_mesh* mesh, const float* pos, int64_t n, const float* dir, int32_t* out);
+int agwn_int_batch_f64(const agwn_mesh* mesh, const double* pos, int64_t n, const double* dir, int32_t* out);
+int agwn_int_batch_f32_dev(const agwn_mesh* mesh, const float* pos, int64_t n, const float* dir, int32_t* out,
+                           agwn_stream stream);
+int agwn_int_batch_f64_dev(const agwn_mesh* mesh, const double* pos, int64_t n, const double* dir, int32_t* out,
+                           agwn_stream stream);
+
+/* Full GWN: out[i] = fractional + integer, eval_gwnr_mesh_batch (gwn_mesh.hh:201-219). seg7 may be NULL iff nseg == 0
+ * (a closed mesh, gwn_mesh.hh:26-27). */
+int agwn_full_batch_f32(const agwn_mesh* mesh, const float* pos, int64_t n, const float* x0, const float* seg7,
+                        int64_t nseg, float* out);
+int agwn_full_batch_f64(const agwn_mesh* mesh, const double* pos, int64_t n, const double* x0, const double* seg7,
+                        int64_t nseg, double* out);
+int agwn_full_batch_f32_dev(const agwn_mesh* mesh, const float* pos, int64_t n, const float* x0, const float* seg7,
+                            int64_t nseg, float* out, agwn_stream stream);
+int agwn_full_batch_f64_dev(const agwn_mesh* mesh, const double* pos, int64_t n, const double* x0,
+                            const double* seg7, int64_t nseg, double* out, agwn_stream stream);
+
+/* ---------------------------------------------------------------------------------------------------------------
+ * Open boundary. Replaces build_boundary_segments (math/common.hh:123-174): the same multiset of oriented weighted
+ * segments; the order is sorted by (min index, max index) -- the reference's order is unordered_map order and so is
+ * unspecified. Writes min(count, cap) segments to out7 (host) and the total count to *count.
+ * --------------------------------------------------------------------------------------------------------------- */
+int agwn_boundary_f32(const float* verts, int64_t nv, const int32_t* idx, int64_t nidx, float* out7, int64_t cap,
+                      int64_t* count);
+int agwn_boundary_f64(const double* verts, int64_t nv, const int32_t* idx, int64_t nidx, double* out7, int64_t cap,
+                      int64_t* count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AGWN_H */

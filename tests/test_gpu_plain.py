@@ -1,0 +1,169 @@
+"""Parity of the plain HIP path (fractional, integer, full GWN) through the C ABI, against the oracle and the
+reference's golden outputs.
+
+Bars (stated per test): integer counts bit-exact with NaiveIntersector<T> (intersector.hh:61-100); fractional FP64
+within 1e-12 of the FP64 restatement; fractional FP32 within 1e-5 of the FP64 restatement evaluated on the same FP32
+inputs (the reference's own FP32 sequential sum is itself not that accurate, see test_c1_f32_vs_reference_f32).
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import load_golden
+from antipodal_amd import meshgen as mg
+
+pytestmark = pytest.mark.gpu
+
+
+def _c1():
+    return load_golden("c1_icosphere_cap")
+
+
+def test_device_present(gpu):
+    assert gpu.device_available()
+
+
+@pytest.mark.parametrize("s,dt", [("f64", np.float64), ("f32", np.float32)])
+def test_c1_integer_term_bit_exact(gpu, s, dt):
+    g = _c1()
+    verts, idx = g["verts"].astype(dt), g["idx"]
+    pos = mg.uniform_points(0, 100_000, [-1.5] * 3, [1.5] * 3).astype(dt)
+    x0 = g[f"x0_{s}"]
+    print(f"case c1 integer {s}: n={len(pos)} ntri={len(idx) // 3}", flush=True)
+    with gpu.HipMesh(verts, idx) as mesh:
+        cnt = gpu.eval_gwnr_mesh_batch_integer(mesh, pos, x0)
+    ref = g[f"count_{s}"].astype(np.int32)
+    assert np.array_equal(cnt, ref), f"{int((cnt != ref).sum())} counts differ"
+
+
+def test_c1_f64_fractional_and_full(gpu, oracle):
+    g = _c1()
+    verts, idx, seg, x0 = g["verts"], g["idx"], g["seg_f64"], g["x0_f64"]
+    pos = mg.uniform_points(0, 100_000, [-1.5] * 3, [1.5] * 3)
+    print("case c1 f64 fractional/full", flush=True)
+    frac = gpu.eval_gwnr_mesh_batch_fractional(pos, x0, seg)
+    sub = g["sub"]
+    assert np.max(np.abs(frac[sub] - g["frac_sub_f64"])) < 1e-12
+    ref = oracle.frac(pos, seg, x0)
+    assert np.max(np.abs(frac - ref)) < 1e-12
+    with gpu.HipMesh(verts, idx) as mesh:
+        full = gpu.eval_gwnr_mesh_batch(mesh, pos, x0, seg)
+    assert np.max(np.abs(full[sub] - g["full_sub_f64"])) < 1e-12
+    # the full GWN against the per-triangle solid-angle sum (test/gwn_baseline.hh), reference tolerance 1e-9
+    assert np.max(np.abs(full[sub] - g["baseline_sub_f64"])) < 1e-9
+
+
+def test_c1_f32_fractional_within_1e5_of_fp64(gpu, oracle):
+    g = _c1()
+    seg32, x0 = g["seg_f32"], g["x0_f32"]
+    pos32 = mg.uniform_points(0, 100_000, [-1.5] * 3, [1.5] * 3).astype(np.float32)
+    print("case c1 f32 fractional", flush=True)
+    frac = gpu.eval_gwnr_mesh_batch_fractional(pos32, x0, seg32)
+    assert frac.dtype == np.float32
+    # FP64 evaluation of the same FP32 inputs
+    ref64 = oracle.frac(pos32.astype(np.float64), seg32.astype(np.float64), x0.astype(np.float64))
+    assert np.max(np.abs(frac.astype(np.float64) - ref64)) < 1e-5
+
+
+def test_c1_f32_vs_reference_f32(gpu):
+    """Against the reference's own FP32 output: within the reference's FP32 test tolerance (1e-4, test_gwn_mesh.cc)."""
+    g = _c1()
+    sub = g["sub"]
+    pos32 = mg.uniform_points(0, 100_000, [-1.5] * 3, [1.5] * 3).astype(np.float32)[sub]
+    frac = gpu.eval_gwnr_mesh_batch_fractional(pos32, g["x0_f32"], g["seg_f32"])
+    assert np.max(np.abs(frac - g["frac_sub_f32"])) < 1e-4
+    with gpu.HipMesh(g["verts"].astype(np.float32), g["idx"]) as mesh:
+        full = gpu.eval_gwnr_mesh_batch(mesh, pos32, g["x0_f32"], g["seg_f32"])
+    assert np.max(np.abs(full - g["full_sub_f32"])) < 1e-4
+
+
+@pytest.mark.parametrize("s,dt,tol", [("f64", np.float64, 1e-9), ("f32", np.float32, 1e-4)])
+def test_cube_fixtures(gpu, s, dt, tol):
+    """test_gwn_mesh.cc / test_intersector.cc fixtures: counts along 4 directions bit-exact (incl. the axis-aligned
+    one, whose rays graze face diagonals and double-count in Moeller-Trumbore -- reproduced, not fixed), full GWN vs
+    the reference and vs the solid-angle baseline within the reference's tolerances."""
+    g = load_golden("cube")
+    v, pts = g["verts"].astype(dt), g["points"].astype(dt)
+    for name in ("closed", "open_y"):
+        idx = g[name]
+        with gpu.HipMesh(v, idx) as mesh:
+            cnt = np.stack([gpu.eval_gwnr_mesh_batch_integer(mesh, pts, d.astype(dt)) for d in g["dirs"]])
+            seg = g["boundary"].astype(dt) if name == "open_y" else None
+            full = gpu.eval_gwnr_mesh_batch(mesh, pts, g[f"x0_{s}"], seg)
+        assert np.array_equal(cnt, g[f"count_{name}_{s}"])
+        assert np.max(np.abs(full - g[f"full_{name}_{s}"])) < tol
+        assert np.max(np.abs(full - g[f"baseline_{name}_{s}"])) < tol
+
+
+@pytest.mark.parametrize("s,dt", [("f64", np.float64), ("f32", np.float32)])
+def test_torus_subsample_counts_bit_exact(gpu, s, dt):
+    """The 1M-triangle holed torus: 384 queries of its 256^3 grid, counts bit-exact with the reference's Naive."""
+    g = load_golden("torus_sub")
+    verts, idx = mg.holed_torus_mesh()
+    pos = g[f"pos_{s}"]
+    print(f"case torus counts {s}", flush=True)
+    with gpu.HipMesh(verts.astype(dt), idx) as mesh:
+        cnt = gpu.eval_gwnr_mesh_batch_integer(mesh, pos, mg.example_x0(dt))
+    assert np.array_equal(cnt, g[f"count_{s}"])
+
+
+@pytest.mark.parametrize("s,dt,tol", [("f64", np.float64, 1e-10), ("f32", np.float32, 1e-5)])
+def test_torus_subsample_fractional(gpu, oracle, s, dt, tol):
+    g = load_golden("torus_sub")
+    verts, idx = mg.holed_torus_mesh()
+    seg = gpu.build_boundary_segments(verts.astype(dt), idx)
+    pos = g[f"pos_{s}"]
+    x0 = mg.example_x0(dt)
+    frac = gpu.eval_gwnr_mesh_batch_fractional(pos, x0, seg)
+    ref64 = oracle.frac(pos.astype(np.float64), seg.astype(np.float64), x0.astype(np.float64))
+    assert np.max(np.abs(frac.astype(np.float64) - ref64)) < tol
+    # and against the reference's own output (its segment order; FP64 sum-order differences only)
+    if dt == np.float64:
+        assert np.max(np.abs(frac - g["frac_f64"])) < 1e-10
+
+
+def test_torus_random_and_near_surface_counts(gpu, oracle):
+    """512 more torus queries (256 uniform in the bbox, 256 at |N(0,1e-3)| normal offsets from the surface): counts
+    bit-exact with the FP32 Moeller-Trumbore restatement."""
+    verts, idx = mg.holed_torus_mesh()
+    lo, hi = mg.expanded_bbox(verts)
+    u = mg.uniform_points(11, 256, lo, hi)
+    tri = idx.reshape(-1, 3)
+    pick = (mg.uniform01(11, 5, 256) * len(tri)).astype(np.int64)
+    a, b, c = verts[tri[pick, 0]], verts[tri[pick, 1]], verts[tri[pick, 2]]
+    n = np.cross(b - a, c - a)
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    near = (a + b + c) / 3 + n * (1e-3 * mg.normal(11, 6, 256))[:, None]
+    pos = np.concatenate([u, near]).astype(np.float32)
+    x0 = mg.example_x0(np.float32)
+    v32 = verts.astype(np.float32)
+    with gpu.HipMesh(v32, idx) as mesh:
+        cnt = gpu.eval_gwnr_mesh_batch_integer(mesh, pos, x0)
+    ref = oracle.naive_count(v32, idx, pos, x0, np.float32)
+    assert np.array_equal(cnt, ref)
+
+
+def test_empty_and_degenerate_inputs(gpu):
+    x0 = mg.example_x0(np.float64)
+    # no queries
+    assert len(gpu.eval_gwnr_mesh_batch_fractional(np.zeros((0, 3)), x0, np.zeros((5, 7)))) == 0
+    # closed mesh: empty boundary -> fractional exactly 0 (gwn_mesh.hh:26-27)
+    pos = mg.uniform_points(3, 1000, [-1] * 3, [1] * 3)
+    assert np.all(gpu.eval_gwnr_mesh_batch_fractional(pos, x0, np.zeros((0, 7))) == 0.0)
+    # a query exactly at a boundary vertex: the pair kernel gives atan2(0, 0) = 0 for the segments at it
+    g = load_golden("cube")
+    bnd = g["boundary"]
+    at = bnd[:, 0:3].copy()
+    got = gpu.eval_gwnr_mesh_batch_fractional(at, x0, bnd)
+    assert np.all(np.isfinite(got))
+    # mesh with no triangles: counts 0
+    with gpu.HipMesh(np.zeros((0, 3)), np.zeros(0, dtype=np.int32)) as mesh:
+        assert np.all(gpu.eval_gwnr_mesh_batch_integer(mesh, pos, x0) == 0)
+    # single triangle (no internal BVH node)
+    tv = np.array([[0, 0, 0], [0, 1, 0], [0, 0, 1.0]])
+    ti = np.array([0, 1, 2], dtype=np.int32)
+    p = np.array([[1.0, 0.2, 0.2], [-1.0, 0.2, 0.2], [1.0, 0.8, 0.8]])
+    with gpu.HipMesh(tv, ti) as mesh:
+        got = gpu.eval_gwnr_mesh_batch_integer(mesh, p, np.array([-1.0, 0.0, 0.0]))
+    assert got.tolist() == [-1, 0, 0]
