@@ -80,6 +80,15 @@ class Oracle(_Lib):
         f(_ptr(pos), len(pos), _ptr(seg), len(seg), _ptr(x), _ptr(out), nthreads)
         return out
 
+    def frac_ext(self, pos, seg7, x0, nthreads=0):
+        """Fractional term of double inputs computed in 80-bit extended precision (the arbiter of FP64 results)."""
+        pos, seg, x = _c(pos, np.float64, 3), _c(seg7, np.float64, 7), _c(x0, np.float64)
+        out = np.empty(len(pos), dtype=np.float64)
+        f = self.fn("frac_batch_f64x")
+        f.argtypes = [_p, _i64, _p, _i64, _p, _p, _int]
+        f(_ptr(pos), len(pos), _ptr(seg), len(seg), _ptr(x), _ptr(out), nthreads)
+        return out
+
     def naive_count(self, verts, idx, pos, direction, dtype=np.float64, nthreads=0):
         s = "f32" if dtype == np.float32 else "f64"
         v, i, pos, d = _c(verts, dtype, 3), _c(idx, np.int32), _c(pos, dtype, 3), _c(direction, dtype)

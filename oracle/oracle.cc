@@ -604,6 +604,21 @@ double orc_pair_half_area_f64(double const* x1, double const* v0, double const* 
 ORC_PLAIN(float, f32)
 ORC_PLAIN(double, f64)
 
+// The fractional term of double inputs evaluated in x87 extended precision (64-bit significand): the arbiter for
+// FP64 results, whose own rounding near a boundary segment (an ill-conditioned atan2) can reach ~1e-12.
+void orc_frac_batch_f64x(double const* pos, int64_t n, double const* seg7, int64_t nseg, double const* x0, double* out,
+                         int nth)
+{
+    using L = long double;
+    std::vector<L> s(size_t(7 * nseg));
+    for (int64_t i = 0; i < 7 * nseg; ++i)
+        s[size_t(i)] = L(seg7[i]);
+    V3<L> const d{L(x0[0]), L(x0[1]), L(x0[2])};
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nt(nth))
+    for (int64_t i = 0; i < n; ++i)
+        out[i] = double(frac_single(V3<L>{L(pos[3 * i]), L(pos[3 * i + 1]), L(pos[3 * i + 2])}, d, s.data(), nseg));
+}
+
 // --- predicates, for pinning against the reference's robust_predicates.hh ------------------------------------------
 #define ORC_PRED(C, SFX)                                                                                               \
     double orc_edge_det_##SFX(C const* q, C const* a, C const* b) { return double(edge_det(load3(q), load3(a), load3(b))); } \

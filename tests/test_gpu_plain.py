@@ -44,9 +44,13 @@ def test_c1_f64_fractional_and_full(gpu, oracle):
     print("case c1 f64 fractional/full", flush=True)
     frac = gpu.eval_gwnr_mesh_batch_fractional(pos, x0, seg)
     sub = g["sub"]
-    assert np.max(np.abs(frac[sub] - g["frac_sub_f64"])) < 1e-12
-    ref = oracle.frac(pos, seg, x0)
-    assert np.max(np.abs(frac - ref)) < 1e-12
+    # 1e-12 bar against the extended-precision evaluation of the same inputs (the FP64 restatement itself is off by
+    # up to ~1.6e-12 on this set, next to a boundary segment where atan2 is ill-conditioned)
+    ext = oracle.frac_ext(pos, seg, x0)
+    assert np.max(np.abs(frac - ext)) < 1e-12
+    # and within the two FP64 evaluations' combined rounding of the reference's own output / the FP64 restatement
+    assert np.max(np.abs(frac[sub] - g["frac_sub_f64"])) < 4e-12
+    assert np.max(np.abs(frac - oracle.frac(pos, seg, x0))) < 4e-12
     with gpu.HipMesh(verts, idx) as mesh:
         full = gpu.eval_gwnr_mesh_batch(mesh, pos, x0, seg)
     assert np.max(np.abs(full[sub] - g["full_sub_f64"])) < 1e-12
@@ -61,9 +65,9 @@ def test_c1_f32_fractional_within_1e5_of_fp64(gpu, oracle):
     print("case c1 f32 fractional", flush=True)
     frac = gpu.eval_gwnr_mesh_batch_fractional(pos32, x0, seg32)
     assert frac.dtype == np.float32
-    # FP64 evaluation of the same FP32 inputs
-    ref64 = oracle.frac(pos32.astype(np.float64), seg32.astype(np.float64), x0.astype(np.float64))
-    assert np.max(np.abs(frac.astype(np.float64) - ref64)) < 1e-5
+    # extended-precision evaluation of the same FP32 inputs
+    ref = oracle.frac_ext(pos32.astype(np.float64), seg32.astype(np.float64), x0.astype(np.float64))
+    assert np.max(np.abs(frac.astype(np.float64) - ref)) < 1e-5
 
 
 def test_c1_f32_vs_reference_f32(gpu):
@@ -108,7 +112,7 @@ def test_torus_subsample_counts_bit_exact(gpu, s, dt):
     assert np.array_equal(cnt, g[f"count_{s}"])
 
 
-@pytest.mark.parametrize("s,dt,tol", [("f64", np.float64, 1e-10), ("f32", np.float32, 1e-5)])
+@pytest.mark.parametrize("s,dt,tol", [("f64", np.float64, 1e-12), ("f32", np.float32, 1e-5)])
 def test_torus_subsample_fractional(gpu, oracle, s, dt, tol):
     g = load_golden("torus_sub")
     verts, idx = mg.holed_torus_mesh()
@@ -116,8 +120,8 @@ def test_torus_subsample_fractional(gpu, oracle, s, dt, tol):
     pos = g[f"pos_{s}"]
     x0 = mg.example_x0(dt)
     frac = gpu.eval_gwnr_mesh_batch_fractional(pos, x0, seg)
-    ref64 = oracle.frac(pos.astype(np.float64), seg.astype(np.float64), x0.astype(np.float64))
-    assert np.max(np.abs(frac.astype(np.float64) - ref64)) < tol
+    ref = oracle.frac_ext(pos.astype(np.float64), seg.astype(np.float64), x0.astype(np.float64))
+    assert np.max(np.abs(frac.astype(np.float64) - ref)) < tol
     # and against the reference's own output (its segment order; FP64 sum-order differences only)
     if dt == np.float64:
         assert np.max(np.abs(frac - g["frac_f64"])) < 1e-10
