@@ -1,19 +1,24 @@
 // Fractional GWN term on gfx950: the boundary sum of gwn_mesh.hh:34-46 / math/common.hh:181-189, batched.
 //
 // Work decomposition. One lane owns Q query points; a 256-lane workgroup owns 256*Q consecutive queries and walks the
-// whole boundary. The boundary is pre-rotated once per call into the pole frame (row 0 = x1 = -x0), packed 32 B per
-// segment, and read with wave-uniform addresses, so each segment arrives in SGPRs through one scalar load and feeds
-// the VALU of all 64 lanes: no LDS traffic and no bank conflicts in the inner loop.
+// whole boundary. The boundary is pre-rotated once per call into the pole frame, packed 32 B per segment, and read
+// with wave-uniform addresses, so each segment arrives in SGPRs through one scalar load and feeds the VALU of all 64
+// lanes: no LDS traffic and no bank conflicts in the inner loop.
 //
-// Per (query, segment) pair in the pole frame, with v = endpoint - query:
-//   num = v0.y v1.z - v0.z v1.y                         (= x1 . (v0 x v1))
-//   den = (l0 + v0.x)(l1 + v1.x) + v0.y v1.y + v0.z v1.z (= l0 l1 + l1 x1.v0 + l0 x1.v1 + v0.v1)
+// Pole frame: rows e1 = x1/|x1| (x1 = -x0), e2, e3 orthonormal and right-handed; coordinates (X, y, z) of
+// v = endpoint - query. With s = |x1| -- NOT assumed to be 1: a normalised float x0 is off unit length by ~1e-8, and
+// the reference's formula is not invariant to |x1| (near the pole the difference reaches 1e-4 rad per pair), so the
+// reference's function is evaluated exactly as written:
+//   num = x1 . (v0 x v1)                             = s (y0 z1 - z0 y1)
+//   den = l0 l1 + l1 x1.v0 + l0 x1.v1 + v0.v1        = l0 l1 + s (l1 X0 + l0 X1) + X0 X1 + y0 y1 + z0 z1
 //   area += w * atan2(num, den)
-// FP32 kernel: the pair is evaluated in FP32 and accumulated in FP64 (one convert + one FP64 FMA per pair; this is
-// cheaper than FP32 two-sum compensation and strictly more accurate). A pair whose (num, den) is small against
-// l0*l1 is ill-conditioned -- the query is near the segment, or an endpoint is seen almost along the pole -- and is
-// re-evaluated in FP64 from the FP64 copy of the segment with the cancellation-free l + v.x = (v.y^2+v.z^2)/(l - v.x).
-// FP64 kernel: every pair in FP64 with the cancellation-free form and two-sum compensated accumulation.
+// FP32 kernel: the pair is evaluated in FP32 and accumulated in FP64 (one convert + one FP64 FMA per pair; cheaper
+// than FP32 two-sum compensation and strictly more accurate). A pair whose (num, den) is small against l0*l1 is
+// ill-conditioned -- the query is near the segment, or an endpoint is seen almost along the pole -- and is
+// re-evaluated in FP64 from the FP64 copy of the segment, with the cancellation-free
+//   l + s X = (X >= 0 ? l + X : (y^2 + z^2) / (l - X)) + (s - 1) X,
+//   den = (l0 + s X0)(l1 + s X1) + (1 - s^2) X0 X1 + y0 y1 + z0 z1.
+// FP64 kernel: every pair in FP64 with that cancellation-free form and two-sum compensated accumulation.
 
 #include "agwn_internal.hpp"
 
@@ -80,25 +85,34 @@ __global__ void __launch_bounds__(kBlock) k_prep_segments(T const* __restrict__ 
     s64[i] = seg_f64{rax, ray, raz, rbx, rby, rbz, w, 0.0};
 }
 
+// |x1| and the exact small quantities derived from it, for the FP64 evaluation
+struct pole
+{
+    double s;   // |x1|
+    double sm1; // s - 1
+    double c;   // 1 - s^2
+};
+
 // One pair in FP64, cancellation-free denominator. v0 = a - p, v1 = b - p in the pole frame.
-__device__ __forceinline__ double pair_f64(double v0x, double v0y, double v0z, double v1x, double v1y, double v1z)
+__device__ __forceinline__ double pair_f64(double v0x, double v0y, double v0z, double v1x, double v1y, double v1z,
+                                           pole const& P)
 {
     double const yz0 = v0y * v0y + v0z * v0z;
     double const yz1 = v1y * v1y + v1z * v1z;
     double const l0 = sqrt(v0x * v0x + yz0);
     double const l1 = sqrt(v1x * v1x + yz1);
-    double const lp0 = v0x >= 0.0 ? l0 + v0x : yz0 / (l0 - v0x);
-    double const lp1 = v1x >= 0.0 ? l1 + v1x : yz1 / (l1 - v1x);
-    double const num = v0y * v1z - v0z * v1y;
-    double const den = lp0 * lp1 + (v0y * v1y + v0z * v1z);
+    double const lp0 = (v0x >= 0.0 ? l0 + v0x : yz0 / (l0 - v0x)) + P.sm1 * v0x;
+    double const lp1 = (v1x >= 0.0 ? l1 + v1x : yz1 / (l1 - v1x)) + P.sm1 * v1x;
+    double const num = P.s * (v0y * v1z - v0z * v1y);
+    double const den = lp0 * lp1 + (P.c * v0x * v1x + (v0y * v1y + v0z * v1z));
     return atan2(num, den);
 }
 
 template <int Q>
 __global__ void __launch_bounds__(kBlock)
     k_frac_f32(float const* __restrict__ pos, int64_t n, seg_f32 const* __restrict__ seg,
-               seg_f64 const* __restrict__ seg64, int64_t nseg, frame R, float tau, int32_t const* __restrict__ counts,
-               float* __restrict__ out)
+               seg_f64 const* __restrict__ seg64, int64_t nseg, frame R, pole P, float tau,
+               int32_t const* __restrict__ counts, float* __restrict__ out)
 {
     int64_t const base = int64_t(blockIdx.x) * (kBlock * Q) + threadIdx.x;
     float px[Q], py[Q], pz[Q];
@@ -123,6 +137,7 @@ __global__ void __launch_bounds__(kBlock)
         acc[q] = 0.0;
     }
 
+    float const sf = float(P.s);
 #pragma unroll 2
     for (int64_t s = 0; s < nseg; ++s)
     {
@@ -135,13 +150,13 @@ __global__ void __launch_bounds__(kBlock)
             float const v1x = g.bx - px[q], v1y = g.by - py[q], v1z = g.bz - pz[q];
             float const l0 = __builtin_sqrtf(v0x * v0x + v0y * v0y + v0z * v0z);
             float const l1 = __builtin_sqrtf(v1x * v1x + v1y * v1y + v1z * v1z);
-            float const num = v0y * v1z - v0z * v1y;
-            float const den = (l0 + v0x) * (l1 + v1x) + (v0y * v1y + v0z * v1z);
+            float const num = sf * (v0y * v1z - v0z * v1y);
+            float const den = l0 * l1 + sf * (l1 * v0x + l0 * v1x) + (v0x * v1x + v0y * v1y + v0z * v1z);
             double a;
             if (__builtin_expect(__builtin_fabsf(num) + __builtin_fabsf(den) < tau * (l0 * l1), 0))
             {
                 seg_f64 const h = seg64[s];
-                a = pair_f64(h.ax - dx[q], h.ay - dy[q], h.az - dz[q], h.bx - dx[q], h.by - dy[q], h.bz - dz[q]);
+                a = pair_f64(h.ax - dx[q], h.ay - dy[q], h.az - dz[q], h.bx - dx[q], h.by - dy[q], h.bz - dz[q], P);
             }
             else
             {
@@ -175,7 +190,7 @@ __device__ __forceinline__ void two_sum(double a, double b, double& s, double& e
 
 __global__ void __launch_bounds__(kBlock)
     k_frac_f64(double const* __restrict__ pos, int64_t n, seg_f64 const* __restrict__ seg, int64_t nseg, frame R,
-               int32_t const* __restrict__ counts, double* __restrict__ out)
+               pole P, int32_t const* __restrict__ counts, double* __restrict__ out)
 {
     int64_t const i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
     double x = 0.0, y = 0.0, z = 0.0;
@@ -193,7 +208,7 @@ __global__ void __launch_bounds__(kBlock)
     for (int64_t s = 0; s < nseg; ++s)
     {
         seg_f64 const g = seg[s];
-        double const a = pair_f64(g.ax - px, g.ay - py, g.az - pz, g.bx - px, g.by - py, g.bz - pz);
+        double const a = pair_f64(g.ax - px, g.ay - py, g.az - pz, g.bx - px, g.by - py, g.bz - pz, P);
         double t, e;
         two_sum(sum, g.w * a, t, e);
         sum = t;
@@ -206,6 +221,14 @@ __global__ void __launch_bounds__(kBlock)
             r += double(counts[i]);
         out[i] = r;
     }
+}
+
+pole make_pole(double x0x, double x0y, double x0z)
+{
+    double const s2 = x0x * x0x + x0y * x0y + x0z * x0z;
+    double const s = std::sqrt(s2);
+    double const sm1 = (s2 - 1.0) / (s + 1.0); // s - 1 without cancellation
+    return pole{s, sm1, -sm1 * (s + 1.0)};
 }
 
 // ill-conditioning threshold of the FP32 kernel: pairs with |num| + |den| < tau * l0 * l1 go to FP64
@@ -233,8 +256,9 @@ int frac_f32(float const* pos, int64_t n, float const* seg7, int64_t nseg, doubl
     int64_t const per_block = int64_t(kBlock) * kQ;
     int64_t const blocks = (n + per_block - 1) / per_block;
     AGWN_CHECK(blocks < (int64_t(1) << 31), "frac_f32: too many queries (%lld)", (long long)n);
+    pole const P = make_pole(x0[0], x0[1], x0[2]);
     k_frac_f32<kQ><<<dim3(unsigned(blocks)), dim3(kBlock), 0, stream>>>(pos, n, s32.as<seg_f32>(), s64.as<seg_f64>(),
-                                                                      nseg, R, kTau, counts, out);
+                                                                      nseg, R, P, kTau, counts, out);
     AGWN_HIP(hipGetLastError());
     return ok;
 }
@@ -255,7 +279,9 @@ int frac_f64(double const* pos, int64_t n, double const* seg7, int64_t nseg, dou
     }
     int64_t const blocks = (n + kBlock - 1) / kBlock;
     AGWN_CHECK(blocks < (int64_t(1) << 31), "frac_f64: too many queries (%lld)", (long long)n);
-    k_frac_f64<<<dim3(unsigned(blocks)), dim3(kBlock), 0, stream>>>(pos, n, s64.as<seg_f64>(), nseg, R, counts, out);
+    pole const P = make_pole(x0[0], x0[1], x0[2]);
+    k_frac_f64<<<dim3(unsigned(blocks)), dim3(kBlock), 0, stream>>>(pos, n, s64.as<seg_f64>(), nseg, R, P, counts,
+                                                                    out);
     AGWN_HIP(hipGetLastError());
     return ok;
 }
