@@ -13,6 +13,8 @@ from . import _lib
 from ._lib import AgwnError, device_available, last_error
 from .api import (
     HipMesh,
+    RobustMeshGwn,
+    eval_gwnr_mesh_batch_robust,
     build_boundary_segments,
     eval_gwnr_mesh_batch,
     eval_gwnr_mesh_batch_fractional,
@@ -22,6 +24,8 @@ from .api import (
 __all__ = [
     "AgwnError",
     "HipMesh",
+    "RobustMeshGwn",
+    "eval_gwnr_mesh_batch_robust",
     "build_boundary_segments",
     "device_available",
     "eval_gwnr_mesh_batch",

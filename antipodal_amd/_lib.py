@@ -47,6 +47,16 @@ _SIG = {
     "agwn_full_batch_f64": (_int, [_p, _p, _i64, _p, _p, _i64, _p]),
     "agwn_full_batch_f32_dev": (_int, [_p, _p, _i64, _p, _p, _i64, _p, _p]),
     "agwn_full_batch_f64_dev": (_int, [_p, _p, _i64, _p, _p, _i64, _p, _p]),
+    "agwn_robust_create": (_int, [_int, _int, _p, _i64, _p, _i64, _p, _i64, C.POINTER(_p)]),
+    "agwn_robust_destroy": (_int, [_p]),
+    "agwn_robust_info": (_int, [_p, C.POINTER(_int), C.POINTER(_int), C.POINTER(_i64), C.POINTER(_i64), _p,
+                                C.POINTER(C.c_double)]),
+    "agwn_robust_eval_batch": (_int, [_p, _p, _i64, _p]),
+    "agwn_robust_fractional_batch": (_int, [_p, _p, _i64, _p]),
+    "agwn_robust_count_batch": (_int, [_p, _p, _i64, _p]),
+    "agwn_robust_eval_batch_dev": (_int, [_p, _p, _i64, _p, _p]),
+    "agwn_robust_fractional_batch_dev": (_int, [_p, _p, _i64, _p, _p]),
+    "agwn_robust_count_batch_dev": (_int, [_p, _p, _i64, _p, _p]),
     "agwn_boundary_f32": (_int, [_p, _i64, _p, _i64, _p, _i64, C.POINTER(_i64)]),
     "agwn_boundary_f64": (_int, [_p, _i64, _p, _i64, _p, _i64, C.POINTER(_i64)]),
 }

@@ -5,6 +5,8 @@
   eval_gwnr_mesh_batch(mesh, positions, x0, boundary)        gwn_mesh.hh:201-219
   build_boundary_segments(vertices, indices)                 math/common.hh:123-174
   HipMesh(vertices, indices)                                 the Intersector concept, intersector/intersector.hh:8-43
+  RobustMeshGwn(vertices, indices, boundary, precision)      gwn_mesh_robust.hh:170-424
+  eval_gwnr_mesh_batch_robust(robust, positions)             gwn_mesh_robust.hh:441-456
 
 Arrays are numpy (host; results come back as numpy) or torch CUDA tensors (device; the *_dev entry points run on the
 given stream and write into `out`). Layouts are the reference's: positions (n, 3), boundary (nseg, 7) with the weight
@@ -176,3 +178,100 @@ def eval_gwnr_mesh_batch_integer(mesh: HipMesh, positions, x0, out=None, stream=
 def eval_gwnr_mesh_batch(mesh: HipMesh, positions, x0, boundary, out=None, stream=None):
     """Full GWN = fractional + integer of every position (gwn_mesh.hh:201-219)."""
     return mesh.full(positions, x0, boundary, out=out, stream=stream)
+
+
+PRECISIONS = {"int64_grid": 0, "int128_grid": 1, "exact_float": 2}
+
+
+def default_precision(dtype) -> str:
+    """robust_predicates.hh:322-323: int64_grid for float, exact_float for double."""
+    return "int64_grid" if np.dtype(dtype) == np.float32 else "exact_float"
+
+
+class RobustMeshGwn:
+    """Prepared robust GWN evaluator (gwn_mesh_robust.hh:170-424) on the GPU.
+
+    RobustMeshGwn(vertices, indices, boundary, precision=None, dtype=None): T = dtype (float32 or float64, default the
+    vertices' dtype); precision one of "int64_grid", "int128_grid", "exact_float" (default per T). Owns its device
+    copies after construction. The ray is fixed to -x (axis()); eval = fractional + signed_intersection_count.
+    """
+
+    def __init__(self, vertices, indices, boundary, precision: str | None = None, dtype=None):
+        v = np.asarray(vertices)
+        dt = np.dtype(dtype) if dtype is not None else (np.dtype(np.float32) if v.dtype == np.float32 else np.dtype(np.float64))
+        self.dtype = dt
+        self.precision = precision or default_precision(dt)
+        if self.precision not in PRECISIONS:
+            raise ValueError(f"unknown precision {self.precision!r}")
+        vv = _host(v, dt, 3)
+        ii = _host(indices, np.int32).reshape(-1)
+        if ii.size % 3:
+            raise ValueError("indices.size() must be a multiple of 3")
+        bb = _host(boundary if boundary is not None else np.zeros((0, 7)), dt, 7)
+        h = C.c_void_p()
+        check(lib.agwn_robust_create(dt.itemsize, PRECISIONS[self.precision], ptr(vv), len(vv), ptr(ii), ii.size,
+                                     ptr(bb) if len(bb) else None, len(bb), C.byref(h)))
+        self._h = h
+
+    @staticmethod
+    def axis():
+        """gwn_mesh_robust.hh:226 -- the fixed ray direction."""
+        return np.array([-1.0, 0.0, 0.0])
+
+    def info(self) -> dict:
+        sb, pr, nt, ns = C.c_int(), C.c_int(), C.c_int64(), C.c_int64()
+        cen = np.zeros(3)
+        sc = C.c_double()
+        check(lib.agwn_robust_info(self._h, C.byref(sb), C.byref(pr), C.byref(nt), C.byref(ns), ptr(cen),
+                                   C.byref(sc)))
+        return {"scalar_bytes": sb.value, "precision": pr.value, "ntri": nt.value, "nseg": ns.value,
+                "center": cen, "scale": sc.value}
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib.agwn_robust_destroy(self._h)
+            self._h = None
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _run(self, what: str, positions, out, stream):
+        is_count = what == "count"
+        if _is_dev(positions):
+            n = positions.numel() // 3
+            if out is None or out.numel() != n:
+                raise ValueError("positions.size() != out.size()")
+            fn = getattr(lib, f"agwn_robust_{what}_batch_dev")
+            check(fn(self._h, ptr(positions), n, ptr(out), _stream_ptr(stream)))
+            return out
+        pos = _host(positions, self.dtype, 3)
+        res = np.empty(len(pos), dtype=np.int32 if is_count else self.dtype)
+        fn = getattr(lib, f"agwn_robust_{what}_batch")
+        check(fn(self._h, ptr(pos), len(pos), ptr(res)))
+        return res
+
+    def eval(self, positions, out=None, stream=None):
+        """gwn_mesh_robust.hh:328 per position."""
+        return self._run("eval", positions, out, stream)
+
+    def fractional(self, positions, out=None, stream=None):
+        """gwn_mesh_robust.hh:264-325 per position."""
+        return self._run("fractional", positions, out, stream)
+
+    def signed_intersection_count(self, positions, out=None, stream=None):
+        """gwn_mesh_robust.hh:231-258 per position (int32)."""
+        return self._run("count", positions, out, stream)
+
+
+def eval_gwnr_mesh_batch_robust(robust: RobustMeshGwn, positions, out=None, stream=None):
+    """gwn_mesh_robust.hh:441-456."""
+    return robust.eval(positions, out=out, stream=stream)

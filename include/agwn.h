@@ -100,6 +100,33 @@ int agwn_full_batch_f64_dev(const agwn_mesh* mesh, const double* pos, int64_t n,
                             const double* seg7, int64_t nseg, double* out, agwn_stream stream);
 
 /* ---------------------------------------------------------------------------------------------------------------
+ * Robust evaluator. Replaces RobustMeshGwn<T, Precision> (gwn_mesh_robust.hh:170-424) and
+ * eval_gwnr_mesh_batch_robust (gwn_mesh_robust.hh:441-456). scalar_bytes = sizeof(T) (4 or 8); precision one of
+ * AGWN_INT64_GRID / AGWN_INT128_GRID / AGWN_EXACT_FLOAT (the reference's defaults: int64_grid for float, exact_float
+ * for double, robust_predicates.hh:322-323). verts/seg7 are T arrays (host); the constructor's quantization frame,
+ * swapped quantized boundary and exact predicates are the reference's (gwn_mesh_robust.hh:177-209, 334-378). The
+ * ray is baked in (-x, gwn_mesh_robust.hh:224-226); the handle owns its device copies (gwn_mesh_robust.hh:152-156).
+ * --------------------------------------------------------------------------------------------------------------- */
+typedef struct agwn_robust agwn_robust;
+int agwn_robust_create(int scalar_bytes, int precision, const void* verts, int64_t nv, const int32_t* idx,
+                       int64_t nidx, const void* seg7, int64_t nseg, agwn_robust** out);
+int agwn_robust_destroy(agwn_robust* robust);
+/* Any output may be NULL. center3/scale: the quantization frame (center 0, scale 1 for exact_float). */
+int agwn_robust_info(const agwn_robust* robust, int* scalar_bytes, int* precision, int64_t* ntri, int64_t* nseg,
+                     double* center3, double* scale);
+/* eval: RobustMeshGwn::eval (gwn_mesh_robust.hh:328) per position -> T out[n] */
+int agwn_robust_eval_batch(const agwn_robust* robust, const void* pos, int64_t n, void* out);
+/* fractional: RobustMeshGwn::fractional (gwn_mesh_robust.hh:264-325) -> T out[n] */
+int agwn_robust_fractional_batch(const agwn_robust* robust, const void* pos, int64_t n, void* out);
+/* count: RobustMeshGwn::signed_intersection_count (gwn_mesh_robust.hh:231-258) -> int32 out[n], bit-exact */
+int agwn_robust_count_batch(const agwn_robust* robust, const void* pos, int64_t n, int32_t* out);
+int agwn_robust_eval_batch_dev(const agwn_robust* robust, const void* pos, int64_t n, void* out, agwn_stream stream);
+int agwn_robust_fractional_batch_dev(const agwn_robust* robust, const void* pos, int64_t n, void* out,
+                                     agwn_stream stream);
+int agwn_robust_count_batch_dev(const agwn_robust* robust, const void* pos, int64_t n, int32_t* out,
+                                agwn_stream stream);
+
+/* ---------------------------------------------------------------------------------------------------------------
  * Open boundary. Replaces build_boundary_segments (math/common.hh:123-174): the same multiset of oriented weighted
  * segments; the order is sorted by (min index, max index) -- the reference's order is unordered_map order and so is
  * unspecified. Writes min(count, cap) segments to out7 (host) and the total count to *count.
