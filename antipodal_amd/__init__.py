@@ -12,6 +12,7 @@ from __future__ import annotations
 from . import _lib
 from ._lib import AgwnError, device_available, last_error
 from .api import (
+    HipGridGwn,
     HipMesh,
     RobustMeshGwn,
     eval_gwnr_mesh_batch_robust,
@@ -23,6 +24,7 @@ from .api import (
 
 __all__ = [
     "AgwnError",
+    "HipGridGwn",
     "HipMesh",
     "RobustMeshGwn",
     "eval_gwnr_mesh_batch_robust",

@@ -57,6 +57,11 @@ _SIG = {
     "agwn_robust_eval_batch_dev": (_int, [_p, _p, _i64, _p, _p]),
     "agwn_robust_fractional_batch_dev": (_int, [_p, _p, _i64, _p, _p]),
     "agwn_robust_count_batch_dev": (_int, [_p, _p, _i64, _p, _p]),
+    "agwn_grid_create": (_int, [C.POINTER(_p)]),
+    "agwn_grid_destroy": (_int, [_p]),
+    "agwn_grid_set_scene": (_int, [_p, _p, _i64, _p, _i64, _p, _i64]),
+    "agwn_grid_eval": (_int, [_p, _p, _p, _p, _p, _int, _int, _int, _p]),
+    "agwn_grid_eval_dev": (_int, [_p, _p, _p, _p, _p, _int, _int, _int, _p, _p]),
     "agwn_boundary_f32": (_int, [_p, _i64, _p, _i64, _p, _i64, C.POINTER(_i64)]),
     "agwn_boundary_f64": (_int, [_p, _i64, _p, _i64, _p, _i64, C.POINTER(_i64)]),
 }

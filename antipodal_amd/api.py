@@ -7,6 +7,7 @@
   HipMesh(vertices, indices)                                 the Intersector concept, intersector/intersector.hh:8-43
   RobustMeshGwn(vertices, indices, boundary, precision)      gwn_mesh_robust.hh:170-424
   eval_gwnr_mesh_batch_robust(robust, positions)             gwn_mesh_robust.hh:441-456
+  HipGridGwn().set_scene(...).eval_grid(...)                 OpenGLMeshGwn, gwn_mesh_opengl.hh:426-643
 
 Arrays are numpy (host; results come back as numpy) or torch CUDA tensors (device; the *_dev entry points run on the
 given stream and write into `out`). Layouts are the reference's: positions (n, 3), boundary (nseg, 7) with the weight
@@ -275,3 +276,55 @@ class RobustMeshGwn:
 def eval_gwnr_mesh_batch_robust(robust: RobustMeshGwn, positions, out=None, stream=None):
     """gwn_mesh_robust.hh:441-456."""
     return robust.eval(positions, out=out, stream=stream)
+
+
+class HipGridGwn:
+    """Dense-grid evaluator (OpenGLMeshGwn, gwn_mesh_opengl.hh:426-643) on HIP. FP32.
+
+    set_scene(vertices, indices, boundary) uploads the scene; eval_grid(start, dx, dy, dz, nx, ny, nz, out=None)
+    returns (or, for a torch CUDA `out`, writes on `stream`) the full GWN in ((iz*ny)+iy)*nx+ix order.
+    """
+
+    def __init__(self):
+        h = C.c_void_p()
+        check(lib.agwn_grid_create(C.byref(h)))
+        self._h = h
+
+    def set_scene(self, vertices, indices, boundary):
+        v = _host(vertices, np.float32, 3)
+        i = _host(indices, np.int32).reshape(-1)
+        if i.size % 3:
+            raise ValueError("indices.size() must be a multiple of 3")
+        b = _host(boundary if boundary is not None else np.zeros((0, 7)), np.float32, 7)
+        check(lib.agwn_grid_set_scene(self._h, ptr(v), len(v), ptr(i), i.size, ptr(b) if len(b) else None, len(b)))
+        return self
+
+    def eval_grid(self, start, dx, dy, dz, nx: int, ny: int, nz: int, out=None, stream=None):
+        vs = [np.ascontiguousarray(np.asarray(a, dtype=np.float32).reshape(3)) for a in (start, dx, dy, dz)]
+        if out is not None and _is_dev(out):
+            if out.numel() != nx * ny * nz:
+                raise ValueError("out.size() != nx*ny*nz")
+            check(lib.agwn_grid_eval_dev(self._h, *[ptr(a) for a in vs], nx, ny, nz, ptr(out), _stream_ptr(stream)))
+            return out
+        res = np.empty(nx * ny * nz, dtype=np.float32) if out is None else out
+        if res.shape != (nx * ny * nz,) or res.dtype != np.float32:
+            raise ValueError("out.size() != nx*ny*nz")
+        check(lib.agwn_grid_eval(self._h, *[ptr(a) for a in vs], nx, ny, nz, ptr(res)))
+        return res
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib.agwn_grid_destroy(self._h)
+            self._h = None
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

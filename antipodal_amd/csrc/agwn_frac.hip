@@ -108,9 +108,9 @@ __device__ __forceinline__ double pair_f64(double v0x, double v0y, double v0z, d
     return atan2(num, den);
 }
 
-template <int Q>
+template <int Q, class Src>
 __global__ void __launch_bounds__(kBlock)
-    k_frac_f32(float const* __restrict__ pos, int64_t n, seg_f32 const* __restrict__ seg,
+    k_frac_f32(Src const src, int64_t n, seg_f32 const* __restrict__ seg,
                seg_f64 const* __restrict__ seg64, int64_t nseg, frame R, pole P, float tau,
                int32_t const* __restrict__ counts, float* __restrict__ out)
 {
@@ -124,9 +124,11 @@ __global__ void __launch_bounds__(kBlock)
         double x = 0.0, y = 0.0, z = 0.0;
         if (i < n)
         {
-            x = double(pos[3 * i + 0]);
-            y = double(pos[3 * i + 1]);
-            z = double(pos[3 * i + 2]);
+            float fx, fy, fz;
+            query_at(src, i, fx, fy, fz);
+            x = double(fx);
+            y = double(fy);
+            z = double(fz);
         }
         dx[q] = R.r[0] * x + R.r[1] * y + R.r[2] * z;
         dy[q] = R.r[3] * x + R.r[4] * y + R.r[5] * z;
@@ -188,18 +190,15 @@ __device__ __forceinline__ void two_sum(double a, double b, double& s, double& e
     e = (a - av) + (b - bv);
 }
 
+template <class Src>
 __global__ void __launch_bounds__(kBlock)
-    k_frac_f64(double const* __restrict__ pos, int64_t n, seg_f64 const* __restrict__ seg, int64_t nseg, frame R,
+    k_frac_f64(Src const src, int64_t n, seg_f64 const* __restrict__ seg, int64_t nseg, frame R,
                pole P, int32_t const* __restrict__ counts, double* __restrict__ out)
 {
     int64_t const i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
     double x = 0.0, y = 0.0, z = 0.0;
     if (i < n)
-    {
-        x = pos[3 * i + 0];
-        y = pos[3 * i + 1];
-        z = pos[3 * i + 2];
-    }
+        query_at(src, i, x, y, z);
     double const px = R.r[0] * x + R.r[1] * y + R.r[2] * z;
     double const py = R.r[3] * x + R.r[4] * y + R.r[5] * z;
     double const pz = R.r[6] * x + R.r[7] * y + R.r[8] * z;
@@ -238,8 +237,9 @@ constexpr int kQ = 2;
 
 frame make_pole_frame(double x0x, double x0y, double x0z) { return make_frame_impl(x0x, x0y, x0z); }
 
-int frac_f32(float const* pos, int64_t n, float const* seg7, int64_t nseg, double const x0[3], int32_t const* counts,
-             float* out, hipStream_t stream)
+template <class Src>
+int frac_f32_src(Src const& pos, int64_t n, float const* seg7, int64_t nseg, double const x0[3],
+                 int32_t const* counts, float* out, hipStream_t stream)
 {
     if (n == 0)
         return ok;
@@ -257,14 +257,16 @@ int frac_f32(float const* pos, int64_t n, float const* seg7, int64_t nseg, doubl
     int64_t const blocks = (n + per_block - 1) / per_block;
     AGWN_CHECK(blocks < (int64_t(1) << 31), "frac_f32: too many queries (%lld)", (long long)n);
     pole const P = make_pole(x0[0], x0[1], x0[2]);
-    k_frac_f32<kQ><<<dim3(unsigned(blocks)), dim3(kBlock), 0, stream>>>(pos, n, s32.as<seg_f32>(), s64.as<seg_f64>(),
-                                                                      nseg, R, P, kTau, counts, out);
+    k_frac_f32<kQ, Src><<<dim3(unsigned(blocks)), dim3(kBlock), 0, stream>>>(pos, n, s32.as<seg_f32>(),
+                                                                           s64.as<seg_f64>(), nseg, R, P, kTau,
+                                                                           counts, out);
     AGWN_HIP(hipGetLastError());
     return ok;
 }
 
-int frac_f64(double const* pos, int64_t n, double const* seg7, int64_t nseg, double const x0[3],
-             int32_t const* counts, double* out, hipStream_t stream)
+template <class Src>
+int frac_f64_src(Src const& pos, int64_t n, double const* seg7, int64_t nseg, double const x0[3],
+                 int32_t const* counts, double* out, hipStream_t stream)
 {
     if (n == 0)
         return ok;
@@ -280,9 +282,16 @@ int frac_f64(double const* pos, int64_t n, double const* seg7, int64_t nseg, dou
     int64_t const blocks = (n + kBlock - 1) / kBlock;
     AGWN_CHECK(blocks < (int64_t(1) << 31), "frac_f64: too many queries (%lld)", (long long)n);
     pole const P = make_pole(x0[0], x0[1], x0[2]);
-    k_frac_f64<<<dim3(unsigned(blocks)), dim3(kBlock), 0, stream>>>(pos, n, s64.as<seg_f64>(), nseg, R, P, counts,
-                                                                    out);
+    k_frac_f64<Src><<<dim3(unsigned(blocks)), dim3(kBlock), 0, stream>>>(pos, n, s64.as<seg_f64>(), nseg, R, P,
+                                                                         counts, out);
     AGWN_HIP(hipGetLastError());
     return ok;
 }
+
+template int frac_f32_src<src_array<float>>(src_array<float> const&, int64_t, float const*, int64_t, double const*,
+                                            int32_t const*, float*, hipStream_t);
+template int frac_f32_src<src_grid>(src_grid const&, int64_t, float const*, int64_t, double const*, int32_t const*,
+                                    float*, hipStream_t);
+template int frac_f64_src<src_array<double>>(src_array<double> const&, int64_t, double const*, int64_t,
+                                             double const*, int32_t const*, double*, hipStream_t);
 } // namespace agwn

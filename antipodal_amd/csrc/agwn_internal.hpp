@@ -113,10 +113,66 @@ struct alignas(64) seg_f64
     double ax, ay, az, bx, by, bz, w, pad;
 };
 
-// Fractional launchers (agwn_frac.hip). pos/seg7/out/counts are device pointers. counts may be null (fractional
+// ---------------------------------------------------------------------------------------------------------------------
+// Query sources: a device array of positions, or a lattice generated in-kernel.
+// ---------------------------------------------------------------------------------------------------------------------
+template <class T>
+struct src_array
+{
+    T const* pos;
+};
+
+// p(ix,iy,iz) = start + dx*ix + dy*iy + dz*iz in float, evaluated left to right per component (the caller-side
+// positions of gwn_mesh_opengl.hh:511-519), flattened in ((iz*ny)+iy)*nx+ix order.
+struct grid_desc
+{
+    float start[3], dx[3], dy[3], dz[3];
+    int64_t nx, ny, nz;
+};
+struct src_grid
+{
+    grid_desc g;
+};
+
+template <class T>
+__device__ __forceinline__ void query_at(src_array<T> const& s, int64_t i, T& x, T& y, T& z)
+{
+    x = s.pos[3 * i];
+    y = s.pos[3 * i + 1];
+    z = s.pos[3 * i + 2];
+}
+
+__device__ __forceinline__ void query_at(src_grid const& s, int64_t i, float& x, float& y, float& z)
+{
+#pragma clang fp contract(off)
+    int64_t const ix = i % s.g.nx;
+    int64_t const t = i / s.g.nx;
+    int64_t const iy = t % s.g.ny;
+    int64_t const iz = t / s.g.ny;
+    float const fx = float(ix), fy = float(iy), fz = float(iz);
+    x = ((s.g.start[0] + s.g.dx[0] * fx) + s.g.dy[0] * fy) + s.g.dz[0] * fz;
+    y = ((s.g.start[1] + s.g.dx[1] * fx) + s.g.dy[1] * fy) + s.g.dz[1] * fz;
+    z = ((s.g.start[2] + s.g.dx[2] * fx) + s.g.dy[2] * fy) + s.g.dz[2] * fz;
+}
+
+// Fractional launchers (agwn_frac.hip). seg7/out/counts are device pointers. counts may be null (fractional
 // only); otherwise out[i] = fractional + counts[i] (the full GWN of gwn_mesh.hh:213-218).
-int frac_f32(float const* pos, int64_t n, float const* seg7, int64_t nseg, double const x0[3], int32_t const* counts,
-             float* out, hipStream_t stream);
-int frac_f64(double const* pos, int64_t n, double const* seg7, int64_t nseg, double const x0[3], int32_t const* counts,
-             double* out, hipStream_t stream);
+// Instantiated for src_array<float>, src_grid (FP32) and src_array<double> (FP64).
+template <class Src>
+int frac_f32_src(Src const& src, int64_t n, float const* seg7, int64_t nseg, double const x0[3],
+                 int32_t const* counts, float* out, hipStream_t stream);
+template <class Src>
+int frac_f64_src(Src const& src, int64_t n, double const* seg7, int64_t nseg, double const x0[3],
+                 int32_t const* counts, double* out, hipStream_t stream);
+
+inline int frac_f32(float const* pos, int64_t n, float const* seg7, int64_t nseg, double const x0[3],
+                    int32_t const* counts, float* out, hipStream_t stream)
+{
+    return frac_f32_src(src_array<float>{pos}, n, seg7, nseg, x0, counts, out, stream);
+}
+inline int frac_f64(double const* pos, int64_t n, double const* seg7, int64_t nseg, double const x0[3],
+                    int32_t const* counts, double* out, hipStream_t stream)
+{
+    return frac_f64_src(src_array<double>{pos}, n, seg7, nseg, x0, counts, out, stream);
+}
 } // namespace agwn

@@ -226,9 +226,9 @@ __device__ __forceinline__ float inv_dir(double d)
     return f != 0.0f ? 1.0f / f : (signbit(f) ? -1e30f : 1e30f);
 }
 
-template <class T>
+template <class T, class Src>
 __global__ void __launch_bounds__(kTB)
-    k_count(T const* __restrict__ pos, int64_t n, T dx, T dy, T dz, bvh_node<float> const* __restrict__ nodes,
+    k_count(Src const src, int64_t n, T dx, T dy, T dz, bvh_node<float> const* __restrict__ nodes,
             leaf_rec<T> const* __restrict__ rec, int32_t const* __restrict__ brute, int32_t const* __restrict__ nbrute_p,
             int use_bvh, int32_t* __restrict__ out)
 {
@@ -236,7 +236,8 @@ __global__ void __launch_bounds__(kTB)
     int64_t const i = int64_t(blockIdx.x) * kTB + threadIdx.x;
     if (i >= n)
         return;
-    T const px = pos[3 * i], py = pos[3 * i + 1], pz = pos[3 * i + 2];
+    T px, py, pz;
+    query_at(src, i, px, py, pz);
     int sum = 0;
     if (use_bvh)
     {
@@ -446,6 +447,13 @@ struct mesh_t final : mesh_impl
             AGWN_HIP(hipMemsetAsync(out, 0, size_t(n) * sizeof(int32_t), s));
             return ok;
         }
+        double qmax = 0.0;
+        AGWN_TRY(max_query_norm(pos, n, s, qmax));
+        return count_src(src_array<T>{pos}, n, qmax, d, out, s);
+    }
+
+    static int max_query_norm(T const* pos, int64_t n, hipStream_t s, double& qmax)
+    {
         dbuf qm;
         AGWN_TRY(qm.alloc(sizeof(unsigned long long), s));
         AGWN_HIP(hipMemsetAsync(qm.p, 0, sizeof(unsigned long long), s));
@@ -455,14 +463,27 @@ struct mesh_t final : mesh_impl
         unsigned long long bits = 0;
         AGWN_HIP(hipMemcpyAsync(&bits, qm.p, sizeof(bits), hipMemcpyDeviceToHost, s));
         AGWN_HIP(hipStreamSynchronize(s));
-        double qmax;
         std::memcpy(&qmax, &bits, sizeof(qmax));
+        return ok;
+    }
+
+    // counts of the queries of `src`, all of norm <= qmax
+    template <class Src>
+    int count_src(Src const& src, int64_t n, double qmax, T const d[3], int32_t* out, hipStream_t s)
+    {
+        if (n == 0)
+            return ok;
+        if (ntri == 0)
+        {
+            AGWN_HIP(hipMemsetAsync(out, 0, size_t(n) * sizeof(int32_t), s));
+            return ok;
+        }
         std::shared_ptr<dir_state<T>> st;
         AGWN_TRY(prepare(d, qmax, s, st));
         int64_t const blocks = (n + kTB - 1) / kTB;
         AGWN_CHECK(blocks < (int64_t(1) << 31), "too many queries (%lld)", (long long)n);
-        k_count<T><<<dim3(unsigned(blocks)), dim3(kTB), 0, s>>>(pos, n, d[0], d[1], d[2], st->nodes, st->rec,
-                                                                st->brute, st->nbrute, ntri > 1 ? 1 : 0, out);
+        k_count<T, Src><<<dim3(unsigned(blocks)), dim3(kTB), 0, s>>>(src, n, d[0], d[1], d[2], st->nodes, st->rec,
+                                                                     st->brute, st->nbrute, ntri > 1 ? 1 : 0, out);
         AGWN_HIP(hipGetLastError());
         // keep the state alive until the kernel has consumed it (it may be replaced by another thread's call)
         AGWN_HIP(hipStreamSynchronize(s));
@@ -685,5 +706,169 @@ int agwn_full_batch_f64_dev(agwn_mesh const* m, double const* pos, int64_t n, do
                             int64_t nseg, double* out, agwn_stream s)
 {
     return guarded2([&] { return full_dev(m, pos, n, x0, seg7, nseg, out, hipStream_t(s)); });
+}
+} // extern "C"
+
+// =====================================================================================================================
+// Dense-grid evaluator: OpenGLMeshGwn::set_scene / eval_grid (gwn_mesh_opengl.hh:426-643) as HIP kernels that
+// generate the lattice points themselves and write the full GWN straight to HBM in the caller's
+// ((iz*ny)+iy)*nx+ix layout -- no readback, no host un-permute. The ray / pole direction is chosen as the reference
+// does: the axis with the fewest cells (ties to the lower axis), x0 = normalize(that axis' step) (:521-550); the full
+// GWN does not depend on that choice. Each cell's value is eval_gwnr_mesh_batch of its lattice point with that x0:
+// integer term bit-exact with NaiveIntersector<float>, fractional as the FP32 fractional kernel.
+// =====================================================================================================================
+struct agwn_grid
+{
+    std::mutex mu;
+    std::unique_ptr<agwn::mesh_t<float>> mesh;
+    float* seg7 = nullptr;
+    int64_t nseg = 0;
+    ~agwn_grid()
+    {
+        if (seg7)
+            (void)hipFree(seg7);
+    }
+};
+
+namespace agwn
+{
+namespace
+{
+int grid_set_scene(agwn_grid* g, float const* verts, int64_t nv, int32_t const* idx, int64_t nidx, float const* seg7,
+                   int64_t nseg)
+{
+    AGWN_CHECK(g != nullptr, "grid handle is null");
+    AGWN_CHECK(nv >= 0 && nidx >= 0 && nseg >= 0, "negative count");
+    AGWN_CHECK(nidx % 3 == 0, "indices.size() = %lld is not a multiple of 3", (long long)nidx);
+    AGWN_CHECK(nidx == 0 || (verts && idx), "null mesh arrays");
+    AGWN_CHECK(nseg == 0 || seg7, "boundary is null with nseg=%lld", (long long)nseg);
+    for (int64_t i = 0; i < nidx; ++i)
+        AGWN_CHECK(idx[i] >= 0 && idx[i] < nv, "index %lld = %d out of range [0, %lld)", (long long)i, idx[i],
+                   (long long)nv);
+    AGWN_TRY(ensure_device());
+    auto m = std::make_unique<mesh_t<float>>();
+    AGWN_TRY(m->create(verts, nv, idx, nidx));
+    float* d = nullptr;
+    if (nseg > 0)
+    {
+        AGWN_HIP(hipMalloc(&d, size_t(nseg) * 7 * sizeof(float)));
+        hipError_t const e = hipMemcpy(d, seg7, size_t(nseg) * 7 * sizeof(float), hipMemcpyHostToDevice);
+        if (e != hipSuccess)
+        {
+            (void)hipFree(d);
+            AGWN_HIP(e);
+        }
+    }
+    std::lock_guard<std::mutex> lk(g->mu);
+    g->mesh = std::move(m);
+    if (g->seg7)
+        (void)hipFree(g->seg7);
+    g->seg7 = d;
+    g->nseg = nseg;
+    return ok;
+}
+
+int grid_eval_dev(agwn_grid* g, float const* start, float const* dx, float const* dy, float const* dz, int nx, int ny,
+                  int nz, float* out, hipStream_t s)
+{
+#pragma clang fp contract(off)
+    AGWN_CHECK(g != nullptr, "grid handle is null");
+    AGWN_CHECK(start && dx && dy && dz, "null grid vector");
+    AGWN_CHECK(nx > 0 && ny > 0 && nz > 0, "grid counts must be positive (%d, %d, %d)", nx, ny, nz);
+    AGWN_CHECK(out != nullptr, "out is null");
+    std::lock_guard<std::mutex> lk(g->mu);
+    AGWN_CHECK(g->mesh != nullptr, "set_scene was not called");
+    grid_desc gd;
+    for (int k = 0; k < 3; ++k)
+    {
+        gd.start[k] = start[k];
+        gd.dx[k] = dx[k];
+        gd.dy[k] = dy[k];
+        gd.dz[k] = dz[k];
+    }
+    gd.nx = nx;
+    gd.ny = ny;
+    gd.nz = nz;
+    int64_t const n = int64_t(nx) * ny * nz;
+    // integration / pole axis: fewest cells, ties to the lower axis (gwn_mesh_opengl.hh:524-531)
+    float const* dirs[3] = {dx, dy, dz};
+    int const counts[3] = {nx, ny, nz};
+    int ia = 0;
+    if (counts[1] < counts[ia])
+        ia = 1;
+    if (counts[2] < counts[ia])
+        ia = 2;
+    float const* a = dirs[ia];
+    float const len = std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+    AGWN_CHECK(len > 0.0f, "the integration axis step is zero");
+    float const x0f[3] = {a[0] / len, a[1] / len, a[2] / len};
+    double const x0d[3] = {double(x0f[0]), double(x0f[1]), double(x0f[2])};
+    // largest query norm over the lattice corners (the lattice is the convex hull of its corners)
+    double qmax = 0.0;
+    for (int c = 0; c < 8; ++c)
+    {
+        double const fx = (c & 1) ? double(nx - 1) : 0.0, fy = (c & 2) ? double(ny - 1) : 0.0,
+                     fz = (c & 4) ? double(nz - 1) : 0.0;
+        double n2 = 0.0;
+        for (int k = 0; k < 3; ++k)
+        {
+            double const v = double(start[k]) + double(dx[k]) * fx + double(dy[k]) * fy + double(dz[k]) * fz;
+            n2 += v * v;
+        }
+        qmax = std::max(qmax, std::sqrt(n2));
+    }
+    qmax = qmax * (1.0 + 1e-5) + 1e-30;
+    dbuf cnt;
+    AGWN_TRY(cnt.alloc(size_t(n) * sizeof(int32_t), s));
+    AGWN_TRY(g->mesh->count_src(src_grid{gd}, n, qmax, x0f, cnt.as<int32_t>(), s));
+    return frac_f32_src(src_grid{gd}, n, g->seg7, g->nseg, x0d, cnt.as<int32_t>(), out, s);
+}
+
+int grid_eval_host(agwn_grid* g, float const* start, float const* dx, float const* dy, float const* dz, int nx, int ny,
+                   int nz, float* out)
+{
+    AGWN_CHECK(nx > 0 && ny > 0 && nz > 0, "grid counts must be positive (%d, %d, %d)", nx, ny, nz);
+    AGWN_CHECK(out != nullptr, "out is null");
+    size_t const n = size_t(nx) * size_t(ny) * size_t(nz);
+    call_stream2 cs;
+    AGWN_HIP(hipStreamCreateWithFlags(&cs.s, hipStreamNonBlocking));
+    dbuf d;
+    AGWN_TRY(d.alloc(n * sizeof(float), cs.s));
+    AGWN_TRY(grid_eval_dev(g, start, dx, dy, dz, nx, ny, nz, d.as<float>(), cs.s));
+    AGWN_HIP(hipMemcpyAsync(out, d.p, n * sizeof(float), hipMemcpyDeviceToHost, cs.s));
+    AGWN_HIP(hipStreamSynchronize(cs.s));
+    return ok;
+}
+} // namespace
+} // namespace agwn
+
+extern "C" {
+int agwn_grid_create(agwn_grid** out)
+{
+    clear_error();
+    AGWN_CHECK(out != nullptr, "out is null");
+    *out = new agwn_grid;
+    return ok;
+}
+int agwn_grid_destroy(agwn_grid* g)
+{
+    clear_error();
+    delete g;
+    return ok;
+}
+int agwn_grid_set_scene(agwn_grid* g, float const* verts, int64_t nv, int32_t const* idx, int64_t nidx,
+                        float const* seg7, int64_t nseg)
+{
+    return guarded2([&] { return grid_set_scene(g, verts, nv, idx, nidx, seg7, nseg); });
+}
+int agwn_grid_eval(agwn_grid* g, float const* start, float const* dx, float const* dy, float const* dz, int nx, int ny,
+                   int nz, float* out)
+{
+    return guarded2([&] { return grid_eval_host(g, start, dx, dy, dz, nx, ny, nz, out); });
+}
+int agwn_grid_eval_dev(agwn_grid* g, float const* start, float const* dx, float const* dy, float const* dz, int nx,
+                       int ny, int nz, float* out, agwn_stream s)
+{
+    return guarded2([&] { return grid_eval_dev(g, start, dx, dy, dz, nx, ny, nz, out, hipStream_t(s)); });
 }
 } // extern "C"

@@ -127,6 +127,22 @@ int agwn_robust_count_batch_dev(const agwn_robust* robust, const void* pos, int6
                                 agwn_stream stream);
 
 /* ---------------------------------------------------------------------------------------------------------------
+ * Dense-grid evaluator. Replaces OpenGLMeshGwn (gwn_mesh_opengl.hh:426-809): set_scene (:471-510) uploads the mesh
+ * (building its LBVH) and the boundary; eval_grid (:519-643) evaluates the full GWN at every lattice point
+ * p = start + ix*dx + iy*dy + iz*dz, out[((iz*ny)+iy)*nx+ix] (host: agwn_grid_eval; device, written straight to HBM
+ * on `stream`: agwn_grid_eval_dev). FP32 throughout, as the reference's evaluator.
+ * --------------------------------------------------------------------------------------------------------------- */
+typedef struct agwn_grid agwn_grid;
+int agwn_grid_create(agwn_grid** out);
+int agwn_grid_destroy(agwn_grid* grid);
+int agwn_grid_set_scene(agwn_grid* grid, const float* verts, int64_t nv, const int32_t* idx, int64_t nidx,
+                        const float* seg7, int64_t nseg);
+int agwn_grid_eval(agwn_grid* grid, const float* start, const float* dx, const float* dy, const float* dz, int nx,
+                   int ny, int nz, float* out);
+int agwn_grid_eval_dev(agwn_grid* grid, const float* start, const float* dx, const float* dy, const float* dz,
+                       int nx, int ny, int nz, float* out, agwn_stream stream);
+
+/* ---------------------------------------------------------------------------------------------------------------
  * Open boundary. Replaces build_boundary_segments (math/common.hh:123-174): the same multiset of oriented weighted
  * segments; the order is sorted by (min index, max index) -- the reference's order is unordered_map order and so is
  * unspecified. Writes min(count, cap) segments to out7 (host) and the total count to *count.
