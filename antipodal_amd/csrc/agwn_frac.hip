@@ -63,13 +63,23 @@ frame make_frame_impl(double x0x, double x0y, double x0z)
     return frame{{ex, ey, ez, fx, fy, fz, gx, gy, gz}};
 }
 
+// Rotates and packs the boundary; slots nseg..npad-1 get zero-weight segments far along +e1 (they add exactly 0 and
+// are never ill-conditioned), so the FP32 kernel walks whole chunks.
 template <class T>
-__global__ void __launch_bounds__(kBlock) k_prep_segments(T const* __restrict__ seg7, int64_t nseg, frame R,
-                                                          seg_f32* __restrict__ s32, seg_f64* __restrict__ s64)
+__global__ void __launch_bounds__(kBlock) k_prep_segments(T const* __restrict__ seg7, int64_t nseg, int64_t npad,
+                                                          frame R, seg_f32* __restrict__ s32,
+                                                          seg_f64* __restrict__ s64)
 {
     int64_t const i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
-    if (i >= nseg)
+    if (i >= npad)
         return;
+    if (i >= nseg)
+    {
+        if (s32)
+            s32[i] = seg_f32{1.0e6f, 0.0f, 0.0f, 1.0e6f, 0.0f, 0.0f, 0.0f, 0.0f};
+        s64[i] = seg_f64{1.0e6, 0.0, 0.0, 1.0e6, 0.0, 0.0, 0.0, 0.0};
+        return;
+    }
     T const* g = seg7 + 7 * i;
     double const ax = double(g[0]), ay = double(g[1]), az = double(g[2]);
     double const bx = double(g[3]), by = double(g[4]), bz = double(g[5]);
@@ -108,63 +118,113 @@ __device__ __forceinline__ double pair_f64(double v0x, double v0y, double v0z, d
     return atan2(num, den);
 }
 
+// atan2 for the FP32 kernel: octant reduction to r = min(|y|,|x|) / max(|y|,|x|) in [0, 1] and a degree-15 odd
+// minimax polynomial atan(r) ~ r P(r^2) fitted here (Lawson's iteratively reweighted least squares on [0, 1]),
+// |error| <= 1.1e-7 evaluated in FP32. Quadrant fix-ups follow C's atan2 (signed zeros included); no NaN/inf inputs
+// reach it (num and den are finite sums of finite products).
+__device__ __forceinline__ float atan2_f32(float y, float x)
+{
+    constexpr float c0 = 0.9999993443489075f, c1 = -0.33329862356185913f, c2 = 0.19946569204330444f,
+                    c3 = -0.13908647000789642f, c4 = 0.09642244875431061f, c5 = -0.055913012474775314f,
+                    c6 = 0.021863453090190887f, c7 = -0.004054709803313017f;
+    constexpr float half_pi = 1.5707963267948966f, pi = 3.141592653589793f;
+    float const ax = __builtin_fabsf(x), ay = __builtin_fabsf(y);
+    float const mx = __builtin_fmaxf(ax, ay), mn = __builtin_fminf(ax, ay);
+    float const r = mn * __builtin_amdgcn_rcpf(__builtin_fmaxf(mx, 1.17549435e-38f));
+    float const t = r * r;
+    float p = c7;
+    p = __builtin_fmaf(p, t, c6);
+    p = __builtin_fmaf(p, t, c5);
+    p = __builtin_fmaf(p, t, c4);
+    p = __builtin_fmaf(p, t, c3);
+    p = __builtin_fmaf(p, t, c2);
+    p = __builtin_fmaf(p, t, c1);
+    p = __builtin_fmaf(p, t, c0);
+    float a = r * p;
+    a = ay > ax ? half_pi - a : a;
+    a = __builtin_signbit(x) ? pi - a : a;
+    return __builtin_copysignf(a, y);
+}
+
+template <class Src>
+__device__ __forceinline__ void query_f64(Src const& src, int64_t i, int64_t n, frame const& R, double& x, double& y,
+                                          double& z)
+{
+    double px = 0.0, py = 0.0, pz = 0.0;
+    if (i < n)
+    {
+        float fx, fy, fz;
+        query_at(src, i, fx, fy, fz);
+        px = double(fx);
+        py = double(fy);
+        pz = double(fz);
+    }
+    x = R.r[0] * px + R.r[1] * py + R.r[2] * pz;
+    y = R.r[3] * px + R.r[4] * py + R.r[5] * pz;
+    z = R.r[6] * px + R.r[7] * py + R.r[8] * pz;
+}
+
+constexpr int kChunk = 4;          // the boundary is padded to a multiple of this (zero-weight segments)
+constexpr int kTile = kBlock;      // segments staged in LDS per pass: one 32-byte segment per lane
+
 template <int Q, class Src>
 __global__ void __launch_bounds__(kBlock)
-    k_frac_f32(Src const src, int64_t n, seg_f32 const* __restrict__ seg,
-               seg_f64 const* __restrict__ seg64, int64_t nseg, frame R, pole P, float tau,
-               int32_t const* __restrict__ counts, float* __restrict__ out)
+    k_frac_f32(Src const src, int64_t n, seg_f32 const* __restrict__ seg, seg_f64 const* __restrict__ seg64,
+               int64_t nseg_padded, frame R, pole P, float tau, int32_t const* __restrict__ counts,
+               float* __restrict__ out)
 {
+    __shared__ seg_f32 tile[kTile];
     int64_t const base = int64_t(blockIdx.x) * (kBlock * Q) + threadIdx.x;
     float px[Q], py[Q], pz[Q];
-    double dx[Q], dy[Q], dz[Q], acc[Q];
+    double acc[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q)
     {
-        int64_t const i = base + int64_t(q) * kBlock;
-        double x = 0.0, y = 0.0, z = 0.0;
-        if (i < n)
-        {
-            float fx, fy, fz;
-            query_at(src, i, fx, fy, fz);
-            x = double(fx);
-            y = double(fy);
-            z = double(fz);
-        }
-        dx[q] = R.r[0] * x + R.r[1] * y + R.r[2] * z;
-        dy[q] = R.r[3] * x + R.r[4] * y + R.r[5] * z;
-        dz[q] = R.r[6] * x + R.r[7] * y + R.r[8] * z;
-        px[q] = float(dx[q]);
-        py[q] = float(dy[q]);
-        pz[q] = float(dz[q]);
+        double x, y, z;
+        query_f64(src, base + int64_t(q) * kBlock, n, R, x, y, z);
+        px[q] = float(x);
+        py[q] = float(y);
+        pz[q] = float(z);
         acc[q] = 0.0;
     }
-
     float const sf = float(P.s);
-#pragma unroll 2
-    for (int64_t s = 0; s < nseg; ++s)
+
+    for (int64_t t0 = 0; t0 < nseg_padded; t0 += kTile)
     {
-        seg_f32 const g = seg[s];
-        double const w = double(g.w);
-#pragma unroll
-        for (int q = 0; q < Q; ++q)
+        __syncthreads(); // the previous tile has been consumed by every wave
+        if (t0 + threadIdx.x < nseg_padded)
+            tile[threadIdx.x] = seg[t0 + threadIdx.x];
+        __syncthreads();
+        int const cnt = int(nseg_padded - t0 < kTile ? nseg_padded - t0 : kTile);
+#pragma unroll 2
+        for (int j = 0; j < cnt; ++j)
         {
-            float const v0x = g.ax - px[q], v0y = g.ay - py[q], v0z = g.az - pz[q];
-            float const v1x = g.bx - px[q], v1y = g.by - py[q], v1z = g.bz - pz[q];
-            float const l0 = __builtin_sqrtf(v0x * v0x + v0y * v0y + v0z * v0z);
-            float const l1 = __builtin_sqrtf(v1x * v1x + v1y * v1y + v1z * v1z);
-            float const num = sf * (v0y * v1z - v0z * v1y);
-            float const den = l0 * l1 + sf * (l1 * v0x + l0 * v1x) + (v0x * v1x + v0y * v1y + v0z * v1z);
-            double a;
-            if (__builtin_expect(__builtin_fabsf(num) + __builtin_fabsf(den) < tau * (l0 * l1), 0))
+            seg_f32 const g = tile[j]; // same address in every lane: an LDS broadcast
+            double const w = double(g.w);
+#pragma unroll
+            for (int q = 0; q < Q; ++q)
             {
-                seg_f64 const h = seg64[s];
-                a = pair_f64(h.ax - dx[q], h.ay - dy[q], h.az - dz[q], h.bx - dx[q], h.by - dy[q], h.bz - dz[q], P);
+                float const v0x = g.ax - px[q], v0y = g.ay - py[q], v0z = g.az - pz[q];
+                float const v1x = g.bx - px[q], v1y = g.by - py[q], v1z = g.bz - pz[q];
+                float const l0 = __builtin_amdgcn_sqrtf(v0x * v0x + v0y * v0y + v0z * v0z);
+                float const l1 = __builtin_amdgcn_sqrtf(v1x * v1x + v1y * v1y + v1z * v1z);
+                float const num = sf * (v0y * v1z - v0z * v1y);
+                float const den = l0 * l1 + sf * (l1 * v0x + l0 * v1x) + (v0x * v1x + v0y * v1y + v0z * v1z);
+                double a;
+                if (__builtin_expect(__builtin_fabsf(num) + __builtin_fabsf(den) < tau * (l0 * l1), 0))
+                {
+                    // ill-conditioned pair: FP64 from the FP64 copies of the segment and the query
+                    seg_f64 const h = seg64[t0 + j];
+                    double dx, dy, dz;
+                    query_f64(src, base + int64_t(q) * kBlock, n, R, dx, dy, dz);
+                    a = pair_f64(h.ax - dx, h.ay - dy, h.az - dz, h.bx - dx, h.by - dy, h.bz - dz, P);
+                }
+                else
+                {
+                    a = double(atan2_f32(num, den));
+                }
+                acc[q] = fma(w, a, acc[q]);
             }
-            else
-            {
-                a = double(atan2f(num, den));
-            }
-            acc[q] = fma(w, a, acc[q]);
         }
     }
 
@@ -244,13 +304,14 @@ int frac_f32_src(Src const& pos, int64_t n, float const* seg7, int64_t nseg, dou
     if (n == 0)
         return ok;
     frame const R = make_frame_impl(x0[0], x0[1], x0[2]);
+    int64_t const npad = ((nseg + kChunk - 1) / kChunk) * kChunk;
     dbuf s32, s64;
-    AGWN_TRY(s32.alloc(size_t(std::max<int64_t>(nseg, 1)) * sizeof(seg_f32), stream));
-    AGWN_TRY(s64.alloc(size_t(std::max<int64_t>(nseg, 1)) * sizeof(seg_f64), stream));
-    if (nseg > 0)
+    AGWN_TRY(s32.alloc(size_t(std::max<int64_t>(npad, kChunk)) * sizeof(seg_f32), stream));
+    AGWN_TRY(s64.alloc(size_t(std::max<int64_t>(npad, kChunk)) * sizeof(seg_f64), stream));
+    if (npad > 0)
     {
-        k_prep_segments<float><<<dim3(unsigned((nseg + kBlock - 1) / kBlock)), dim3(kBlock), 0, stream>>>(
-            seg7, nseg, R, s32.as<seg_f32>(), s64.as<seg_f64>());
+        k_prep_segments<float><<<dim3(unsigned((npad + kBlock - 1) / kBlock)), dim3(kBlock), 0, stream>>>(
+            seg7, nseg, npad, R, s32.as<seg_f32>(), s64.as<seg_f64>());
         AGWN_HIP(hipGetLastError());
     }
     int64_t const per_block = int64_t(kBlock) * kQ;
@@ -258,7 +319,7 @@ int frac_f32_src(Src const& pos, int64_t n, float const* seg7, int64_t nseg, dou
     AGWN_CHECK(blocks < (int64_t(1) << 31), "frac_f32: too many queries (%lld)", (long long)n);
     pole const P = make_pole(x0[0], x0[1], x0[2]);
     k_frac_f32<kQ, Src><<<dim3(unsigned(blocks)), dim3(kBlock), 0, stream>>>(pos, n, s32.as<seg_f32>(),
-                                                                           s64.as<seg_f64>(), nseg, R, P, kTau,
+                                                                           s64.as<seg_f64>(), npad, R, P, kTau,
                                                                            counts, out);
     AGWN_HIP(hipGetLastError());
     return ok;
@@ -276,7 +337,7 @@ int frac_f64_src(Src const& pos, int64_t n, double const* seg7, int64_t nseg, do
     if (nseg > 0)
     {
         k_prep_segments<double><<<dim3(unsigned((nseg + kBlock - 1) / kBlock)), dim3(kBlock), 0, stream>>>(
-            seg7, nseg, R, nullptr, s64.as<seg_f64>());
+            seg7, nseg, nseg, R, nullptr, s64.as<seg_f64>());
         AGWN_HIP(hipGetLastError());
     }
     int64_t const blocks = (n + kBlock - 1) / kBlock;
