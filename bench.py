@@ -176,6 +176,16 @@ def main() -> int:
         if cpu:
             log(f"cpu baseline {cpu['value']:.1f} queries/s on {cpu['cores']} threads")
 
+    # HBM bytes per launch of the dominant kernel, from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE passes
+    # (profiles/traffic.json, scaled by the copy calibration); null when no profile of this workload is committed
+    traffic = None
+    tj = ROOT / "profiles" / "traffic.json"
+    if tj.exists() and n == 256:
+        try:
+            traffic = json.loads(tj.read_text())["k_frac_f32"]["hbm_bytes_per_launch"]
+        except (KeyError, ValueError):
+            traffic = None
+
     value = nq / (ms_per_step * 1e-3)
     line = {
         "metric": "GWN queries/s, 1M-tri open mesh; surrogate+fallback vs plain, 0 mislabels",
@@ -205,7 +215,8 @@ def main() -> int:
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE x2 + WRITE_SIZE, see profiles/)",
             "kernel_ms": frac_ms,
             "flop_per_launch": pairs * FLOP_PER_PAIR,
         },
