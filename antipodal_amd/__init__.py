@@ -21,9 +21,11 @@ from .api import (
     eval_gwnr_mesh_batch_fractional,
     eval_gwnr_mesh_batch_integer,
 )
+from .surrogate import GwnSurrogate, fit_surrogate
 
 __all__ = [
     "AgwnError",
+    "GwnSurrogate",
     "HipGridGwn",
     "HipMesh",
     "RobustMeshGwn",
@@ -33,5 +35,6 @@ __all__ = [
     "eval_gwnr_mesh_batch",
     "eval_gwnr_mesh_batch_fractional",
     "eval_gwnr_mesh_batch_integer",
+    "fit_surrogate",
     "last_error",
 ]

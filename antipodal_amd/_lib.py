@@ -26,7 +26,7 @@ class AgwnError(RuntimeError):
         self.code = code
 
 
-# argument signatures; every entry point returns int except the three informational ones
+# argument signatures; every entry point returns an int status except the informational ones
 _SIG = {
     "agwn_last_error": (C.c_char_p, []),
     "agwn_version": (_int, []),
@@ -64,6 +64,18 @@ _SIG = {
     "agwn_grid_eval_dev": (_int, [_p, _p, _p, _p, _p, _int, _int, _int, _p, _p]),
     "agwn_boundary_f32": (_int, [_p, _i64, _p, _i64, _p, _i64, C.POINTER(_i64)]),
     "agwn_boundary_f64": (_int, [_p, _i64, _p, _i64, _p, _i64, C.POINTER(_i64)]),
+    "agwn_mlp_num_params": (_i64, []),
+    "agwn_mlp_create": (_int, [_p, _p, C.c_float, C.c_uint64, C.POINTER(_p)]),
+    "agwn_mlp_destroy": (_int, [_p]),
+    "agwn_mlp_info": (_int, [_p, _p, _p, _p]),
+    "agwn_mlp_get_params": (_int, [_p, _p]),
+    "agwn_mlp_set_params": (_int, [_p, _p]),
+    "agwn_mlp_train_dev": (_int, [_p, _p, _p, _i64, _int, _int, C.c_float, C.c_uint64, _p, _p]),
+    "agwn_mlp_grad": (_int, [_p, _p, _p, _i64, _p, _p, C.POINTER(C.c_float)]),
+    "agwn_mlp_predict_dev": (_int, [_p, _p, _i64, _p, _p]),
+    "agwn_label_surrogate_dev": (_int, [_p, _p, _p, _i64, _p, _p, _i64, C.c_float, _i64, C.c_uint64, _p, _p, _p]),
+    "agwn_grid_label_surrogate_dev": (_int, [_p, _p, _p, _p, _p, _p, _int, _int, _int, C.c_float, _i64, C.c_uint64,
+                                             _p, _p, _p]),
 }
 
 
@@ -77,6 +89,14 @@ def header_symbols() -> list[str]:
 def _load() -> C.CDLL:
     if not LIB_PATH.exists():
         raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() (hipcc --offload-arch=gfx950)")
+    # torch-ROCm ships its own HIP and HSA runtimes under the same sonames (libamdhip64.so.7, libhsa-runtime64.so.1)
+    # but NEEDs them by their unversioned names. Loaded first, they satisfy libagwn.so's dependencies and the process
+    # keeps one runtime; loaded after libagwn.so they become a second HSA instance that cannot open the device, and
+    # torch then reports no GPU. So torch, when present, is imported before the library is opened.
+    try:
+        import torch  # noqa: F401
+    except ImportError:  # pragma: no cover - the C ABI does not need torch
+        pass
     lib = C.CDLL(str(LIB_PATH))
     for name, (res, args) in _SIG.items():
         fn = getattr(lib, name, None)

@@ -8,6 +8,8 @@
 #include <cstdio>
 #include <string>
 
+struct agwn_mlp; // surrogate network handle (agwn_mlp.hip)
+
 namespace agwn
 {
 // ---------------------------------------------------------------------------------------------------------------------
@@ -164,6 +166,19 @@ int frac_f32_src(Src const& src, int64_t n, float const* seg7, int64_t nseg, dou
 template <class Src>
 int frac_f64_src(Src const& src, int64_t n, double const* seg7, int64_t nseg, double const x0[3],
                  int32_t const* counts, double* out, hipStream_t stream);
+
+// Surrogate routing (agwn_mlp.hip), instantiated for src_array<float> and src_grid:
+//   mlp_route      network labels of n points -> labels; the band / audit points are appended to list (2*i + audit)
+//                  and counted in counters[0] (band), [1] (audit), [2] (list length); counters start zeroed
+//   gather_routed  pts[e] = the query of list entry e (the coordinates the exact kernels would compute)
+//   route_apply    labels of list entries from their exact values w; audit disagreements counted in counters[3]
+template <class Src>
+int mlp_route(agwn_mlp const* mlp, Src const& src, int64_t n, float delta, uint32_t audit_mod, uint32_t audit_seed,
+              int8_t* labels, int64_t* list, unsigned long long* counters, hipStream_t s);
+template <class Src>
+int gather_routed(Src const& src, int64_t const* list, int64_t m, float* pts, hipStream_t s);
+int route_apply(int64_t const* list, int64_t m, float const* w, int8_t* labels, unsigned long long* counters,
+                hipStream_t s);
 
 inline int frac_f32(float const* pos, int64_t n, float const* seg7, int64_t nseg, double const x0[3],
                     int32_t const* counts, float* out, hipStream_t stream)

@@ -768,28 +768,33 @@ int grid_set_scene(agwn_grid* g, float const* verts, int64_t nv, int32_t const* 
     return ok;
 }
 
-int grid_eval_dev(agwn_grid* g, float const* start, float const* dx, float const* dy, float const* dz, int nx, int ny,
-                  int nz, float* out, hipStream_t s)
+// The lattice, the integration / pole axis and the query-norm bound of one eval_grid call.
+struct grid_call
+{
+    grid_desc gd;
+    int64_t n;
+    float x0f[3];
+    double x0d[3];
+    double qmax;
+};
+
+int grid_setup(float const* start, float const* dx, float const* dy, float const* dz, int nx, int ny, int nz,
+               grid_call& c)
 {
 #pragma clang fp contract(off)
-    AGWN_CHECK(g != nullptr, "grid handle is null");
     AGWN_CHECK(start && dx && dy && dz, "null grid vector");
     AGWN_CHECK(nx > 0 && ny > 0 && nz > 0, "grid counts must be positive (%d, %d, %d)", nx, ny, nz);
-    AGWN_CHECK(out != nullptr, "out is null");
-    std::lock_guard<std::mutex> lk(g->mu);
-    AGWN_CHECK(g->mesh != nullptr, "set_scene was not called");
-    grid_desc gd;
     for (int k = 0; k < 3; ++k)
     {
-        gd.start[k] = start[k];
-        gd.dx[k] = dx[k];
-        gd.dy[k] = dy[k];
-        gd.dz[k] = dz[k];
+        c.gd.start[k] = start[k];
+        c.gd.dx[k] = dx[k];
+        c.gd.dy[k] = dy[k];
+        c.gd.dz[k] = dz[k];
     }
-    gd.nx = nx;
-    gd.ny = ny;
-    gd.nz = nz;
-    int64_t const n = int64_t(nx) * ny * nz;
+    c.gd.nx = nx;
+    c.gd.ny = ny;
+    c.gd.nz = nz;
+    c.n = int64_t(nx) * ny * nz;
     // integration / pole axis: fewest cells, ties to the lower axis (gwn_mesh_opengl.hh:524-531)
     float const* dirs[3] = {dx, dy, dz};
     int const counts[3] = {nx, ny, nz};
@@ -801,14 +806,17 @@ int grid_eval_dev(agwn_grid* g, float const* start, float const* dx, float const
     float const* a = dirs[ia];
     float const len = std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
     AGWN_CHECK(len > 0.0f, "the integration axis step is zero");
-    float const x0f[3] = {a[0] / len, a[1] / len, a[2] / len};
-    double const x0d[3] = {double(x0f[0]), double(x0f[1]), double(x0f[2])};
+    for (int k = 0; k < 3; ++k)
+    {
+        c.x0f[k] = a[k] / len;
+        c.x0d[k] = double(c.x0f[k]);
+    }
     // largest query norm over the lattice corners (the lattice is the convex hull of its corners)
     double qmax = 0.0;
-    for (int c = 0; c < 8; ++c)
+    for (int cc = 0; cc < 8; ++cc)
     {
-        double const fx = (c & 1) ? double(nx - 1) : 0.0, fy = (c & 2) ? double(ny - 1) : 0.0,
-                     fz = (c & 4) ? double(nz - 1) : 0.0;
+        double const fx = (cc & 1) ? double(nx - 1) : 0.0, fy = (cc & 2) ? double(ny - 1) : 0.0,
+                     fz = (cc & 4) ? double(nz - 1) : 0.0;
         double n2 = 0.0;
         for (int k = 0; k < 3; ++k)
         {
@@ -817,11 +825,86 @@ int grid_eval_dev(agwn_grid* g, float const* start, float const* dx, float const
         }
         qmax = std::max(qmax, std::sqrt(n2));
     }
-    qmax = qmax * (1.0 + 1e-5) + 1e-30;
+    c.qmax = qmax * (1.0 + 1e-5) + 1e-30;
+    return ok;
+}
+
+int grid_eval_dev(agwn_grid* g, float const* start, float const* dx, float const* dy, float const* dz, int nx, int ny,
+                  int nz, float* out, hipStream_t s)
+{
+    AGWN_CHECK(g != nullptr, "grid handle is null");
+    AGWN_CHECK(out != nullptr, "out is null");
+    grid_call c;
+    AGWN_TRY(grid_setup(start, dx, dy, dz, nx, ny, nz, c));
+    std::lock_guard<std::mutex> lk(g->mu);
+    AGWN_CHECK(g->mesh != nullptr, "set_scene was not called");
     dbuf cnt;
-    AGWN_TRY(cnt.alloc(size_t(n) * sizeof(int32_t), s));
-    AGWN_TRY(g->mesh->count_src(src_grid{gd}, n, qmax, x0f, cnt.as<int32_t>(), s));
-    return frac_f32_src(src_grid{gd}, n, g->seg7, g->nseg, x0d, cnt.as<int32_t>(), out, s);
+    AGWN_TRY(cnt.alloc(size_t(c.n) * sizeof(int32_t), s));
+    AGWN_TRY(g->mesh->count_src(src_grid{c.gd}, c.n, c.qmax, c.x0f, cnt.as<int32_t>(), s));
+    return frac_f32_src(src_grid{c.gd}, c.n, g->seg7, g->nseg, c.x0d, cnt.as<int32_t>(), out, s);
+}
+
+// Surrogate labelling with the exact FP32 path as fallback (see mlp_route): network labels for every query, then the
+// band and audit points are gathered, evaluated exactly (counts + fractional with the same kernels, x0 and query
+// coordinates as the plain evaluation) and relabelled.
+template <class Src>
+int label_surrogate(mesh_t<float>& mesh, agwn_mlp const* mlp, Src const& src, int64_t n, double qmax,
+                    float const x0f[3], float const* seg7, int64_t nseg, float delta, int64_t audit_every,
+                    uint64_t audit_seed, int8_t* labels, int64_t* stats, hipStream_t s)
+{
+    AGWN_CHECK(mlp != nullptr, "mlp handle is null");
+    AGWN_CHECK(std::isfinite(delta) && delta >= 0.0f, "delta must be finite and >= 0");
+    AGWN_CHECK(audit_every >= 0 && audit_every <= int64_t(UINT32_MAX), "audit_every out of range");
+    AGWN_CHECK(n == 0 || labels != nullptr, "labels is null");
+    AGWN_CHECK(nseg == 0 || seg7 != nullptr, "boundary is null with nseg=%lld", (long long)nseg);
+    unsigned long long cnt[4] = {0, 0, 0, 0};
+    if (n > 0)
+    {
+        dbuf dc, list;
+        AGWN_TRY(dc.alloc(sizeof(cnt), s));
+        AGWN_TRY(list.alloc(size_t(n) * sizeof(int64_t), s));
+        AGWN_HIP(hipMemsetAsync(dc.p, 0, sizeof(cnt), s));
+        AGWN_TRY(mlp_route(mlp, src, n, delta, uint32_t(audit_every), uint32_t(audit_seed ^ (audit_seed >> 32)), labels,
+                           list.as<int64_t>(), dc.as<unsigned long long>(), s));
+        AGWN_HIP(hipMemcpyAsync(cnt, dc.p, sizeof(cnt), hipMemcpyDeviceToHost, s));
+        AGWN_HIP(hipStreamSynchronize(s));
+        int64_t const m = int64_t(cnt[2]);
+        if (m > 0)
+        {
+            double const x0d[3] = {double(x0f[0]), double(x0f[1]), double(x0f[2])};
+            dbuf pts, c, w;
+            AGWN_TRY(pts.alloc(size_t(m) * 3 * sizeof(float), s));
+            AGWN_TRY(c.alloc(size_t(m) * sizeof(int32_t), s));
+            AGWN_TRY(w.alloc(size_t(m) * sizeof(float), s));
+            AGWN_TRY(gather_routed(src, list.as<int64_t>(), m, pts.as<float>(), s));
+            AGWN_TRY(mesh.count_src(src_array<float>{pts.as<float>()}, m, qmax, x0f, c.as<int32_t>(), s));
+            AGWN_TRY(frac_f32(pts.as<float>(), m, seg7, nseg, x0d, c.as<int32_t>(), w.as<float>(), s));
+            AGWN_TRY(route_apply(list.as<int64_t>(), m, w.as<float>(), labels, dc.as<unsigned long long>(), s));
+            AGWN_HIP(hipMemcpyAsync(cnt, dc.p, sizeof(cnt), hipMemcpyDeviceToHost, s));
+            AGWN_HIP(hipStreamSynchronize(s));
+        }
+    }
+    if (stats)
+    {
+        stats[0] = int64_t(cnt[0]);
+        stats[1] = int64_t(cnt[1]);
+        stats[2] = int64_t(cnt[2]);
+        stats[3] = int64_t(cnt[3]);
+    }
+    return ok;
+}
+
+int grid_label_dev(agwn_grid* g, agwn_mlp const* mlp, float const* start, float const* dx, float const* dy,
+                   float const* dz, int nx, int ny, int nz, float delta, int64_t audit_every, uint64_t audit_seed,
+                   int8_t* labels, int64_t* stats, hipStream_t s)
+{
+    AGWN_CHECK(g != nullptr, "grid handle is null");
+    grid_call c;
+    AGWN_TRY(grid_setup(start, dx, dy, dz, nx, ny, nz, c));
+    std::lock_guard<std::mutex> lk(g->mu);
+    AGWN_CHECK(g->mesh != nullptr, "set_scene was not called");
+    return label_surrogate(*g->mesh, mlp, src_grid{c.gd}, c.n, c.qmax, c.x0f, g->seg7, g->nseg, delta, audit_every,
+                           audit_seed, labels, stats, s);
 }
 
 int grid_eval_host(agwn_grid* g, float const* start, float const* dx, float const* dy, float const* dz, int nx, int ny,
@@ -870,5 +953,34 @@ int agwn_grid_eval_dev(agwn_grid* g, float const* start, float const* dx, float 
                        int ny, int nz, float* out, agwn_stream s)
 {
     return guarded2([&] { return grid_eval_dev(g, start, dx, dy, dz, nx, ny, nz, out, hipStream_t(s)); });
+}
+int agwn_grid_label_surrogate_dev(agwn_grid* g, agwn_mlp const* mlp, float const* start, float const* dx,
+                                  float const* dy, float const* dz, int nx, int ny, int nz, float delta,
+                                  int64_t audit_every, uint64_t audit_seed, int8_t* labels, int64_t* stats,
+                                  agwn_stream s)
+{
+    return guarded2([&] {
+        return grid_label_dev(g, mlp, start, dx, dy, dz, nx, ny, nz, delta, audit_every, audit_seed, labels, stats,
+                              hipStream_t(s));
+    });
+}
+int agwn_label_surrogate_dev(agwn_mesh const* m, agwn_mlp const* mlp, float const* pos, int64_t n, float const* x0,
+                             float const* seg7, int64_t nseg, float delta, int64_t audit_every, uint64_t audit_seed,
+                             int8_t* labels, int64_t* stats, agwn_stream s)
+{
+    return guarded2([&]() -> int {
+        AGWN_CHECK(m != nullptr && m->impl != nullptr, "mesh handle is null");
+        AGWN_CHECK(m->impl->scalar_bytes() == 4, "surrogate labelling needs an f32 mesh");
+        AGWN_CHECK(n >= 0, "negative count");
+        AGWN_CHECK(n == 0 || pos != nullptr, "positions are null");
+        AGWN_CHECK(x0 != nullptr, "x0 is null");
+        auto& mesh = static_cast<mesh_t<float>&>(*m->impl);
+        hipStream_t const st = hipStream_t(s);
+        double qmax = 0.0;
+        if (n > 0)
+            AGWN_TRY(mesh_t<float>::max_query_norm(pos, n, st, qmax));
+        return label_surrogate(mesh, mlp, src_array<float>{pos}, n, qmax, x0, seg7, nseg, delta, audit_every,
+                               audit_seed, labels, stats, st);
+    });
 }
 } // extern "C"

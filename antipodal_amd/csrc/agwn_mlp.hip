@@ -1,0 +1,1153 @@
+// Learned inside/outside surrogate: a per-mesh Fourier-feature MLP fitted to exact GWN values, with hand-written BF16
+// MFMA kernels (FP32 accumulation) for the forward pass, the backward pass and Adam, and a fused inference kernel that
+// labels points and routes the uncertain ones to the exact path.
+//
+// Network (fixed shape):
+//   u_a = (p_a - c_a) / h_a                     c, h: centre and half-extent of the training box, per axis
+//   features f[2j], f[2j+1] = sin(pi w_k u_a), cos(pi w_k u_a),  j = 32 a + k,  w_k = F^(k/31)   (192 features;
+//                                                                               F = max_freq, 128 by default)
+//   z_l = H_{l-1} W_l + b_l,  H_l = silu(z_l)   l = 0..3, 256 units, H_{-1} = f
+//   pred = H_3 . wo + bo
+// Rounding points (what the kernels store, and what the numerics tests emulate): features, weights W_l, activations
+// H_l and backward activations dz_l are BF16; products accumulate in FP32; z_l, biases, wo, bo, the loss and every
+// gradient are FP32.
+//
+// MFMA: v_mfma_f32_32x32x16_bf16. Lane l holds A[row = l%32][k = 8(l/32) .. +8] and B[k = 8(l/32) .. +8][col = l%32];
+// accumulator e of lane l is C[8(e/4) + 4(l/32) + e%4][l%32]. Every GEMM here is written as C[r][c] = sum_k A[r][k]
+// B[c][k] with both operands k-contiguous ("row" arrays), so each lane loads 16 contiguous bytes per operand.
+//
+// Parameter vector (FP32 master copy, in this order): W0 [192 x 256], b0 [256], W1..W3 [256 x 256] with their biases,
+// wo [256], bo. W_l[k][n] maps input k to unit n. BF16 copies Wr_l (= W_l, row-major) and Wt_l ([n][k]) feed MFMA.
+
+#include "../../include/agwn.h"
+#include "agwn_internal.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <vector>
+
+namespace agwn
+{
+namespace
+{
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(__bf16)))) __bf16 bf16x4;
+typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
+
+constexpr int kFreq = 32;
+constexpr int kIn = 3 * kFreq * 2; // 192
+constexpr int kH = 256;
+constexpr int kL = 4; // hidden layers
+constexpr int kB = 256;
+
+constexpr int64_t kInDim[kL] = {kIn, kH, kH, kH};
+
+struct layout
+{
+    int64_t w[kL], b[kL], wo, bo, total;
+    constexpr layout() : w{}, b{}, wo(0), bo(0), total(0)
+    {
+        int64_t o = 0;
+        for (int l = 0; l < kL; ++l)
+        {
+            w[l] = o;
+            o += kInDim[l] * kH;
+            b[l] = o;
+            o += kH;
+        }
+        wo = o;
+        o += kH;
+        bo = o;
+        o += 1;
+        total = o;
+    }
+};
+constexpr layout kLay{};
+
+// feature map of one MLP (the training box and the frequencies)
+struct feat_params
+{
+    float c[3], inv_h[3];
+    float w[kFreq];
+};
+
+__device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
+__device__ __forceinline__ float silu_grad(float z)
+{
+    float const s = 1.0f / (1.0f + __expf(-z));
+    return s * (1.0f + z * (1.0f - s));
+}
+
+// feature pair j (0..95) of a point: axis a = j / 32, frequency k = j % 32
+__device__ __forceinline__ void feature_pair(feat_params const& F, float x, float y, float z, int j, float& s, float& c)
+{
+    int const a = j / kFreq, k = j % kFreq;
+    float const p = a == 0 ? x : (a == 1 ? y : z);
+    float const u = (p - F.c[a]) * F.inv_h[a];
+    sincospif(F.w[k] * u, &s, &c);
+}
+
+// Training features: row m of X [rows x 192] and column m of XT [192 x rows] for sample perm[m] (m < nvalid; rows past
+// nvalid are zero).
+__global__ void __launch_bounds__(kB) k_features(float const* __restrict__ pts, int64_t const* __restrict__ perm,
+                                                 int64_t nvalid, int64_t rows, feat_params F, __bf16* __restrict__ X,
+                                                 __bf16* __restrict__ XT)
+{
+    int64_t const t = int64_t(blockIdx.x) * kB + threadIdx.x;
+    if (t >= rows * (kIn / 2))
+        return;
+    // consecutive threads take consecutive samples of one feature pair: the XT stores coalesce
+    int const j = int(t / rows);
+    int64_t const m = t % rows;
+    float s = 0.0f, c = 0.0f;
+    if (m < nvalid)
+    {
+        int64_t const src = perm ? perm[m] : m;
+        feature_pair(F, pts[3 * src], pts[3 * src + 1], pts[3 * src + 2], j, s, c);
+    }
+    __bf16 const bs = __bf16(s), bc = __bf16(c);
+    X[m * kIn + 2 * j] = bs;
+    X[m * kIn + 2 * j + 1] = bc;
+    XT[int64_t(2 * j) * rows + m] = bs;
+    XT[int64_t(2 * j + 1) * rows + m] = bc;
+}
+
+// ---------------------------------------------------------------------------------------------------------------------
+// Training GEMM: C[r][c] = sum_{k in chunk} A[r*lda + k] * B[c*ldb + k], r < R, c < Cn. Block tile 128 x 128 (4 waves,
+// 2 x 2, each 64 x 64 = 2 x 2 MFMA blocks); R, Cn multiples of 64, K chunk a multiple of 16. blockIdx.x tiles c,
+// blockIdx.y tiles r, blockIdx.z splits K (partial sums, EPI_PART). The epilogue gets each lane's runs of 4
+// consecutive rows r..r+3 at one column c.
+// ---------------------------------------------------------------------------------------------------------------------
+enum epi_kind
+{
+    EPI_FWD,  // r = unit n, c = sample m: z = acc + bias[n] -> Z[m][n] (FP32), H[m][n], HT[n][m] = bf16(silu(z))
+    EPI_DX,   // r = input k, c = sample m: dz = acc * silu'(Z[m][k]) -> D[m][k], DT[k][m] (BF16)
+    EPI_PART, // r = input k, c = unit n: P[blockIdx.z][k][n] = acc
+};
+
+struct epi_args
+{
+    float const* bias;
+    float const* zin; // EPI_DX: pre-activations of the layer below [rows x R]
+    float* zout;      // EPI_FWD
+    __bf16* h;        // EPI_FWD: H / EPI_DX: D, [rows x R]
+    __bf16* ht;       // transposed copy [R x rows]
+    float* part;      // EPI_PART
+    int64_t rows;     // samples in the batch (EPI_FWD / EPI_DX)
+};
+
+template <int EPI>
+__global__ void __launch_bounds__(256)
+    k_gemm(__bf16 const* __restrict__ A, int64_t lda, __bf16 const* __restrict__ B, int64_t ldb, int64_t R, int64_t Cn,
+           int64_t K, int64_t kchunk, epi_args E)
+{
+    int const lane = threadIdx.x & 63;
+    int const wave = threadIdx.x >> 6;
+    int64_t const r0 = int64_t(blockIdx.y) * 128 + (wave >> 1) * 64;
+    int64_t const c0 = int64_t(blockIdx.x) * 128 + (wave & 1) * 64;
+    if (r0 >= R || c0 >= Cn)
+        return;
+    int64_t const kb = int64_t(blockIdx.z) * kchunk;
+    int64_t const ke = kb + kchunk < K ? kb + kchunk : K;
+    int const li = lane & 31;
+    int const kh = (lane >> 5) * 8;
+    __bf16 const* pa[2] = {A + (r0 + li) * lda + kh, A + (r0 + 32 + li) * lda + kh};
+    __bf16 const* pb[2] = {B + (c0 + li) * ldb + kh, B + (c0 + 32 + li) * ldb + kh};
+    f32x16 acc[2][2] = {};
+    bf16x8 a[2], b[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+    {
+        a[i] = *reinterpret_cast<bf16x8 const*>(pa[i] + kb);
+        b[i] = *reinterpret_cast<bf16x8 const*>(pb[i] + kb);
+    }
+    for (int64_t k0 = kb; k0 < ke; k0 += 16)
+    {
+        // prefetch the next k step (the last iteration re-reads its own step)
+        int64_t const kn = k0 + 16 < ke ? k0 + 16 : k0;
+        bf16x8 an[2], bn[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+        {
+            an[i] = *reinterpret_cast<bf16x8 const*>(pa[i] + kn);
+            bn[i] = *reinterpret_cast<bf16x8 const*>(pb[i] + kn);
+        }
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+        {
+            a[i] = an[i];
+            b[i] = bn[i];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+            {
+                int64_t const r = r0 + i * 32 + 8 * q + 4 * (lane >> 5); // rows r .. r+3
+                int64_t const c = c0 + j * 32 + li;
+                float v[4];
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    v[e] = acc[i][j][4 * q + e];
+                if constexpr (EPI == EPI_FWD)
+                {
+                    float z[4];
+                    bf16x4 hv;
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+                    {
+                        z[e] = v[e] + E.bias[r + e];
+                        hv[e] = __bf16(silu(z[e]));
+                    }
+                    *reinterpret_cast<float4*>(E.zout + c * R + r) = make_float4(z[0], z[1], z[2], z[3]);
+                    *reinterpret_cast<bf16x4*>(E.h + c * R + r) = hv;
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+                        E.ht[(r + e) * E.rows + c] = hv[e];
+                }
+                else if constexpr (EPI == EPI_DX)
+                {
+                    float4 const zz = *reinterpret_cast<float4 const*>(E.zin + c * R + r);
+                    float const zv[4] = {zz.x, zz.y, zz.z, zz.w};
+                    bf16x4 dv;
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+                        dv[e] = __bf16(v[e] * silu_grad(zv[e]));
+                    *reinterpret_cast<bf16x4*>(E.h + c * R + r) = dv;
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+                        E.ht[(r + e) * E.rows + c] = dv[e];
+                }
+                else
+                {
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+                        E.part[(int64_t(blockIdx.z) * R + r + e) * Cn + c] = v[e];
+                }
+            }
+}
+
+// out[i] = sum over p of part[p * count + i], in a fixed order (deterministic)
+__global__ void __launch_bounds__(kB) k_sum_parts(float const* __restrict__ part, int64_t parts, int64_t count,
+                                                  float* __restrict__ out)
+{
+    int64_t const i = int64_t(blockIdx.x) * kB + threadIdx.x;
+    if (i >= count)
+        return;
+    float s = 0.0f;
+    for (int64_t p = 0; p < parts; ++p)
+        s += part[p * count + i];
+    out[i] = s;
+}
+
+// Bias gradient: out[n] = sum_m DT[n][m] (one block per unit n)
+__global__ void __launch_bounds__(kB) k_rowsum(__bf16 const* __restrict__ DT, int64_t rows, float* __restrict__ out)
+{
+    __shared__ float red[kB / 64];
+    __bf16 const* row = DT + int64_t(blockIdx.x) * rows;
+    float s = 0.0f;
+    for (int64_t m = threadIdx.x; m < rows; m += kB)
+        s += float(row[m]);
+    for (int off = 32; off > 0; off >>= 1)
+        s += __shfl_xor(s, off);
+    if ((threadIdx.x & 63) == 0)
+        red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        out[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+constexpr int64_t kObl = 64; // blocks of the output-layer kernel
+
+// Output layer and loss. Wave-per-sample; lane l covers units 4l .. 4l+3.
+//   pred = H3 . wo + bo;  g = 2 (pred - y) / nvalid;  dz3 = bf16(g wo silu'(Z3));  per-block partials of g H3 (dwo),
+//   g (dbo) and (pred - y)^2.
+// y is read through perm (sample order of the batch). Without D only pred is written.
+__global__ void __launch_bounds__(kB)
+    k_out_loss(__bf16 const* __restrict__ H, float const* __restrict__ Z, float const* __restrict__ wo,
+               float const* __restrict__ bo_p, float const* __restrict__ y, int64_t const* __restrict__ perm,
+               int64_t nvalid, int64_t rows, float* __restrict__ pred, __bf16* __restrict__ D, __bf16* __restrict__ DT,
+               float* __restrict__ part_wo, float* __restrict__ part_sc)
+{
+    int const lane = threadIdx.x & 63;
+    int const wave = threadIdx.x >> 6;
+    int64_t const per = (rows + kObl - 1) / kObl;
+    int64_t const r0 = int64_t(blockIdx.x) * per;
+    int64_t const r1 = r0 + per < rows ? r0 + per : rows;
+    float w[4];
+    for (int j = 0; j < 4; ++j)
+        w[j] = wo[4 * lane + j];
+    float const bo = *bo_p;
+    float const inv = 1.0f / float(nvalid);
+    float gw[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    float gb = 0.0f, loss = 0.0f;
+    for (int64_t m = r0 + wave; m < r1; m += 4)
+    {
+        bf16x4 const hv = *reinterpret_cast<bf16x4 const*>(H + m * kH + 4 * lane);
+        float h[4];
+        float s = 0.0f;
+        for (int j = 0; j < 4; ++j)
+        {
+            h[j] = float(hv[j]);
+            s += h[j] * w[j];
+        }
+        for (int off = 32; off > 0; off >>= 1)
+            s += __shfl_xor(s, off);
+        float const p = s + bo;
+        bool const valid = m < nvalid;
+        if (pred && lane == 0 && valid)
+            pred[m] = p;
+        if (!D)
+            continue;
+        float const target = valid ? y[perm ? perm[m] : m] : p;
+        float const g = valid ? 2.0f * (p - target) * inv : 0.0f;
+        float4 const zz = *reinterpret_cast<float4 const*>(Z + m * kH + 4 * lane);
+        float const zv[4] = {zz.x, zz.y, zz.z, zz.w};
+        bf16x4 dv;
+        for (int j = 0; j < 4; ++j)
+        {
+            dv[j] = __bf16(g * w[j] * silu_grad(zv[j]));
+            gw[j] += g * h[j];
+        }
+        *reinterpret_cast<bf16x4*>(D + m * kH + 4 * lane) = dv;
+        for (int j = 0; j < 4; ++j)
+            DT[int64_t(4 * lane + j) * rows + m] = dv[j];
+        gb += g;
+        loss += valid ? (p - target) * (p - target) : 0.0f;
+    }
+    if (!D)
+        return;
+    __shared__ float sw[4][kH];
+    __shared__ float ss[4][2];
+    for (int j = 0; j < 4; ++j)
+        sw[wave][4 * lane + j] = gw[j];
+    if (lane == 0)
+    {
+        ss[wave][0] = gb;
+        ss[wave][1] = loss;
+    }
+    __syncthreads();
+    int const t = threadIdx.x;
+    part_wo[int64_t(blockIdx.x) * kH + t] = (sw[0][t] + sw[1][t]) + (sw[2][t] + sw[3][t]);
+    if (t < 2)
+        part_sc[int64_t(blockIdx.x) * 2 + t] = (ss[0][t] + ss[1][t]) + (ss[2][t] + ss[3][t]);
+}
+
+// Reduces the output-layer partials: g_wo, g_bo, and adds the batch's squared-error sum to loss_acc (double).
+__global__ void __launch_bounds__(kB) k_out_final(float const* __restrict__ part_wo, float const* __restrict__ part_sc,
+                                                  float* __restrict__ g_wo, float* __restrict__ g_bo,
+                                                  double* __restrict__ loss_acc)
+{
+    int const t = threadIdx.x;
+    float s = 0.0f;
+    for (int64_t p = 0; p < kObl; ++p)
+        s += part_wo[p * kH + t];
+    g_wo[t] = s;
+    if (t == 0)
+    {
+        float gb = 0.0f, l = 0.0f;
+        for (int64_t p = 0; p < kObl; ++p)
+        {
+            gb += part_sc[2 * p];
+            l += part_sc[2 * p + 1];
+        }
+        *g_bo = gb;
+        if (loss_acc)
+            *loss_acc += double(l);
+    }
+}
+
+struct wcopies
+{
+    __bf16* wr[kL];
+    __bf16* wt[kL];
+};
+
+__device__ __forceinline__ void store_bf16_copies(wcopies const& C, int64_t i, float v)
+{
+#pragma unroll
+    for (int l = 0; l < kL; ++l)
+    {
+        int64_t const o = kLay.w[l];
+        int64_t const K = kInDim[l];
+        if (i >= o && i < o + K * kH)
+        {
+            int64_t const k = (i - o) / kH, n = (i - o) % kH;
+            __bf16 const b = __bf16(v);
+            C.wr[l][k * kH + n] = b;
+            C.wt[l][n * K + k] = b;
+        }
+    }
+}
+
+// Adam on the FP32 master copy (bias-corrected: c1 = 1/(1-b1^t), c2 = 1/(1-b2^t)), then the BF16 operand copies.
+__global__ void __launch_bounds__(kB) k_adam(float* __restrict__ w, float* __restrict__ m1, float* __restrict__ m2,
+                                             float const* __restrict__ g, int64_t n, float lr, float b1, float b2,
+                                             float eps, float c1, float c2, wcopies C)
+{
+    int64_t const i = int64_t(blockIdx.x) * kB + threadIdx.x;
+    if (i >= n)
+        return;
+    float const gi = g[i];
+    float const a = b1 * m1[i] + (1.0f - b1) * gi;
+    float const v = b2 * m2[i] + (1.0f - b2) * gi * gi;
+    m1[i] = a;
+    m2[i] = v;
+    float const wn = w[i] - lr * (a * c1) / (sqrtf(v * c2) + eps);
+    w[i] = wn;
+    store_bf16_copies(C, i, wn);
+}
+
+__global__ void __launch_bounds__(kB) k_cast(float const* __restrict__ w, int64_t n, wcopies C)
+{
+    int64_t const i = int64_t(blockIdx.x) * kB + threadIdx.x;
+    if (i < n)
+        store_bf16_copies(C, i, w[i]);
+}
+
+// sample order of a batch: row j -> (offset + (b0 + j) * stride) mod n with stride coprime to n (a permutation per
+// epoch)
+__global__ void __launch_bounds__(kB) k_perm(int64_t* __restrict__ perm, int64_t rows, int64_t b0, int64_t offset,
+                                             int64_t stride, int64_t n)
+{
+    int64_t const j = int64_t(blockIdx.x) * kB + threadIdx.x;
+    if (j >= rows)
+        return;
+    unsigned __int128 const t = (unsigned __int128)(uint64_t(b0 + j)) * uint64_t(stride) + uint64_t(offset);
+    perm[j] = int64_t(uint64_t(t % uint64_t(n)));
+}
+
+// ---------------------------------------------------------------------------------------------------------------------
+// Fused inference: a workgroup takes 128 points through the features, the 4 hidden layers and the output layer with
+// the activations in one LDS buffer (128 x 264 BF16, 66 KiB: two workgroups per CU). Each layer computes
+// out^T[n][m] = sum_k Wt[n][k] act[m][k]: wave w owns units [64w, 64w + 64) (A operand = Wt rows, from L2) for all 128
+// points (B operand = LDS rows), so a lane's 4 consecutive accumulators are 4 consecutive units of one point and go
+// back to LDS as one 8-byte write after a barrier. The output layer is a 256-long dot per point.
+// Optionally labels the points (pred > 0.5) and appends those with |pred - 0.5| <= delta (and a hashed audit sample
+// of the rest) to a routing list: entry = 2 * index + (audit ? 1 : 0).
+// ---------------------------------------------------------------------------------------------------------------------
+constexpr int kTileP = 128;
+constexpr int kLdsStride = kH + 8; // BF16 per LDS row: 528 B rows keep the 16-lane groups of ds_read_b128 conflict-free
+
+struct infer_params
+{
+    __bf16 const* wt[kL];
+    float const* b[kL];
+    float const* wo;
+    float const* bo;
+};
+
+struct route_params
+{
+    float delta;
+    uint32_t audit_mod; // 0: no audit
+    uint32_t audit_seed;
+    int8_t* labels;
+    int64_t* list;
+    unsigned long long* counters; // [0] band, [1] audit, [2] list length, [3] audit disagreements
+};
+
+__device__ __forceinline__ uint32_t hash32(uint64_t x)
+{
+    x ^= x >> 33;
+    x *= 0xFF51AFD7ED558CCDull;
+    x ^= x >> 33;
+    x *= 0xC4CEB9FE1A85EC53ull;
+    x ^= x >> 33;
+    return uint32_t(x);
+}
+
+template <class Src, bool ROUTE>
+__global__ void __launch_bounds__(256, 2) k_mlp_infer(Src const src, int64_t n, feat_params F, infer_params P,
+                                                      float* __restrict__ pred, route_params RP)
+{
+    __shared__ __attribute__((aligned(16))) __bf16 act[kTileP * kLdsStride];
+    __shared__ float predl[kTileP];
+    int const lane = threadIdx.x & 63;
+    int const wave = threadIdx.x >> 6;
+    int64_t const base = int64_t(blockIdx.x) * kTileP;
+
+    for (int t = threadIdx.x; t < kTileP * (kIn / 2); t += 256)
+    {
+        int const p = t % kTileP, j = t / kTileP;
+        float s = 0.0f, c = 0.0f;
+        int64_t const i = base + p;
+        if (i < n)
+        {
+            float x, y, z;
+            query_at(src, i, x, y, z);
+            feature_pair(F, x, y, z, j, s, c);
+        }
+        act[p * kLdsStride + 2 * j] = __bf16(s);
+        act[p * kLdsStride + 2 * j + 1] = __bf16(c);
+    }
+    __syncthreads();
+
+    int const li = lane & 31;
+    int const kh = (lane >> 5) * 8;
+    int const n0 = wave * 64;
+#pragma unroll 1
+    for (int l = 0; l < kL; ++l)
+    {
+        int const K = l == 0 ? kIn : kH;
+        __bf16 const* wa0 = P.wt[l] + int64_t(n0 + li) * K + kh;
+        __bf16 const* wa1 = wa0 + int64_t(32) * K;
+        f32x16 acc[2][4] = {};
+        bf16x8 a0 = *reinterpret_cast<bf16x8 const*>(wa0);
+        bf16x8 a1 = *reinterpret_cast<bf16x8 const*>(wa1);
+        for (int k0 = 0; k0 < K; k0 += 16)
+        {
+            int const kn = k0 + 16 < K ? k0 + 16 : k0;
+            bf16x8 const na0 = *reinterpret_cast<bf16x8 const*>(wa0 + kn);
+            bf16x8 const na1 = *reinterpret_cast<bf16x8 const*>(wa1 + kn);
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+            {
+                bf16x8 const b = *reinterpret_cast<bf16x8 const*>(act + (j * 32 + li) * kLdsStride + k0 + kh);
+                acc[0][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b, acc[0][j], 0, 0, 0);
+                acc[1][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b, acc[1][j], 0, 0, 0);
+            }
+            a0 = na0;
+            a1 = na1;
+        }
+        __syncthreads(); // every wave has read the layer input
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+            {
+                int const u = n0 + i * 32 + 8 * q + 4 * (lane >> 5); // units u .. u+3
+                float const b0 = P.b[l][u], b1 = P.b[l][u + 1], b2 = P.b[l][u + 2], b3 = P.b[l][u + 3];
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                {
+                    bf16x4 hv;
+                    hv[0] = __bf16(silu(acc[i][j][4 * q] + b0));
+                    hv[1] = __bf16(silu(acc[i][j][4 * q + 1] + b1));
+                    hv[2] = __bf16(silu(acc[i][j][4 * q + 2] + b2));
+                    hv[3] = __bf16(silu(acc[i][j][4 * q + 3] + b3));
+                    *reinterpret_cast<bf16x4*>(act + (j * 32 + li) * kLdsStride + u) = hv;
+                }
+            }
+        __syncthreads();
+    }
+    // output layer, summed in the order of the training kernel (4 products per lane, then a butterfly)
+    float w[4];
+    for (int j = 0; j < 4; ++j)
+        w[j] = P.wo[4 * lane + j];
+    float const bo = *P.bo;
+    for (int p = wave; p < kTileP; p += 4)
+    {
+        bf16x4 const hv = *reinterpret_cast<bf16x4 const*>(act + p * kLdsStride + 4 * lane);
+        float s = 0.0f;
+        for (int j = 0; j < 4; ++j)
+            s += float(hv[j]) * w[j];
+        for (int off = 32; off > 0; off >>= 1)
+            s += __shfl_xor(s, off);
+        if (lane == 0)
+            predl[p] = s + bo;
+    }
+    __syncthreads();
+    if (threadIdx.x >= kTileP)
+        return;
+    int64_t const i = base + threadIdx.x;
+    bool const live = i < n;
+    float const pv = predl[threadIdx.x];
+    if (pred && live)
+        pred[i] = pv;
+    if constexpr (ROUTE)
+    {
+        float const d = fabsf(pv - 0.5f);
+        bool const band = live && !(d > RP.delta); // NaN routes too
+        bool const audit = live && !band && RP.audit_mod != 0 &&
+                           hash32(uint64_t(i) * 0x9E3779B97F4A7C15ull + RP.audit_seed) % RP.audit_mod == 0;
+        if (live)
+            RP.labels[i] = pv > 0.5f ? 1 : 0;
+        bool const take = band || audit;
+        uint64_t const mask = __ballot(take);
+        uint64_t const mband = __ballot(band);
+        if (mask == 0)
+            return;
+        int const leader = __ffsll((unsigned long long)mask) - 1;
+        unsigned long long basei = 0;
+        if (lane == leader)
+        {
+            basei = atomicAdd(RP.counters + 2, (unsigned long long)__popcll(mask));
+            atomicAdd(RP.counters + 0, (unsigned long long)__popcll(mband));
+            if (mask != mband)
+                atomicAdd(RP.counters + 1, (unsigned long long)__popcll(mask & ~mband));
+        }
+        basei = __shfl(basei, leader);
+        if (take)
+        {
+            int const rank = __popcll(mask & ((1ull << lane) - 1ull));
+            RP.list[basei + rank] = 2 * i + (audit ? 1 : 0);
+        }
+    }
+}
+
+// Applies the exact labels of the routed points: w[e] is the exact GWN of list entry e. Audit entries whose
+// surrogate label disagrees are counted in counters[3].
+__global__ void __launch_bounds__(kB) k_apply(int64_t const* __restrict__ list, int64_t m, float const* __restrict__ w,
+                                              int8_t* __restrict__ labels, unsigned long long* __restrict__ counters)
+{
+    int64_t const e = int64_t(blockIdx.x) * kB + threadIdx.x;
+    if (e >= m)
+        return;
+    int64_t const code = list[e];
+    int64_t const i = code >> 1;
+    int8_t const lab = w[e] > 0.5f ? 1 : 0;
+    if ((code & 1) && labels[i] != lab)
+        atomicAdd(counters + 3, 1ull);
+    labels[i] = lab;
+}
+
+// gather of routed points: pts[e] = the query of list entry e
+template <class Src>
+__global__ void __launch_bounds__(kB) k_gather_routed(Src const src, int64_t const* __restrict__ list, int64_t m,
+                                                      float* __restrict__ pts)
+{
+    int64_t const e = int64_t(blockIdx.x) * kB + threadIdx.x;
+    if (e >= m)
+        return;
+    float x, y, z;
+    query_at(src, list[e] >> 1, x, y, z);
+    pts[3 * e] = x;
+    pts[3 * e + 1] = y;
+    pts[3 * e + 2] = z;
+}
+} // namespace
+
+// ---------------------------------------------------------------------------------------------------------------------
+// the model object
+// ---------------------------------------------------------------------------------------------------------------------
+struct mlp_impl
+{
+    feat_params F{};
+    float lo[3]{}, hi[3]{};
+    float max_freq = 0.0f;
+    float* w = nullptr;  // FP32 master parameters
+    float* m1 = nullptr; // Adam moments
+    float* m2 = nullptr;
+    float* g = nullptr; // gradient
+    __bf16* wr[kL] = {};
+    __bf16* wt[kL] = {};
+    int64_t step = 0;
+    std::mutex mu;
+
+    mlp_impl() = default;
+    mlp_impl(mlp_impl const&) = delete;
+    ~mlp_impl()
+    {
+        for (float* p : {w, m1, m2, g})
+            if (p)
+                (void)hipFree(p);
+        for (int l = 0; l < kL; ++l)
+        {
+            if (wr[l])
+                (void)hipFree(wr[l]);
+            if (wt[l])
+                (void)hipFree(wt[l]);
+        }
+    }
+
+    int alloc()
+    {
+        size_t const P = size_t(kLay.total) * sizeof(float);
+        AGWN_HIP(hipMalloc(&w, P));
+        AGWN_HIP(hipMalloc(&m1, P));
+        AGWN_HIP(hipMalloc(&m2, P));
+        AGWN_HIP(hipMalloc(&g, P));
+        AGWN_HIP(hipMemset(m1, 0, P));
+        AGWN_HIP(hipMemset(m2, 0, P));
+        AGWN_HIP(hipMemset(g, 0, P));
+        for (int l = 0; l < kL; ++l)
+        {
+            AGWN_HIP(hipMalloc(&wr[l], size_t(kInDim[l] * kH) * sizeof(__bf16)));
+            AGWN_HIP(hipMalloc(&wt[l], size_t(kInDim[l] * kH) * sizeof(__bf16)));
+        }
+        return ok;
+    }
+
+    wcopies copies() const
+    {
+        wcopies C{};
+        for (int l = 0; l < kL; ++l)
+        {
+            C.wr[l] = wr[l];
+            C.wt[l] = wt[l];
+        }
+        return C;
+    }
+
+    int cast(hipStream_t s)
+    {
+        k_cast<<<dim3(unsigned((kLay.total + kB - 1) / kB)), dim3(kB), 0, s>>>(w, kLay.total, copies());
+        AGWN_HIP(hipGetLastError());
+        return ok;
+    }
+
+    infer_params iparams() const
+    {
+        infer_params P{};
+        for (int l = 0; l < kL; ++l)
+        {
+            P.wt[l] = wt[l];
+            P.b[l] = w + kLay.b[l];
+        }
+        P.wo = w + kLay.wo;
+        P.bo = w + kLay.bo;
+        return P;
+    }
+};
+} // namespace agwn
+
+struct agwn_mlp
+{
+    agwn::mlp_impl impl;
+};
+
+namespace agwn
+{
+namespace
+{
+uint64_t splitmix(uint64_t& s)
+{
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+feat_params make_feats(float const* lo, float const* hi, float max_freq)
+{
+    feat_params F{};
+    for (int a = 0; a < 3; ++a)
+    {
+        F.c[a] = 0.5f * (lo[a] + hi[a]);
+        F.inv_h[a] = 1.0f / (0.5f * (hi[a] - lo[a]));
+    }
+    for (int k = 0; k < kFreq; ++k)
+        F.w[k] = float(std::exp2(std::log2(double(max_freq)) * double(k) / double(kFreq - 1)));
+    return F;
+}
+
+// Scratch of one forward/backward pass over `rows` samples.
+struct train_ws
+{
+    int64_t rows = 0, parts = 0, kchunk = 0;
+    dbuf X0, X0T;       // features [rows x 192], [192 x rows]
+    dbuf H[kL], HT[kL]; // BF16 activations and their transposes
+    dbuf Z[kL];         // FP32 pre-activations [rows x 256]
+    dbuf D[2], DT[2];   // BF16 backward activations (ping-pong over layers)
+    dbuf part, pwo, psc, perm, loss;
+    int alloc(int64_t r, hipStream_t s)
+    {
+        rows = r;
+        // split the batch reduction of dW so that its launch has >= 256 workgroups
+        kchunk = std::max<int64_t>(256, ((rows / 64) + 15) / 16 * 16);
+        parts = (rows + kchunk - 1) / kchunk;
+        size_t const R = size_t(rows);
+        AGWN_TRY(X0.alloc(R * kIn * sizeof(__bf16), s));
+        AGWN_TRY(X0T.alloc(R * kIn * sizeof(__bf16), s));
+        for (int l = 0; l < kL; ++l)
+        {
+            AGWN_TRY(H[l].alloc(R * kH * sizeof(__bf16), s));
+            AGWN_TRY(HT[l].alloc(R * kH * sizeof(__bf16), s));
+            AGWN_TRY(Z[l].alloc(R * kH * sizeof(float), s));
+        }
+        for (int i = 0; i < 2; ++i)
+        {
+            AGWN_TRY(D[i].alloc(R * kH * sizeof(__bf16), s));
+            AGWN_TRY(DT[i].alloc(R * kH * sizeof(__bf16), s));
+        }
+        AGWN_TRY(part.alloc(size_t(parts) * kH * kH * sizeof(float), s));
+        AGWN_TRY(pwo.alloc(size_t(kObl) * kH * sizeof(float), s));
+        AGWN_TRY(psc.alloc(size_t(kObl) * 2 * sizeof(float), s));
+        AGWN_TRY(perm.alloc(R * sizeof(int64_t), s));
+        AGWN_TRY(loss.alloc(sizeof(double), s));
+        return ok;
+    }
+};
+
+// Forward (and, with y, backward) pass of the rows whose features are in ws.X0. The gradient lands in M.g, the
+// batch's squared-error sum is added to *ws.loss.
+int pass(mlp_impl& M, train_ws& ws, int64_t nvalid, float const* y, int64_t const* perm, float* pred, hipStream_t s)
+{
+    int64_t const rows = ws.rows;
+    dim3 const blk(256);
+    bool const bwd = y != nullptr;
+    for (int l = 0; l < kL; ++l)
+    {
+        int64_t const K = kInDim[l];
+        __bf16 const* x = l == 0 ? ws.X0.as<__bf16>() : ws.H[l - 1].as<__bf16>();
+        epi_args E{};
+        E.bias = M.w + kLay.b[l];
+        E.zout = ws.Z[l].as<float>();
+        E.h = ws.H[l].as<__bf16>();
+        E.ht = ws.HT[l].as<__bf16>();
+        E.rows = rows;
+        k_gemm<EPI_FWD><<<dim3(unsigned(rows / 128), kH / 128, 1), blk, 0, s>>>(M.wt[l], K, x, K, kH, rows, K, K, E);
+    }
+    int const top = (kL - 1) & 1;
+    k_out_loss<<<dim3(unsigned(kObl)), blk, 0, s>>>(
+        ws.H[kL - 1].as<__bf16>(), ws.Z[kL - 1].as<float>(), M.w + kLay.wo, M.w + kLay.bo, y, perm, nvalid, rows, pred,
+        bwd ? ws.D[top].as<__bf16>() : nullptr, bwd ? ws.DT[top].as<__bf16>() : nullptr, ws.pwo.as<float>(),
+        ws.psc.as<float>());
+    AGWN_HIP(hipGetLastError());
+    if (!bwd)
+        return ok;
+    k_out_final<<<dim3(1), blk, 0, s>>>(ws.pwo.as<float>(), ws.psc.as<float>(), M.g + kLay.wo, M.g + kLay.bo,
+                                        ws.loss.as<double>());
+    for (int l = kL - 1; l >= 0; --l)
+    {
+        int64_t const K = kInDim[l];
+        __bf16 const* xt = l == 0 ? ws.X0T.as<__bf16>() : ws.HT[l - 1].as<__bf16>();
+        __bf16 const* dt = ws.DT[l & 1].as<__bf16>();
+        // dW_l[k][n] = sum_m XT_l[k][m] DT_l[n][m], split over the batch, partials summed in a fixed order
+        epi_args E{};
+        E.part = ws.part.as<float>();
+        k_gemm<EPI_PART><<<dim3(kH / 128, unsigned((K + 127) / 128), unsigned(ws.parts)), blk, 0, s>>>(
+            xt, rows, dt, rows, K, kH, rows, ws.kchunk, E);
+        k_sum_parts<<<dim3(unsigned((K * kH + kB - 1) / kB)), blk, 0, s>>>(ws.part.as<float>(), ws.parts, K * kH,
+                                                                           M.g + kLay.w[l]);
+        k_rowsum<<<dim3(kH), blk, 0, s>>>(dt, rows, M.g + kLay.b[l]);
+        if (l > 0)
+        {
+            // D_{l-1}[m][k] = (sum_n Wr_l[k][n] D_l[m][n]) silu'(Z_{l-1}[m][k])
+            epi_args E2{};
+            E2.zin = ws.Z[l - 1].as<float>();
+            E2.h = ws.D[(l - 1) & 1].as<__bf16>();
+            E2.ht = ws.DT[(l - 1) & 1].as<__bf16>();
+            E2.rows = rows;
+            k_gemm<EPI_DX><<<dim3(unsigned(rows / 128), unsigned((K + 127) / 128), 1), blk, 0, s>>>(
+                M.wr[l], kH, ws.D[l & 1].as<__bf16>(), kH, K, rows, kH, kH, E2);
+        }
+    }
+    AGWN_HIP(hipGetLastError());
+    return ok;
+}
+
+int64_t gcd64(int64_t a, int64_t b)
+{
+    while (b)
+    {
+        int64_t const t = a % b;
+        a = b;
+        b = t;
+    }
+    return a;
+}
+
+int mlp_create(float const* lo, float const* hi, float max_freq, uint64_t seed, agwn_mlp** out)
+{
+    AGWN_CHECK(out && lo && hi, "null argument");
+    *out = nullptr;
+    AGWN_CHECK(std::isfinite(max_freq) && max_freq >= 1.0f && max_freq <= 4096.0f,
+               "max_freq must be in [1, 4096] (got %g)", double(max_freq));
+    for (int a = 0; a < 3; ++a)
+        AGWN_CHECK(std::isfinite(lo[a]) && std::isfinite(hi[a]) && hi[a] > lo[a], "empty or non-finite box on axis %d",
+                   a);
+    AGWN_TRY(ensure_device());
+    auto h = std::make_unique<agwn_mlp>();
+    mlp_impl& M = h->impl;
+    M.F = make_feats(lo, hi, max_freq);
+    M.max_freq = max_freq;
+    for (int a = 0; a < 3; ++a)
+    {
+        M.lo[a] = lo[a];
+        M.hi[a] = hi[a];
+    }
+    AGWN_TRY(M.alloc());
+    // He-uniform hidden layers, small output layer, zero biases, bo = 0.5 (the undecided value)
+    std::vector<float> w(size_t(kLay.total), 0.0f);
+    uint64_t st = seed ^ 0x6A09E667F3BCC908ull;
+    auto uni = [&] { return float(double(splitmix(st) >> 11) * 0x1p-53) * 2.0f - 1.0f; };
+    for (int l = 0; l < kL; ++l)
+    {
+        float const a = std::sqrt(6.0f / float(kInDim[l]));
+        for (int64_t i = 0; i < kInDim[l] * kH; ++i)
+            w[size_t(kLay.w[l] + i)] = a * uni();
+    }
+    float const ao = 0.1f * std::sqrt(6.0f / float(kH));
+    for (int j = 0; j < kH; ++j)
+        w[size_t(kLay.wo + j)] = ao * uni();
+    w[size_t(kLay.bo)] = 0.5f;
+    AGWN_HIP(hipMemcpy(M.w, w.data(), w.size() * sizeof(float), hipMemcpyHostToDevice));
+    AGWN_TRY(M.cast(nullptr));
+    AGWN_HIP(hipDeviceSynchronize());
+    *out = h.release();
+    return ok;
+}
+
+int mlp_set_params(agwn_mlp* h, float const* p)
+{
+    AGWN_CHECK(h && p, "null argument");
+    mlp_impl& M = h->impl;
+    std::lock_guard<std::mutex> lk(M.mu);
+    size_t const P = size_t(kLay.total) * sizeof(float);
+    AGWN_HIP(hipMemcpy(M.w, p, P, hipMemcpyHostToDevice));
+    AGWN_HIP(hipMemset(M.m1, 0, P));
+    AGWN_HIP(hipMemset(M.m2, 0, P));
+    M.step = 0;
+    AGWN_TRY(M.cast(nullptr));
+    AGWN_HIP(hipDeviceSynchronize());
+    return ok;
+}
+
+int mlp_get_params(agwn_mlp const* h, float* p)
+{
+    AGWN_CHECK(h && p, "null argument");
+    AGWN_HIP(hipMemcpy(p, h->impl.w, size_t(kLay.total) * sizeof(float), hipMemcpyDeviceToHost));
+    return ok;
+}
+
+int mlp_train(agwn_mlp* h, float const* pts, float const* y, int64_t n, int batch, int epochs, float lr, uint64_t seed,
+              float* loss_host, hipStream_t s)
+{
+    AGWN_CHECK(h != nullptr, "mlp handle is null");
+    AGWN_CHECK(pts && y && n > 0, "empty training set");
+    AGWN_CHECK(batch >= 128 && batch % 128 == 0 && batch <= (1 << 22),
+               "batch must be a multiple of 128 in [128, 2^22] (got %d)", batch);
+    AGWN_CHECK(epochs >= 0, "negative epochs");
+    AGWN_CHECK(std::isfinite(lr) && lr > 0.0f, "learning rate must be positive");
+    mlp_impl& M = h->impl;
+    std::lock_guard<std::mutex> lk(M.mu);
+    train_ws ws;
+    AGWN_TRY(ws.alloc(batch, s));
+    int64_t const B = batch;
+    int64_t const steps_per_epoch = (n + B - 1) / B;
+    int64_t const total = steps_per_epoch * epochs;
+    float const b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    uint64_t st = seed ^ 0xBB67AE8584CAA73Bull;
+    wcopies const C = M.copies();
+    int64_t it = 0;
+    for (int e = 0; e < epochs; ++e)
+    {
+        int64_t stride = int64_t(splitmix(st) % uint64_t(n)) | 1;
+        while (gcd64(stride, n) != 1)
+            stride += 2;
+        int64_t const offset = int64_t(splitmix(st) % uint64_t(n));
+        AGWN_HIP(hipMemsetAsync(ws.loss.p, 0, sizeof(double), s));
+        for (int64_t b = 0; b < steps_per_epoch; ++b, ++it)
+        {
+            k_perm<<<dim3(unsigned((B + kB - 1) / kB)), dim3(kB), 0, s>>>(ws.perm.as<int64_t>(), B, b * B, offset,
+                                                                         stride, n);
+            k_features<<<dim3(unsigned((B * (kIn / 2) + kB - 1) / kB)), dim3(kB), 0, s>>>(
+                pts, ws.perm.as<int64_t>(), B, B, M.F, ws.X0.as<__bf16>(), ws.X0T.as<__bf16>());
+            AGWN_HIP(hipGetLastError());
+            AGWN_TRY(pass(M, ws, B, y, ws.perm.as<int64_t>(), nullptr, s));
+            ++M.step;
+            // cosine decay from lr to lr / 20 over the run
+            double const frac = total > 1 ? double(it) / double(total - 1) : 0.0;
+            float const lr_t = float(double(lr) * (0.05 + 0.475 * (1.0 + std::cos(3.141592653589793 * frac))));
+            float const c1 = float(1.0 / (1.0 - std::pow(double(b1), double(M.step))));
+            float const c2 = float(1.0 / (1.0 - std::pow(double(b2), double(M.step))));
+            k_adam<<<dim3(unsigned((kLay.total + kB - 1) / kB)), dim3(kB), 0, s>>>(M.w, M.m1, M.m2, M.g, kLay.total,
+                                                                                   lr_t, b1, b2, eps, c1, c2, C);
+            AGWN_HIP(hipGetLastError());
+        }
+        if (loss_host)
+        {
+            double L = 0.0;
+            AGWN_HIP(hipMemcpyAsync(&L, ws.loss.p, sizeof(double), hipMemcpyDeviceToHost, s));
+            AGWN_HIP(hipStreamSynchronize(s));
+            loss_host[e] = float(L / double(steps_per_epoch * B));
+        }
+    }
+    AGWN_HIP(hipStreamSynchronize(s));
+    return ok;
+}
+
+// One forward + backward pass over host samples (no update): gradient of the mean squared error, predictions, loss.
+int mlp_grad(agwn_mlp* h, float const* pts, float const* y, int64_t n, float* grad, float* pred, float* loss)
+{
+    AGWN_CHECK(h != nullptr, "mlp handle is null");
+    AGWN_CHECK(pts && y && n > 0 && n <= (1 << 20), "n must be in [1, 2^20]");
+    AGWN_CHECK(grad || pred || loss, "no output requested");
+    mlp_impl& M = h->impl;
+    std::lock_guard<std::mutex> lk(M.mu);
+    hipStream_t s = nullptr;
+    AGWN_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    struct sguard
+    {
+        hipStream_t s;
+        ~sguard() { (void)hipStreamDestroy(s); }
+    } sg{s};
+    int64_t const rows = (n + 127) / 128 * 128;
+    {
+        train_ws ws;
+        AGWN_TRY(ws.alloc(rows, s));
+        dbuf dp, dy, dpred;
+        AGWN_TRY(dp.alloc(size_t(n) * 3 * sizeof(float), s));
+        AGWN_TRY(dy.alloc(size_t(n) * sizeof(float), s));
+        AGWN_TRY(dpred.alloc(size_t(n) * sizeof(float), s));
+        AGWN_HIP(hipMemcpyAsync(dp.p, pts, size_t(n) * 3 * sizeof(float), hipMemcpyHostToDevice, s));
+        AGWN_HIP(hipMemcpyAsync(dy.p, y, size_t(n) * sizeof(float), hipMemcpyHostToDevice, s));
+        AGWN_HIP(hipMemsetAsync(ws.loss.p, 0, sizeof(double), s));
+        k_features<<<dim3(unsigned((rows * (kIn / 2) + kB - 1) / kB)), dim3(kB), 0, s>>>(
+            dp.as<float>(), nullptr, n, rows, M.F, ws.X0.as<__bf16>(), ws.X0T.as<__bf16>());
+        AGWN_HIP(hipGetLastError());
+        AGWN_TRY(pass(M, ws, n, dy.as<float>(), nullptr, dpred.as<float>(), s));
+        if (grad)
+            AGWN_HIP(hipMemcpyAsync(grad, M.g, size_t(kLay.total) * sizeof(float), hipMemcpyDeviceToHost, s));
+        if (pred)
+            AGWN_HIP(hipMemcpyAsync(pred, dpred.p, size_t(n) * sizeof(float), hipMemcpyDeviceToHost, s));
+        double L = 0.0;
+        AGWN_HIP(hipMemcpyAsync(&L, ws.loss.p, sizeof(double), hipMemcpyDeviceToHost, s));
+        AGWN_HIP(hipStreamSynchronize(s));
+        if (loss)
+            *loss = float(L / double(n));
+    }
+    AGWN_HIP(hipStreamSynchronize(s));
+    return ok;
+}
+
+template <class Src, bool ROUTE>
+int infer_launch(mlp_impl const& M, Src const& src, int64_t n, float* pred, route_params const& RP, hipStream_t s)
+{
+    if (n == 0)
+        return ok;
+    int64_t const blocks = (n + kTileP - 1) / kTileP;
+    AGWN_CHECK(blocks < (int64_t(1) << 31), "too many points (%lld)", (long long)n);
+    k_mlp_infer<Src, ROUTE><<<dim3(unsigned(blocks)), dim3(256), 0, s>>>(src, n, M.F, M.iparams(), pred, RP);
+    AGWN_HIP(hipGetLastError());
+    return ok;
+}
+} // namespace
+
+int64_t mlp_num_params() { return kLay.total; }
+
+int mlp_predict_dev(agwn_mlp const* h, float const* pts, int64_t n, float* out, hipStream_t s)
+{
+    AGWN_CHECK(h != nullptr, "mlp handle is null");
+    AGWN_CHECK(n >= 0, "negative count");
+    AGWN_CHECK(n == 0 || (pts && out), "null array");
+    return infer_launch<src_array<float>, false>(h->impl, src_array<float>{pts}, n, out, route_params{}, s);
+}
+
+template <class Src>
+int mlp_route(agwn_mlp const* h, Src const& src, int64_t n, float delta, uint32_t audit_mod, uint32_t audit_seed,
+              int8_t* labels, int64_t* list, unsigned long long* counters, hipStream_t s)
+{
+    AGWN_CHECK(h != nullptr, "mlp handle is null");
+    route_params RP{delta, audit_mod, audit_seed, labels, list, counters};
+    return infer_launch<Src, true>(h->impl, src, n, nullptr, RP, s);
+}
+template int mlp_route<src_array<float>>(agwn_mlp const*, src_array<float> const&, int64_t, float, uint32_t,
+                                         uint32_t, int8_t*, int64_t*, unsigned long long*, hipStream_t);
+template int mlp_route<src_grid>(agwn_mlp const*, src_grid const&, int64_t, float, uint32_t, uint32_t, int8_t*,
+                                 int64_t*, unsigned long long*, hipStream_t);
+
+int route_apply(int64_t const* list, int64_t m, float const* w, int8_t* labels, unsigned long long* counters,
+                hipStream_t s)
+{
+    if (m == 0)
+        return ok;
+    k_apply<<<dim3(unsigned((m + kB - 1) / kB)), dim3(kB), 0, s>>>(list, m, w, labels, counters);
+    AGWN_HIP(hipGetLastError());
+    return ok;
+}
+
+template <class Src>
+int gather_routed(Src const& src, int64_t const* list, int64_t m, float* pts, hipStream_t s)
+{
+    if (m == 0)
+        return ok;
+    k_gather_routed<Src><<<dim3(unsigned((m + kB - 1) / kB)), dim3(kB), 0, s>>>(src, list, m, pts);
+    AGWN_HIP(hipGetLastError());
+    return ok;
+}
+template int gather_routed<src_array<float>>(src_array<float> const&, int64_t const*, int64_t, float*, hipStream_t);
+template int gather_routed<src_grid>(src_grid const&, int64_t const*, int64_t, float*, hipStream_t);
+} // namespace agwn
+
+// ---------------------------------------------------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------------------------------------------------
+namespace
+{
+template <class F>
+int guarded_mlp(F&& f)
+{
+    agwn::clear_error();
+    try
+    {
+        return f();
+    }
+    catch (std::bad_alloc const&)
+    {
+        agwn::set_error("host allocation failed");
+        return agwn::err_alloc;
+    }
+    catch (...)
+    {
+        agwn::set_error("unexpected C++ exception");
+        return agwn::err_hip;
+    }
+}
+} // namespace
+
+extern "C" {
+int64_t agwn_mlp_num_params(void) { return agwn::mlp_num_params(); }
+int agwn_mlp_create(float const* lo, float const* hi, float max_freq, uint64_t seed, agwn_mlp** out)
+{
+    return guarded_mlp([&] { return agwn::mlp_create(lo, hi, max_freq, seed, out); });
+}
+int agwn_mlp_destroy(agwn_mlp* m)
+{
+    agwn::clear_error();
+    delete m;
+    return agwn::ok;
+}
+int agwn_mlp_info(agwn_mlp const* m, float* lo, float* hi, float* max_freq)
+{
+    return guarded_mlp([&]() -> int {
+        AGWN_CHECK(m != nullptr, "mlp handle is null");
+        for (int a = 0; a < 3; ++a)
+        {
+            if (lo)
+                lo[a] = m->impl.lo[a];
+            if (hi)
+                hi[a] = m->impl.hi[a];
+        }
+        if (max_freq)
+            *max_freq = m->impl.max_freq;
+        return agwn::ok;
+    });
+}
+int agwn_mlp_get_params(agwn_mlp const* m, float* out)
+{
+    return guarded_mlp([&] { return agwn::mlp_get_params(m, out); });
+}
+int agwn_mlp_set_params(agwn_mlp* m, float const* in)
+{
+    return guarded_mlp([&] { return agwn::mlp_set_params(m, in); });
+}
+int agwn_mlp_train_dev(agwn_mlp* m, float const* pts, float const* target, int64_t n, int batch, int epochs, float lr,
+                       uint64_t seed, float* loss_per_epoch, agwn_stream s)
+{
+    return guarded_mlp(
+        [&] { return agwn::mlp_train(m, pts, target, n, batch, epochs, lr, seed, loss_per_epoch, hipStream_t(s)); });
+}
+int agwn_mlp_grad(agwn_mlp* m, float const* pts, float const* target, int64_t n, float* grad, float* pred, float* loss)
+{
+    return guarded_mlp([&] { return agwn::mlp_grad(m, pts, target, n, grad, pred, loss); });
+}
+int agwn_mlp_predict_dev(agwn_mlp const* m, float const* pts, int64_t n, float* out, agwn_stream s)
+{
+    return guarded_mlp([&] { return agwn::mlp_predict_dev(m, pts, n, out, hipStream_t(s)); });
+}
+} // extern "C"

@@ -1,0 +1,239 @@
+"""Learned inside/outside surrogate over the C ABI (libagwn.so): a per-mesh Fourier-feature MLP whose training,
+inference and routing all run in the library's HIP kernels (BF16 MFMA, FP32 accumulation).
+
+  GwnSurrogate(lo, hi, max_freq, seed)   the network for positions in the box [lo, hi]
+    .train(pts, target, ...)        Adam on the mean squared error (device tensors)
+    .grad(pts, target)              one forward/backward pass without an update (host arrays; gradient checks)
+    .predict(pts)                   fused inference
+    .label(mesh, pts, x0, boundary, delta, ...)           surrogate + exact fallback labels of points
+    .label_grid(grid, start, dx, dy, dz, nx, ny, nz, delta, ...)   the same over an eval_grid lattice
+  training_samples(...)             the training distribution: points near the surface and uniform in the box
+  fit_surrogate(...)                samples -> exact GWN targets (HipMesh.full) -> training on jittered inputs ->
+                                    band calibration on unjittered validation points
+
+Labels are GWN > 0.5. The exact FP32 path (the reference's eval_gwnr_mesh_batch, gwn_mesh.hh:201-219, as evaluated by
+HipMesh / HipGridGwn) decides every point whose prediction lies within `delta` of 0.5, and an audit sample of the rest.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+
+import numpy as np
+
+from ._lib import check, lib, ptr
+from .api import HipGridGwn, HipMesh, _is_dev, _stream_ptr, _x0
+
+
+def num_params() -> int:
+    return int(lib.agwn_mlp_num_params())
+
+
+class GwnSurrogate:
+    """The network for positions in the box [lo, hi]: Fourier frequencies max_freq^(k/31), k = 0..31, per axis;
+    deterministic initialisation from `seed`."""
+
+    def __init__(self, lo, hi, max_freq: float = 128.0, seed: int = 0):
+        self.lo = np.ascontiguousarray(np.asarray(lo, dtype=np.float32).reshape(3))
+        self.hi = np.ascontiguousarray(np.asarray(hi, dtype=np.float32).reshape(3))
+        self.max_freq = float(max_freq)
+        h = C.c_void_p()
+        check(lib.agwn_mlp_create(ptr(self.lo), ptr(self.hi), C.c_float(max_freq), C.c_uint64(seed), C.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def get_params(self) -> np.ndarray:
+        out = np.empty(num_params(), dtype=np.float32)
+        check(lib.agwn_mlp_get_params(self._h, ptr(out)))
+        return out
+
+    def set_params(self, p) -> None:
+        p = np.ascontiguousarray(np.asarray(p, dtype=np.float32).reshape(-1))
+        if p.size != num_params():
+            raise ValueError(f"expected {num_params()} parameters, got {p.size}")
+        check(lib.agwn_mlp_set_params(self._h, ptr(p)))
+
+    def train(self, pts, target, batch: int = 16384, epochs: int = 8, lr: float = 2e-3, seed: int = 0,
+              stream=None) -> np.ndarray:
+        """Adam over device tensors pts (n, 3) float32 and target (n,) float32; returns each epoch's mean loss."""
+        if not (_is_dev(pts) and _is_dev(target)):
+            raise TypeError("train takes device tensors")
+        n = target.numel()
+        if pts.numel() != 3 * n:
+            raise ValueError("pts.size() != 3 * target.size()")
+        losses = np.zeros(max(epochs, 1), dtype=np.float32)
+        check(lib.agwn_mlp_train_dev(self._h, ptr(pts), ptr(target), n, int(batch), int(epochs), C.c_float(lr),
+                                     C.c_uint64(seed), ptr(losses), _stream_ptr(stream)))
+        return losses[:epochs]
+
+    def grad(self, pts, target):
+        """(gradient float32[num_params], predictions float32[n], mean squared error) of host arrays."""
+        p = np.ascontiguousarray(np.asarray(pts, dtype=np.float32).reshape(-1, 3))
+        y = np.ascontiguousarray(np.asarray(target, dtype=np.float32).reshape(-1))
+        if len(p) != len(y):
+            raise ValueError("pts and target sizes differ")
+        g = np.empty(num_params(), dtype=np.float32)
+        pred = np.empty(len(y), dtype=np.float32)
+        loss = C.c_float(0.0)
+        check(lib.agwn_mlp_grad(self._h, ptr(p), ptr(y), len(y), ptr(g), ptr(pred), C.byref(loss)))
+        return g, pred, float(loss.value)
+
+    def predict(self, pts, out=None, stream=None):
+        """Network output of device points (n, 3) float32."""
+        import torch
+
+        n = pts.numel() // 3
+        if out is None:
+            out = torch.empty(n, dtype=torch.float32, device=pts.device)
+        check(lib.agwn_mlp_predict_dev(self._h, ptr(pts), n, ptr(out), _stream_ptr(stream)))
+        return out
+
+    def label(self, mesh: HipMesh, pts, x0, boundary, delta: float, audit_every: int = 0, audit_seed: int = 0,
+              labels=None, stream=None):
+        """Inside/outside labels (int8) of device points; returns (labels, stats dict)."""
+        import torch
+
+        n = pts.numel() // 3
+        if labels is None:
+            labels = torch.empty(n, dtype=torch.int8, device=pts.device)
+        nseg = 0 if boundary is None else boundary.numel() // 7
+        st = np.zeros(4, dtype=np.int64)
+        check(lib.agwn_label_surrogate_dev(mesh.handle, self._h, ptr(pts), n, ptr(_x0(x0, np.float32)),
+                                           ptr(boundary) if nseg else None, nseg, C.c_float(delta), int(audit_every),
+                                           C.c_uint64(audit_seed), ptr(labels), ptr(st), _stream_ptr(stream)))
+        return labels, _stats(st, n)
+
+    def label_grid(self, grid: HipGridGwn, start, dx, dy, dz, nx: int, ny: int, nz: int, delta: float,
+                   audit_every: int = 0, audit_seed: int = 0, labels=None, stream=None):
+        """Labels of an eval_grid lattice, in its ((iz*ny)+iy)*nx+ix layout; returns (labels, stats dict)."""
+        import torch
+
+        n = nx * ny * nz
+        if labels is None:
+            labels = torch.empty(n, dtype=torch.int8, device="cuda")
+        vs = [np.ascontiguousarray(np.asarray(a, dtype=np.float32).reshape(3)) for a in (start, dx, dy, dz)]
+        st = np.zeros(4, dtype=np.int64)
+        check(lib.agwn_grid_label_surrogate_dev(grid._h, self._h, *[ptr(a) for a in vs], nx, ny, nz, C.c_float(delta),
+                                                int(audit_every), C.c_uint64(audit_seed), ptr(labels), ptr(st),
+                                                _stream_ptr(stream)))
+        return labels, _stats(st, n)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib.agwn_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+def _stats(st: np.ndarray, n: int) -> dict:
+    return {"n": n, "band": int(st[0]), "audit": int(st[1]), "routed": int(st[2]), "audit_disagree": int(st[3]),
+            "fallback_fraction": float(st[2]) / max(n, 1)}
+
+
+def training_samples(verts, idx, lo, hi, n: int, near_frac: float = 0.7, band: float = 0.05, seed: int = 0):
+    """n float32 points: near_frac of them at signed normal offsets t = band * s * u^2 (s = +-1, u ~ U(0,1)) from
+    area-uniform surface points (so all within `band` of the surface, denser close to it), the rest uniform in
+    [lo, hi]."""
+    rng = np.random.default_rng(seed)
+    v = np.asarray(verts, dtype=np.float64).reshape(-1, 3)
+    tri = np.asarray(idx, dtype=np.int64).reshape(-1, 3)
+    a, b, c = v[tri[:, 0]], v[tri[:, 1]], v[tri[:, 2]]
+    nrm = np.cross(b - a, c - a)
+    area = np.linalg.norm(nrm, axis=1)
+    good = area > 0
+    nrm[good] /= area[good, None]
+    cdf = np.cumsum(area)
+    n_near = int(round(n * near_frac))
+    t = rng.random(n_near) * cdf[-1]
+    k = np.minimum(np.searchsorted(cdf, t, side="right"), len(tri) - 1)
+    r1 = np.sqrt(rng.random(n_near))
+    r2 = rng.random(n_near)
+    p = (1 - r1)[:, None] * a[k] + (r1 * (1 - r2))[:, None] * b[k] + (r1 * r2)[:, None] * c[k]
+    off = band * np.where(rng.random(n_near) < 0.5, -1.0, 1.0) * rng.random(n_near) ** 2
+    p += off[:, None] * nrm[k]
+    lo = np.asarray(lo, dtype=np.float64)
+    hi = np.asarray(hi, dtype=np.float64)
+    u = lo + (hi - lo) * rng.random((n - n_near, 3))
+    pts = np.concatenate([p, u]).astype(np.float32)
+    return pts[rng.permutation(n)]
+
+
+def calibrate_delta(pred: np.ndarray, exact: np.ndarray, factor: float = 1.5, margin: float = 0.02) -> float:
+    """Band half-width from validation predictions: factor x the largest |pred - 0.5| of a wrongly labelled
+    validation point, plus margin."""
+    wrong = (pred > 0.5) != (exact > 0.5)
+    worst = float(np.max(np.abs(pred[wrong] - 0.5))) if wrong.any() else 0.0
+    return factor * worst + margin
+
+
+def fit_surrogate(verts, idx, boundary, x0, lo, hi, n_train: int = 1 << 24, n_val: int = 1 << 20,
+                  epochs: int = 8, batch: int = 16384, lr: float = 2e-3, seed: int = 0, mesh: HipMesh | None = None,
+                  max_freq: float = 128.0, jitter: float | None = None, target_clip=(-0.25, 1.25),
+                  factor: float = 1.5, margin: float = 0.02, log=None):
+    """Trains a GwnSurrogate for the mesh on exact FP32 GWN values computed by the library, and picks the routing band.
+
+    The network sees each training point displaced by Gaussian noise of standard deviation `jitter` (default 0.35 %
+    of the box diagonal) while its target stays the exact GWN of the undisplaced point: the mean-squared-error optimum
+    is then the GWN blurred at that scale, so predictions fall towards 0.5 within a few `jitter` of the surface --
+    where a sharp fit would be confidently wrong about a noisy surface -- and stay near 0 or 1 elsewhere. The band
+    half-width delta is calibrated on undisplaced validation points.
+
+    Returns (surrogate, delta, info) -- info holds the timings, per-epoch losses and validation statistics.
+    """
+    import torch
+
+    info: dict = {}
+    t0 = time.perf_counter()
+    own_mesh = mesh is None
+    if own_mesh:
+        mesh = HipMesh(np.asarray(verts, dtype=np.float32), idx, dtype=np.float32)
+    seg = torch.from_numpy(np.ascontiguousarray(np.asarray(boundary, dtype=np.float32).reshape(-1, 7))).cuda()
+    pts_h = training_samples(verts, idx, lo, hi, n_train + n_val, seed=seed)
+    info["sample_s"] = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    pts = torch.from_numpy(pts_h).cuda()
+    w = torch.empty(len(pts_h), dtype=torch.float32, device="cuda")
+    mesh.full(pts, x0, seg if len(seg) else None, out=w)
+    torch.cuda.synchronize()
+    info["label_s"] = time.perf_counter() - t1
+    if jitter is None:
+        jitter = 0.0035 * float(np.linalg.norm(np.asarray(hi, dtype=np.float64) - np.asarray(lo, dtype=np.float64)))
+    gen = torch.Generator(device="cuda").manual_seed(seed + 1)
+    tr_p = (pts[:n_train] + jitter * torch.randn((n_train, 3), device="cuda", generator=gen)).contiguous()
+    va_p = pts[n_train:].contiguous()
+    tr_y = w[:n_train].clamp(*target_clip).contiguous()
+    va_y = w[n_train:]
+    info["jitter"] = jitter
+    info["max_freq"] = max_freq
+    net = GwnSurrogate(lo, hi, max_freq=max_freq, seed=seed)
+    t2 = time.perf_counter()
+    losses = net.train(tr_p, tr_y, batch=batch, epochs=epochs, lr=lr, seed=seed)
+    torch.cuda.synchronize()
+    info["train_s"] = time.perf_counter() - t2
+    info["losses"] = [float(x) for x in losses]
+    pred = net.predict(va_p).cpu().numpy()
+    exact = va_y.cpu().numpy()
+    wrong = (pred > 0.5) != (exact > 0.5)
+    delta = calibrate_delta(pred, exact, factor, margin)
+    info.update(val_points=n_val, val_wrong=int(wrong.sum()),
+                val_max_wrong=float(np.max(np.abs(pred[wrong] - 0.5))) if wrong.any() else 0.0, delta=delta,
+                val_band_fraction=float(np.mean(np.abs(pred - 0.5) <= delta)), fit_s=time.perf_counter() - t0)
+    if log:
+        log(f"surrogate: {info}")
+    if own_mesh:
+        mesh.close()
+    return net, delta, info

@@ -1,15 +1,24 @@
-"""Benchmark: full GWN (fractional + integer term) of a 256^3 query grid against the 1M-triangle open mesh, one MI355X.
+"""Benchmark: inside/outside labels (GWN > 0.5) of a 512^3 query grid against the 1M-triangle open mesh, one MI355X --
+learned surrogate + exact fallback, against the plain exact method on the same grid.
 
-Workload (the plain-method GPU configuration): seeded noisy torus R=1, r=0.35, 1353 x 473 quads (2 triangles each),
-sigma = 0.01 r Gaussian vertex noise, 400 disk holes of radius 0.05 -> 1,004,810 triangles, 31,376 unit-weight
-boundary segments; queries = the 256^3 lattice over the bounding box expanded by 10 %; x0 = normalize(0.31,0.57,0.76);
-FP32. A step is one eval_gwnr_mesh_batch over all 16,777,216 queries, inputs resident in HBM (device pointers through
-the C ABI, on torch's current stream).
+Workload: seeded noisy torus R=1, r=0.35, 1353 x 473 quads (2 triangles each), sigma = 0.01 r Gaussian vertex noise,
+400 disk holes of radius 0.05 -> 1,004,810 triangles, 31,376 unit-weight boundary segments; queries = the 512^3
+lattice over the bounding box expanded by 10 % (134,217,728 points, generated in-kernel); FP32.
+
+  plain      one eval_grid (HipGridGwn): full GWN (fractional + integer term) of every lattice point, labels = w > 0.5
+  surrogate  a per-mesh MLP (192 Fourier features, 4 x 256 SiLU, BF16 MFMA) trained once on 2^24 exact GWN samples
+             (70 % within 0.05 of the surface, 30 % uniform in the box); each step labels the lattice with the fused
+             inference kernel, routes points with |pred - 0.5| <= delta plus a 1/256 audit sample to the exact kernels,
+             and takes their exact labels. delta comes from a held-out validation set (1.5 x the worst wrong-side
+             margin + 0.02). Training happens before the timed region and is reported (train_s).
+The headline value is surrogate steps/s x queries; mislabels = labels that differ from the plain run's (must be 0).
 
 Reported beside it:
-  roofline      the dominant kernel (the fractional boundary sum, k_frac_f32): algorithmic FLOP per launch =
-                n_queries * n_segments * 56 (the per-pair count fixed in the design notes), over its average launch
-                time measured with HIP events on the launch stream, against MI355X FP32 vector peak 157.3 TFLOP/s
+  roofline      the dominant kernel of a surrogate step (the exact fractional sum over the routed points, k_frac_f32,
+                unless inference takes longer): its launch on exactly the routed points, timed with HIP events on the
+                launch stream; algorithmic FLOP per launch = routed x segments x 56 (per-pair count of the design
+                notes) against MI355X FP32 vector peak 157.3 TFLOP/s (or 492,032 FLOP per point against the 2.5 PFLOP/s
+                BF16 MFMA peak for inference)
   cpu_baseline  the reference's own eval_gwnr_mesh_batch_fractional<float> (oracle/_ref, compiled from its headers)
                 with its OpenMPDispatcher on this host's cores, on a bounded seeded sample of the grid's queries,
                 scaled to queries/s; the integer term is counted as free on the CPU (no Embree here), which favours
@@ -30,14 +39,17 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 PEAK_FP32_TFLOPS = 157.3
+PEAK_BF16_TFLOPS = 2500.0
 FLOP_PER_PAIR = 56
+MLP_FLOP_PER_POINT = 2 * (192 * 256 + 3 * 256 * 256 + 256)
+METRIC = "GWN queries/s, 1M-tri open mesh; surrogate+fallback vs plain, 0 mislabels"
 
 
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(verts32, idx, pos32_sample, seg32, x0, budget_s: float) -> dict | None:
+def cpu_baseline(pos32_sample, seg32, x0, budget_s: float, grid_desc: str) -> dict | None:
     """The reference's FP32 fractional term on the host cores, over a bounded sample of the benchmark's queries."""
     try:
         from oracle import Reference
@@ -63,17 +75,47 @@ def cpu_baseline(verts32, idx, pos32_sample, seg32, x0, budget_s: float) -> dict
         "unit": "queries/s",
         "cores": threads,
         "kind": "reference",
-        "sample": f"{n} seeded queries of the 256^3 grid x {len(seg32)} segments, eval_gwnr_mesh_batch_fractional<float> "
-                  f"with OpenMPDispatcher ({threads} threads), {dt:.1f} s; integer term not timed (taken as 0)",
+        "sample": f"{n} seeded queries of the {grid_desc} x {len(seg32)} segments, "
+                  f"eval_gwnr_mesh_batch_fractional<float> with OpenMPDispatcher ({threads} threads), {dt:.1f} s; "
+                  f"integer term not timed (taken as 0)",
     }
+
+
+def lattice_points(start, dx, dy, dz, nx, ny, nz, idx: np.ndarray) -> np.ndarray:
+    """Float32 coordinates of lattice cells idx, evaluated as the kernels do (start + dx*ix + dy*iy + dz*iz, left to
+    right per component)."""
+    f = np.float32
+    ix = (idx % nx).astype(f)
+    iy = ((idx // nx) % ny).astype(f)
+    iz = (idx // (nx * ny)).astype(f)
+    s, a, b, c = (np.asarray(v, dtype=f) for v in (start, dx, dy, dz))
+    return (((s[None, :] + a[None, :] * ix[:, None]) + b[None, :] * iy[:, None]) + c[None, :] * iz[:, None]).astype(f)
+
+
+def copy_calibration() -> int:
+    import torch
+
+    a = torch.ones(1 << 28, dtype=torch.float32, device="cuda:0")  # 1 GiB
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": a.numel() * 4, "copies": 3}), flush=True)
+    return 0
 
 
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--grid", type=int, default=256)
+    ap.add_argument("--grid", type=int, default=512)
+    ap.add_argument("--plain-steps", type=int, default=1)
+    ap.add_argument("--train-samples", type=int, default=1 << 24)
+    ap.add_argument("--epochs", type=int, default=16)
+    ap.add_argument("--batch", type=int, default=16384)
+    ap.add_argument("--lr", type=float, default=3e-3)
+    ap.add_argument("--audit-every", type=int, default=256)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -82,15 +124,7 @@ def main() -> int:
     args = ap.parse_args()
 
     if args.copy_calibration:
-        import torch
-
-        a = torch.ones(1 << 28, dtype=torch.float32, device="cuda:0")  # 1 GiB
-        b = torch.empty_like(a)
-        for _ in range(3):
-            b.copy_(a)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_calibration_bytes": a.numel() * 4, "copies": 3}), flush=True)
-        return 0
+        return copy_calibration()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (queries shard trivially by "
@@ -105,90 +139,146 @@ def main() -> int:
     if not agwn.device_available():
         raise SystemExit(f"no HIP device: {agwn.last_error()}")
     dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
 
     t = time.perf_counter()
     verts, idx = mg.holed_torus_mesh()
     v32 = verts.astype(np.float32)
     seg32 = agwn.build_boundary_segments(v32, idx)
     lo, hi = mg.expanded_bbox(verts)
-    n = args.grid
-    pos32 = mg.grid_points(lo, hi, n, dtype=np.float32)
-    x0 = mg.example_x0(np.float32)
-    nq, nseg, ntri = len(pos32), len(seg32), len(idx) // 3
+    nres = args.grid
+    start, dx, dy, dz = mg.grid_axes(lo, hi, nres)
+    nq, nseg, ntri = nres ** 3, len(seg32), len(idx) // 3
+    # the evaluator's integration / pole axis: the smallest axis (all equal -> x), x0 = its normalised step
+    x0 = np.array([1.0, 0.0, 0.0], dtype=np.float32)
     log(f"inputs: {ntri} triangles, {nseg} segments, {nq} queries ({time.perf_counter() - t:.1f} s)")
 
-    pos_t = torch.from_numpy(pos32).to(dev)
-    seg_t = torch.from_numpy(seg32).to(dev)
-    out_t = torch.empty(nq, dtype=torch.float32, device=dev)
-    frac_t = torch.empty(nq, dtype=torch.float32, device=dev)
-
     t = time.perf_counter()
-    mesh = agwn.HipMesh(v32, idx)
+    grid = agwn.HipGridGwn().set_scene(v32, idx, seg32)
+    mesh = agwn.HipMesh(v32, idx, dtype=np.float32)
     torch.cuda.synchronize()
     build_s = time.perf_counter() - t
-    log(f"LBVH build {build_s * 1e3:.1f} ms")
+    log(f"scene + LBVH builds {build_s * 1e3:.1f} ms")
 
-    for w in range(args.warmup):
-        agwn.eval_gwnr_mesh_batch(mesh, pos_t, x0, seg_t, out=out_t)
-        torch.cuda.synchronize()
-        log(f"warmup {w + 1}/{args.warmup} done")
-
+    # ---- plain exact method on the lattice -------------------------------------------------------------------------
+    w = torch.empty(nq, dtype=torch.float32, device=dev)
+    grid.eval_grid(start, dx, dy, dz, nres, nres, nres, out=w)  # warm-up (prepares the per-direction BVH state)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
+    for _ in range(args.plain_steps):
+        grid.eval_grid(start, dx, dy, dz, nres, nres, nres, out=w)
+    torch.cuda.synchronize()
+    plain_ms = (time.perf_counter() - t0) * 1e3 / args.plain_steps
+    exact = (w > 0.5).to(torch.int8)
+    log(f"plain: {plain_ms:.1f} ms/step = {nq / plain_ms * 1e3 / 1e6:.2f} Mq/s")
+
+    # ---- surrogate: fit (outside the timed region) -------------------------------------------------------------------
+    net, delta, fit = agwn.fit_surrogate(v32, idx, seg32, x0, lo.astype(np.float32), hi.astype(np.float32),
+                                         n_train=args.train_samples, n_val=1 << 20, epochs=args.epochs,
+                                         batch=args.batch, lr=args.lr, seed=0, mesh=mesh)
+    log(f"fit: {fit['fit_s']:.1f} s (train {fit['train_s']:.1f} s), losses {fit['losses'][0]:.4g} -> "
+        f"{fit['losses'][-1]:.4g}, validation wrong {fit['val_wrong']}/{fit['val_points']}, delta {delta:.4f}")
+
+    # ---- surrogate + fallback steps ----------------------------------------------------------------------------------
+    lab = torch.empty(nq, dtype=torch.int8, device=dev)
+    st = None
+    for k in range(args.warmup):
+        _, st = net.label_grid(grid, start, dx, dy, dz, nres, nres, nres, delta, audit_every=args.audit_every,
+                               audit_seed=1000 + k, labels=lab)
+        log(f"warmup {k + 1}/{args.warmup}: {st}")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    stats = []
     for k in range(args.steps):
-        agwn.eval_gwnr_mesh_batch(mesh, pos_t, x0, seg_t, out=out_t)
+        _, st = net.label_grid(grid, start, dx, dy, dz, nres, nres, nres, delta, audit_every=args.audit_every,
+                               audit_seed=k, labels=lab)
+        stats.append(st)
     torch.cuda.synchronize()
     total_s = time.perf_counter() - t0
     ms_per_step = total_s * 1e3 / args.steps
-    log(f"timed {args.steps} steps: {ms_per_step:.2f} ms/step")
+    mislabels = int((lab != exact).sum().item())
+    st = stats[-1]
+    log(f"surrogate+fallback: {ms_per_step:.1f} ms/step = {nq / ms_per_step * 1e3 / 1e6:.2f} Mq/s, "
+        f"speedup {plain_ms / ms_per_step:.2f}x, routed {st['fallback_fraction']:.4f}, mislabels {mislabels}")
 
-    # dominant kernel: the fractional boundary sum, timed with HIP events on its launch stream
-    stream = torch.cuda.current_stream()
+    # ---- dominant kernel: the step's launches, re-run alone on the same inputs with HIP events on their stream ------
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    reps = max(1, min(3, args.steps))
+    sample_idx = np.sort(np.random.default_rng(0).choice(nq, size=min(nq, 1 << 22), replace=False))
+    pts_s = torch.from_numpy(lattice_points(start, dx, dy, dz, nres, nres, nres, sample_idx)).to(dev)
+    pred_s = net.predict(pts_s)
     ev0.record(stream)
-    for _ in range(reps):
-        agwn.eval_gwnr_mesh_batch_fractional(pos_t, x0, seg_t, out=frac_t, stream=stream)
+    net.predict(pts_s, out=pred_s, stream=stream)
     ev1.record(stream)
     ev1.synchronize()
-    frac_ms = ev0.elapsed_time(ev1) / reps
-    int_t = torch.empty(nq, dtype=torch.int32, device=dev)
+    infer_ms = ev0.elapsed_time(ev1) * nq / len(sample_idx)
+    # the routed points of the last step: its band (same inference, same coordinates) -- audit points are a hashed
+    # 1/audit_every sample, so the band plus that many arbitrary non-band points reproduce the launch size
+    routed = st["routed"]
+    n_frac = max(1, routed)
+    band_idx = sample_idx[(pred_s - 0.5).abs().cpu().numpy() <= delta]
+    reps = int(np.ceil(n_frac / max(len(band_idx), 1)))
+    rp = np.tile(band_idx, reps)[:n_frac] if len(band_idx) else sample_idx[:n_frac]
+    pts_r = torch.from_numpy(lattice_points(start, dx, dy, dz, nres, nres, nres, rp)).to(dev)
+    seg_t = torch.from_numpy(seg32).to(dev)
+    frac_r = torch.empty(len(rp), dtype=torch.float32, device=dev)
+    agwn.eval_gwnr_mesh_batch_fractional(pts_r, x0, seg_t, out=frac_r, stream=stream)
     ev0.record(stream)
-    agwn.eval_gwnr_mesh_batch_integer(mesh, pos_t, x0, out=int_t, stream=stream)
+    agwn.eval_gwnr_mesh_batch_fractional(pts_r, x0, seg_t, out=frac_r, stream=stream)
+    ev1.record(stream)
+    ev1.synchronize()
+    frac_ms = ev0.elapsed_time(ev1)
+    cnt_r = torch.empty(len(rp), dtype=torch.int32, device=dev)
+    ev0.record(stream)
+    agwn.eval_gwnr_mesh_batch_integer(mesh, pts_r, x0, out=cnt_r, stream=stream)
     ev1.record(stream)
     ev1.synchronize()
     int_ms = ev0.elapsed_time(ev1)
-    pairs = nq * nseg
-    achieved = pairs * FLOP_PER_PAIR / (frac_ms * 1e-3) / 1e12
-    log(f"fractional {frac_ms:.2f} ms ({pairs / (frac_ms * 1e-3) / 1e12:.3f} Tpair/s), integer {int_ms:.2f} ms")
-
-    # sanity: the full GWN must be frac + count
-    chk = (out_t - frac_t - int_t.to(torch.float32)).abs().max().item()
+    # sanity on the same points: full = fractional + integer
+    full_r = torch.empty_like(frac_r)
+    agwn.eval_gwnr_mesh_batch(mesh, pts_r, x0, seg_t, out=full_r, stream=stream)
+    chk = (full_r - frac_r - cnt_r.to(torch.float32)).abs().max().item()
     if not chk < 1e-4:
         raise SystemExit(f"full != fractional + integer (max diff {chk})")
+    log(f"kernels per step: inference {infer_ms:.2f} ms, fractional {frac_ms:.2f} ms and integer {int_ms:.2f} ms "
+        f"on {len(rp)} routed points")
+    if frac_ms >= infer_ms:
+        achieved = len(rp) * nseg * FLOP_PER_PAIR / (frac_ms * 1e-3) / 1e12
+        roof = {"bound": "valu", "kernel": "k_frac_f32", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
+                "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "kernel_ms": frac_ms,
+                "flop_per_launch": len(rp) * nseg * FLOP_PER_PAIR, "launch_points": len(rp)}
+        traffic_key = "k_frac_f32"
+    else:
+        achieved = nq * MLP_FLOP_PER_POINT / (infer_ms * 1e-3) / 1e12
+        roof = {"bound": "mfma", "kernel": "k_mlp_infer", "achieved": achieved, "peak": PEAK_BF16_TFLOPS,
+                "unit": "TFLOP/s", "frac": achieved / PEAK_BF16_TFLOPS, "kernel_ms": infer_ms,
+                "flop_per_launch": nq * MLP_FLOP_PER_POINT, "launch_points": nq}
+        traffic_key = "k_mlp_infer"
+    # HBM bytes per launch of that kernel, from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE passes of this
+    # workload (profiles/traffic.json, scaled by the copy calibration); null when none is committed
+    roof["traffic"] = None
+    tj = ROOT / "profiles" / "traffic.json"
+    if tj.exists():
+        try:
+            tr = json.loads(tj.read_text())
+            if tr.get("workload_grid") == nres:
+                roof["traffic"] = tr[traffic_key]["hbm_bytes_per_launch"]
+        except (KeyError, ValueError):
+            pass
+    roof["traffic_unit"] = "bytes per launch (rocprofv3 FETCH_SIZE x2 + WRITE_SIZE, see profiles/)"
 
     cpu = None
     if not args.no_cpu:
         rng = np.random.default_rng(0)
-        sample = pos32[np.sort(rng.choice(nq, size=min(nq, 200_000), replace=False))]
+        sample = lattice_points(start, dx, dy, dz, nres, nres, nres,
+                                np.sort(rng.choice(nq, size=min(nq, 200_000), replace=False)))
         log("cpu baseline ...")
-        cpu = cpu_baseline(v32, idx, sample, seg32, x0, args.cpu_seconds)
+        cpu = cpu_baseline(sample, seg32, x0, args.cpu_seconds, f"{nres}^3 grid")
         if cpu:
             log(f"cpu baseline {cpu['value']:.1f} queries/s on {cpu['cores']} threads")
 
-    # HBM bytes per launch of the dominant kernel, from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE passes
-    # (profiles/traffic.json, scaled by the copy calibration); null when no profile of this workload is committed
-    traffic = None
-    tj = ROOT / "profiles" / "traffic.json"
-    if tj.exists() and n == 256:
-        try:
-            traffic = json.loads(tj.read_text())["k_frac_f32"]["hbm_bytes_per_launch"]
-        except (KeyError, ValueError):
-            traffic = None
-
     value = nq / (ms_per_step * 1e-3)
     line = {
-        "metric": "GWN queries/s, 1M-tri open mesh; surrogate+fallback vs plain, 0 mislabels",
+        "metric": METRIC,
         "value": value,
         "unit": "queries/s",
         "n_gpus": 1,
@@ -199,32 +289,42 @@ def main() -> int:
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded holed torus + 256^3 lattice)",
+        "data": f"synthetic (seeded holed torus + {nres}^3 lattice; surrogate trained on seeded samples)",
         "config": {
-            "workload": f"plain full GWN (fractional + integer) of a {n}^3 grid, holed noisy torus",
+            "workload": f"inside/outside labels of a {nres}^3 grid over the 1,004,810-triangle holed noisy torus: "
+                        f"surrogate (BF16 MLP) + exact FP32 fallback",
             "triangles": ntri,
             "segments": nseg,
             "queries": nq,
-            "method": "plain (exact, no surrogate)",
             "parallelism": "single GPU",
         },
-        "roofline": {
-            "bound": "valu",
-            "kernel": "k_frac_f32",
-            "achieved": achieved,
-            "peak": PEAK_FP32_TFLOPS,
-            "unit": "TFLOP/s",
-            "frac": achieved / PEAK_FP32_TFLOPS,
-            "traffic": traffic,
-            "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE x2 + WRITE_SIZE, see profiles/)",
-            "kernel_ms": frac_ms,
-            "flop_per_launch": pairs * FLOP_PER_PAIR,
+        "plain": {"value": nq / (plain_ms * 1e-3), "unit": "queries/s", "ms_per_step": plain_ms,
+                  "steps": args.plain_steps, "method": "exact full GWN (fractional + integer), FP32"},
+        "speedup_vs_plain": plain_ms / ms_per_step,
+        "mislabels": mislabels,
+        "surrogate": {
+            "delta": delta,
+            "fallback_fraction": st["fallback_fraction"],
+            "band": st["band"],
+            "audit": st["audit"],
+            "audit_disagree": sum(s["audit_disagree"] for s in stats),
+            "train_samples": args.train_samples,
+            "epochs": args.epochs,
+            "batch": args.batch,
+            "train_s": fit["train_s"],
+            "fit_s": fit["fit_s"],
+            "loss_first_last": [fit["losses"][0], fit["losses"][-1]],
+            "val_wrong": fit["val_wrong"],
+            "val_points": fit["val_points"],
         },
-        "integer_term_ms": int_ms,
-        "lbvh_build_ms": build_s * 1e3,
+        "kernels_ms": {"k_mlp_infer": infer_ms, "k_frac_f32": frac_ms, "k_count": int_ms},
+        "roofline": roof,
+        "build_ms": build_s * 1e3,
         "cpu_baseline": cpu,
     }
+    net.close()
     mesh.close()
+    grid.close()
     print(json.dumps(line), flush=True)
     return 0
 

@@ -152,6 +152,51 @@ int agwn_boundary_f32(const float* verts, int64_t nv, const int32_t* idx, int64_
 int agwn_boundary_f64(const double* verts, int64_t nv, const int32_t* idx, int64_t nidx, double* out7, int64_t cap,
                       int64_t* count);
 
+/* ---------------------------------------------------------------------------------------------------------------
+ * Learned inside/outside surrogate (no counterpart in the reference: an accelerator in front of the exact path, which
+ * stays the judge). A per-mesh MLP -- 192 Fourier features of the position normalised to a box (32 frequencies
+ * max_freq^(k/31) per axis, sin and cos), 4 hidden layers x 256 SiLU, linear output -- with BF16 weights/activations and
+ * FP32 accumulation, trained on GPU-computed exact GWN values by the library's own MFMA forward/backward kernels and
+ * Adam (FP32 master weights). Labels are GWN > 0.5 ("inside").
+ * --------------------------------------------------------------------------------------------------------------- */
+typedef struct agwn_mlp agwn_mlp;
+/* Number of FP32 parameters (W0, b0, ..., W3, b3, wo, bo; W_l row-major [fan_in x 256]). */
+int64_t agwn_mlp_num_params(void);
+/* New network for positions in the box [lo, hi] (host float[3] each), top frequency max_freq in [1, 4096] (periods
+ * over the box's full width / 2), deterministic init from seed. */
+int agwn_mlp_create(const float* lo, const float* hi, float max_freq, uint64_t seed, agwn_mlp** out);
+int agwn_mlp_destroy(agwn_mlp* mlp);
+/* The box and top frequency (any output may be NULL). */
+int agwn_mlp_info(const agwn_mlp* mlp, float* lo, float* hi, float* max_freq);
+/* Host copies of the FP32 parameters (set resets the Adam state). */
+int agwn_mlp_get_params(const agwn_mlp* mlp, float* out);
+int agwn_mlp_set_params(agwn_mlp* mlp, const float* in);
+/* Adam (b1 0.9, b2 0.999, eps 1e-8, cosine decay lr -> lr/20) on the mean squared error against target[i] over
+ * pts[i] (device float[3n], float[n]); each epoch visits every sample once in a seeded order, batch a multiple of
+ * 128. loss_per_epoch (host float[epochs], may be NULL) receives each epoch's mean loss. Synchronous. */
+int agwn_mlp_train_dev(agwn_mlp* mlp, const float* pts, const float* target, int64_t n, int batch, int epochs,
+                       float lr, uint64_t seed, float* loss_per_epoch, agwn_stream stream);
+/* One forward + backward pass over host samples (n <= 2^20) without an update: grad (float[num_params]), pred
+ * (float[n]), loss (mean squared error); outputs may be NULL. For gradient checks. */
+int agwn_mlp_grad(agwn_mlp* mlp, const float* pts, const float* target, int64_t n, float* grad, float* pred,
+                  float* loss);
+/* Fused inference (device arrays): out[i] = network(pts[i]). */
+int agwn_mlp_predict_dev(const agwn_mlp* mlp, const float* pts, int64_t n, float* out, agwn_stream stream);
+/* Surrogate + fallback labelling of points: labels[i] = (GWN(pts[i]) > 0.5). The network labels every point; points
+ * with |pred - 0.5| <= delta, plus an audit sample of about 1/audit_every of the others (0: none, hashed with
+ * audit_seed), go through the exact FP32 path (agwn_full_batch_f32 with this mesh, x0 and boundary) and take its label.
+ * stats (host int64[4]): points routed by the band, audited, routed in all, audited points whose network label
+ * disagreed with the exact one. Device pos/seg7/labels; host x0. Synchronous. */
+int agwn_label_surrogate_dev(const agwn_mesh* mesh, const agwn_mlp* mlp, const float* pos, int64_t n, const float* x0,
+                             const float* seg7, int64_t nseg, float delta, int64_t audit_every, uint64_t audit_seed,
+                             int8_t* labels, int64_t* stats, agwn_stream stream);
+/* The same over the lattice of agwn_grid_eval (labels in its layout): routed points take the labels of
+ * agwn_grid_eval_dev's values, bit for bit. */
+int agwn_grid_label_surrogate_dev(agwn_grid* grid, const agwn_mlp* mlp, const float* start, const float* dx,
+                                  const float* dy, const float* dz, int nx, int ny, int nz, float delta,
+                                  int64_t audit_every, uint64_t audit_seed, int8_t* labels, int64_t* stats,
+                                  agwn_stream stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,197 @@
+"""The learned surrogate on the GPU: MFMA forward/backward against the float64 oracle (with the kernels' BF16 rounding
+points emulated), Adam, the fused inference kernel, training, and surrogate + exact-fallback labelling.
+
+Bars:
+* gradient: per parameter block, |g_gpu - g_oracle| / |g_oracle| <= 1e-2 (2-norms) -- the kernels accumulate in FP32
+  and round intermediate activations to BF16 exactly where the oracle does, so what remains is FP32 summation order
+  and the BF16 rounding flips it causes; predictions within 1e-3 absolute, loss within 1e-3 relative;
+* fused inference == the training kernels' forward pass (1e-6);
+* Adam's first step: every weight moves by -lr g/(|g| + eps) of the GPU gradient (float32 rounding);
+* labels: 0 mislabels against the exact path on every point labelled; routed points take the exact label bit for bit.
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+import pytest
+
+from antipodal_amd import meshgen as mg
+from oracle import mlp as om
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def c1():
+    v, idx = mg.icosphere_cap_mesh()
+    lo, hi = mg.expanded_bbox(v, 0.10)
+    return v, idx, lo.astype(np.float32), hi.astype(np.float32)
+
+
+def block_errors(g, ref):
+    lay = om.LAYOUT
+    out = {}
+    for l in range(om.LAYERS):
+        for name, off, cnt in ((f"W{l}", lay["w"][l], om.IN_DIM[l] * om.H), (f"b{l}", lay["b"][l], om.H)):
+            a, b = g[off: off + cnt], ref[off: off + cnt]
+            out[name] = float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+    a, b = g[lay["wo"]: lay["wo"] + om.H], ref[lay["wo"]: lay["wo"] + om.H]
+    out["wo"] = float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+    out["bo"] = float(abs(g[lay["bo"]] - ref[lay["bo"]]) / max(abs(ref[lay["bo"]]), 1e-30))
+    return out
+
+
+def test_grad_matches_oracle(gpu, c1):
+    v, idx, lo, hi = c1
+    rng = np.random.default_rng(0)
+    n = 1000  # not a multiple of 128: exercises the padded rows
+    pts = (lo + (hi - lo) * rng.random((n, 3))).astype(np.float32)
+    y = rng.random(n).astype(np.float32)
+    with gpu.GwnSurrogate(lo, hi, seed=3) as net:
+        p = net.get_params()
+        g, pred, loss = net.grad(pts, y)
+    loss_o, g_o, pred_o = om.loss_grad(p.astype(np.float64), pts, y, lo, hi, emulate=True)
+    errs = block_errors(g.astype(np.float64), g_o)
+    print("gradient block errors:", {k: f"{e:.2e}" for k, e in errs.items()})
+    assert max(errs.values()) <= 1e-2, errs
+    assert np.max(np.abs(pred - pred_o)) <= 1e-3
+    assert abs(loss - loss_o) <= 1e-3 * loss_o
+    # and the emulation matters less than the bar: the exact float64 gradient is close too
+    _, g_x, _ = om.loss_grad(p.astype(np.float64), pts, y, lo, hi, emulate=False)
+    assert np.linalg.norm(g - g_x) <= 0.05 * np.linalg.norm(g_x)
+
+
+def test_predict_equals_training_forward(gpu, c1):
+    import torch
+
+    v, idx, lo, hi = c1
+    rng = np.random.default_rng(1)
+    n = 3000
+    pts = (lo + (hi - lo) * rng.random((n, 3))).astype(np.float32)
+    with gpu.GwnSurrogate(lo, hi, seed=5) as net:
+        _, pred_train, _ = net.grad(pts, np.zeros(n, dtype=np.float32))
+        pred = net.predict(torch.from_numpy(pts).cuda()).cpu().numpy()
+    np.testing.assert_allclose(pred, pred_train, atol=1e-6, rtol=0)
+
+
+def test_adam_first_step(gpu, c1):
+    import torch
+
+    v, idx, lo, hi = c1
+    rng = np.random.default_rng(2)
+    n = 512
+    pts = (lo + (hi - lo) * rng.random((n, 3))).astype(np.float32)
+    y = rng.random(n).astype(np.float32)
+    lr = 1e-3
+    with gpu.GwnSurrogate(lo, hi, seed=7) as net:
+        p0 = net.get_params()
+        g, _, _ = net.grad(pts, y)
+        losses = net.train(torch.from_numpy(pts).cuda(), torch.from_numpy(y).cuda(), batch=n, epochs=1, lr=lr)
+        p1 = net.get_params()
+    assert np.isfinite(losses).all()
+    want = p0.astype(np.float64) - lr * g / (np.abs(g) + 1e-8)
+    # the training step sums the batch in another order: coordinates whose gradient is within float32 noise of 0
+    # may move differently; all others move by lr to float32 precision
+    big = np.abs(g) > 1e-4 * np.max(np.abs(g))
+    err = np.abs(p1 - want)
+    assert np.max(err[big]) <= 1e-5 * lr + 4 * np.spacing(np.abs(p0[big]).astype(np.float32)).max()
+    assert np.mean(err > 2e-2 * lr) < 2e-3
+
+
+@pytest.fixture(scope="module")
+def trained_c1(gpu, c1):
+    """A network fitted to the small open mesh (icosphere cap): 2^16 training and 2^14 validation samples."""
+    v, idx, lo, hi = c1
+    bnd = gpu.build_boundary_segments(v.astype(np.float32), idx)
+    x0 = mg.example_x0(np.float32)
+    t = time.perf_counter()
+    net, delta, info = gpu.fit_surrogate(v.astype(np.float32), idx, bnd, x0, lo, hi, n_train=1 << 16, n_val=1 << 14,
+                                         epochs=40, batch=2048, lr=2e-3, seed=11)
+    print(f"fit {time.perf_counter() - t:.2f}s", info)
+    yield net, delta, info, bnd, x0
+    net.close()
+
+
+def test_training_reduces_loss(trained_c1):
+    net, delta, info, bnd, x0 = trained_c1
+    L = info["losses"]
+    assert all(np.isfinite(L))
+    assert L[-1] < 0.2 * L[0], L
+    assert info["val_wrong"] < 0.02 * info["val_points"], info
+    assert delta < 0.5, info
+
+
+def test_grid_labels_zero_mislabels(gpu, c1, trained_c1):
+    import torch
+
+    v, idx, lo, hi = c1
+    net, delta, info, bnd, x0 = trained_c1
+    nres = 64
+    start, dx, dy, dz = mg.grid_axes(lo, hi, nres)
+    with gpu.HipGridGwn() as grid:
+        grid.set_scene(v.astype(np.float32), idx, bnd)
+        w = torch.empty(nres ** 3, dtype=torch.float32, device="cuda")
+        t = time.perf_counter()
+        grid.eval_grid(start, dx, dy, dz, nres, nres, nres, out=w)
+        torch.cuda.synchronize()
+        t_plain = time.perf_counter() - t
+        exact = (w > 0.5).to(torch.int8)
+        t = time.perf_counter()
+        lab, st = net.label_grid(grid, start, dx, dy, dz, nres, nres, nres, delta, audit_every=64, audit_seed=1)
+        torch.cuda.synchronize()
+        t_sur = time.perf_counter() - t
+        mislabels = int((lab != exact).sum())
+        print(f"grid {nres}^3: plain {t_plain * 1e3:.1f} ms, surrogate+fallback {t_sur * 1e3:.1f} ms, stats {st}, "
+              f"mislabels {mislabels}")
+        assert mislabels == 0
+        assert st["audit_disagree"] == 0
+        assert st["fallback_fraction"] < 0.5, st
+        # everything routed: the labels are the exact path's, bit for bit
+        lab_all, st_all = net.label_grid(grid, start, dx, dy, dz, nres, nres, nres, float("inf"))
+        assert st_all["band"] == nres ** 3 and st_all["routed"] == nres ** 3
+        assert torch.equal(lab_all, exact)
+
+
+def test_point_labels_and_audit(gpu, c1, trained_c1):
+    import torch
+
+    v, idx, lo, hi = c1
+    net, delta, info, bnd, x0 = trained_c1
+    rng = np.random.default_rng(4)
+    n = 200_000
+    pts = torch.from_numpy((lo + (hi - lo) * rng.random((n, 3))).astype(np.float32)).cuda()
+    seg = torch.from_numpy(np.ascontiguousarray(bnd, dtype=np.float32)).cuda()
+    with gpu.HipMesh(v.astype(np.float32), idx, dtype=np.float32) as mesh:
+        w = torch.empty(n, dtype=torch.float32, device="cuda")
+        mesh.full(pts, x0, seg, out=w)
+        exact = (w > 0.5).to(torch.int8)
+        lab, st = net.label(mesh, pts, x0, seg, delta, audit_every=32, audit_seed=9)
+        assert int((lab != exact).sum()) == 0, st
+        # no band: network labels except for the audit sample, which takes the exact labels
+        pred = net.predict(pts)
+        netlab = (pred > 0.5).to(torch.int8)
+        lab0, st0 = net.label(mesh, pts, x0, seg, 0.0, audit_every=8, audit_seed=2)
+        assert st0["audit"] > n // 16 and st0["band"] <= 16
+        changed = lab0 != netlab
+        assert torch.equal(lab0[changed], exact[changed])
+        assert int(changed.sum()) == st0["audit_disagree"]
+
+
+def test_bad_arguments(gpu, c1):
+    import torch
+
+    v, idx, lo, hi = c1
+    with pytest.raises(gpu.AgwnError):
+        gpu.GwnSurrogate(hi, lo)
+    with gpu.GwnSurrogate(lo, hi) as net:
+        pts = torch.zeros((256, 3), dtype=torch.float32, device="cuda")
+        y = torch.zeros(256, dtype=torch.float32, device="cuda")
+        with pytest.raises(gpu.AgwnError, match="multiple of 128"):
+            net.train(pts, y, batch=100, epochs=1)
+        with gpu.HipMesh(v.astype(np.float32), idx, dtype=np.float32) as mesh:
+            with pytest.raises(gpu.AgwnError, match="delta"):
+                net.label(mesh, pts, mg.example_x0(np.float32), None, -1.0)
+        with gpu.HipMesh(v, idx, dtype=np.float64) as mesh64:
+            with pytest.raises(gpu.AgwnError, match="f32 mesh"):
+                net.label(mesh64, pts, mg.example_x0(np.float32), None, 0.1)
