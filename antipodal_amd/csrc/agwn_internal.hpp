@@ -168,10 +168,11 @@ int frac_f64_src(Src const& src, int64_t n, double const* seg7, int64_t nseg, do
                  int32_t const* counts, double* out, hipStream_t stream);
 
 // Surrogate routing (agwn_mlp.hip), instantiated for src_array<float> and src_grid:
-//   mlp_route      network labels of n points -> labels; the band / audit points are appended to list (2*i + audit)
-//                  and counted in counters[0] (band), [1] (audit), [2] (list length); counters start zeroed
+//   mlp_route      network labels of n points -> labels; the band / near-surface / audit points are appended to list
+//                  (2*i + audit) and counted in counters[0] (band), [1] (near surface or outside the box), [2] (audit),
+//                  [3] (list length); counters (5 of them) start zeroed
 //   gather_routed  pts[e] = the query of list entry e (the coordinates the exact kernels would compute)
-//   route_apply    labels of list entries from their exact values w; audit disagreements counted in counters[3]
+//   route_apply    labels of list entries from their exact values w; audit disagreements counted in counters[4]
 template <class Src>
 int mlp_route(agwn_mlp const* mlp, Src const& src, int64_t n, float delta, uint32_t audit_mod, uint32_t audit_seed,
               int8_t* labels, int64_t* list, unsigned long long* counters, hipStream_t s);

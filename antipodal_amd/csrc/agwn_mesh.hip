@@ -853,11 +853,11 @@ int label_surrogate(mesh_t<float>& mesh, agwn_mlp const* mlp, Src const& src, in
                     uint64_t audit_seed, int8_t* labels, int64_t* stats, hipStream_t s)
 {
     AGWN_CHECK(mlp != nullptr, "mlp handle is null");
-    AGWN_CHECK(std::isfinite(delta) && delta >= 0.0f, "delta must be finite and >= 0");
+    AGWN_CHECK(delta >= 0.0f, "delta must be >= 0 (+inf routes every point)"); // rejects NaN
     AGWN_CHECK(audit_every >= 0 && audit_every <= int64_t(UINT32_MAX), "audit_every out of range");
     AGWN_CHECK(n == 0 || labels != nullptr, "labels is null");
     AGWN_CHECK(nseg == 0 || seg7 != nullptr, "boundary is null with nseg=%lld", (long long)nseg);
-    unsigned long long cnt[4] = {0, 0, 0, 0};
+    unsigned long long cnt[5] = {0, 0, 0, 0, 0};
     if (n > 0)
     {
         dbuf dc, list;
@@ -868,7 +868,7 @@ int label_surrogate(mesh_t<float>& mesh, agwn_mlp const* mlp, Src const& src, in
                            list.as<int64_t>(), dc.as<unsigned long long>(), s));
         AGWN_HIP(hipMemcpyAsync(cnt, dc.p, sizeof(cnt), hipMemcpyDeviceToHost, s));
         AGWN_HIP(hipStreamSynchronize(s));
-        int64_t const m = int64_t(cnt[2]);
+        int64_t const m = int64_t(cnt[3]);
         if (m > 0)
         {
             double const x0d[3] = {double(x0f[0]), double(x0f[1]), double(x0f[2])};
@@ -885,12 +885,8 @@ int label_surrogate(mesh_t<float>& mesh, agwn_mlp const* mlp, Src const& src, in
         }
     }
     if (stats)
-    {
-        stats[0] = int64_t(cnt[0]);
-        stats[1] = int64_t(cnt[1]);
-        stats[2] = int64_t(cnt[2]);
-        stats[3] = int64_t(cnt[3]);
-    }
+        for (int k = 0; k < 5; ++k)
+            stats[k] = int64_t(cnt[k]);
     return ok;
 }
 

@@ -75,10 +75,13 @@ struct feat_params
     float w[kFreq];
 };
 
-__device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
+// SiLU and its derivative with the hardware reciprocal (1 ulp): an IEEE division costs ~10 VALU instructions, which
+// would dominate the fused inference kernel's epilogues. Training and inference share these definitions.
+__device__ __forceinline__ float sigmoid(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
+__device__ __forceinline__ float silu(float z) { return z * sigmoid(z); }
 __device__ __forceinline__ float silu_grad(float z)
 {
-    float const s = 1.0f / (1.0f + __expf(-z));
+    float const s = sigmoid(z);
     return s * (1.0f + z * (1.0f - s));
 }
 
@@ -435,8 +438,9 @@ __global__ void __launch_bounds__(kB) k_perm(int64_t* __restrict__ perm, int64_t
 // out^T[n][m] = sum_k Wt[n][k] act[m][k]: wave w owns units [64w, 64w + 64) (A operand = Wt rows, from L2) for all 128
 // points (B operand = LDS rows), so a lane's 4 consecutive accumulators are 4 consecutive units of one point and go
 // back to LDS as one 8-byte write after a barrier. The output layer is a 256-long dot per point.
-// Optionally labels the points (pred > 0.5) and appends those with |pred - 0.5| <= delta (and a hashed audit sample
-// of the rest) to a routing list: entry = 2 * index + (audit ? 1 : 0).
+// Optionally labels the points (pred > 0.5) and appends to a routing list (entry = 2 * index + (audit ? 1 : 0)) the
+// points in the near-surface mask or outside the box, those with |pred - 0.5| <= delta, and a hashed audit sample of
+// the rest.
 // ---------------------------------------------------------------------------------------------------------------------
 constexpr int kTileP = 128;
 constexpr int kLdsStride = kH + 8; // BF16 per LDS row: 528 B rows keep the 16-lane groups of ds_read_b128 conflict-free
@@ -449,6 +453,15 @@ struct infer_params
     float const* bo;
 };
 
+// Near-surface voxel mask over the training box: bit ((iz*ny)+iy)*nx+ix is set when some triangle comes within eps
+// of that voxel (conservatively). Points in set voxels, and points outside the box, are not the network's to label.
+struct surface_mask
+{
+    uint32_t const* bits; // null: no mask
+    float lo[3], inv[3];  // voxel coordinate = (p - lo) * inv
+    int32_t n[3];
+};
+
 struct route_params
 {
     float delta;
@@ -456,8 +469,94 @@ struct route_params
     uint32_t audit_seed;
     int8_t* labels;
     int64_t* list;
-    unsigned long long* counters; // [0] band, [1] audit, [2] list length, [3] audit disagreements
+    unsigned long long* counters; // [0] band, [1] near surface / outside the box, [2] audit, [3] list length,
+                                  // [4] audit disagreements
+    surface_mask mask;
 };
+
+__device__ __forceinline__ bool near_surface(surface_mask const& m, float x, float y, float z)
+{
+    if (!m.bits)
+        return false;
+    float const f[3] = {(x - m.lo[0]) * m.inv[0], (y - m.lo[1]) * m.inv[1], (z - m.lo[2]) * m.inv[2]};
+    int iv[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+    {
+        if (!(f[a] >= 0.0f && f[a] < float(m.n[a]))) // outside the box, or NaN
+            return true;
+        iv[a] = min(int(f[a]), m.n[a] - 1);
+    }
+    int64_t const v = (int64_t(iv[2]) * m.n[1] + iv[1]) * m.n[0] + iv[0];
+    return (m.bits[v >> 5] >> (v & 31)) & 1u;
+}
+
+// Marks the voxels within eps of triangle t: every voxel of the triangle's bounding box grown by eps that also passes
+// the separating-plane test along the triangle normal, |n . (c - a)| <= eps + sum_k |n_k| h_k (c: voxel centre, h:
+// half voxel). Both tests are necessary conditions for a point of the voxel to lie within eps of the triangle, so the
+// mask is conservative; slack covers float rounding.
+__global__ void __launch_bounds__(kB) k_mark_surface(float const* __restrict__ v, int32_t const* __restrict__ idx,
+                                                     int64_t ntri, surface_mask m, float eps,
+                                                     uint32_t* __restrict__ bits)
+{
+    int64_t const t = int64_t(blockIdx.x) * kB + threadIdx.x;
+    if (t >= ntri)
+        return;
+    float p[3][3];
+    for (int c = 0; c < 3; ++c)
+        for (int a = 0; a < 3; ++a)
+            p[c][a] = v[3 * int64_t(idx[3 * t + c]) + a];
+    int i0[3], i1[3];
+    for (int a = 0; a < 3; ++a)
+    {
+        float const lo = fminf(fminf(p[0][a], p[1][a]), p[2][a]) - eps;
+        float const hi = fmaxf(fmaxf(p[0][a], p[1][a]), p[2][a]) + eps;
+        float const f0 = floorf((lo - m.lo[a]) * m.inv[a]);
+        float const f1 = floorf((hi - m.lo[a]) * m.inv[a]);
+        if (!(f1 >= 0.0f && f0 < float(m.n[a])))
+            return; // entirely outside the box (or NaN coordinates)
+        i0[a] = f0 < 0.0f ? 0 : int(f0);
+        i1[a] = f1 >= float(m.n[a]) ? m.n[a] - 1 : int(f1);
+    }
+    double const e1[3] = {double(p[1][0]) - p[0][0], double(p[1][1]) - p[0][1], double(p[1][2]) - p[0][2]};
+    double const e2[3] = {double(p[2][0]) - p[0][0], double(p[2][1]) - p[0][1], double(p[2][2]) - p[0][2]};
+    double nx = e1[1] * e2[2] - e1[2] * e2[1], ny = e1[2] * e2[0] - e1[0] * e2[2], nz = e1[0] * e2[1] - e1[1] * e2[0];
+    double const nl = sqrt(nx * nx + ny * ny + nz * nz);
+    bool const plane = nl > 0.0;
+    if (plane)
+    {
+        nx /= nl;
+        ny /= nl;
+        nz /= nl;
+    }
+    double const h[3] = {0.5 / m.inv[0], 0.5 / m.inv[1], 0.5 / m.inv[2]};
+    double const slack = 1e-5 * (fabs(double(p[0][0])) + fabs(double(p[0][1])) + fabs(double(p[0][2])) + 2.0 * h[0] +
+                                 2.0 * h[1] + 2.0 * h[2]);
+    double const reach = double(eps) + fabs(nx) * h[0] + fabs(ny) * h[1] + fabs(nz) * h[2] + slack;
+    for (int iz = i0[2]; iz <= i1[2]; ++iz)
+        for (int iy = i0[1]; iy <= i1[1]; ++iy)
+            for (int ix = i0[0]; ix <= i1[0]; ++ix)
+            {
+                if (plane)
+                {
+                    double const cx = double(m.lo[0]) + (ix + 0.5) * 2.0 * h[0] - p[0][0];
+                    double const cy = double(m.lo[1]) + (iy + 0.5) * 2.0 * h[1] - p[0][1];
+                    double const cz = double(m.lo[2]) + (iz + 0.5) * 2.0 * h[2] - p[0][2];
+                    if (fabs(nx * cx + ny * cy + nz * cz) > reach)
+                        continue;
+                }
+                int64_t const vi = (int64_t(iz) * m.n[1] + iy) * m.n[0] + ix;
+                atomicOr(bits + (vi >> 5), 1u << (vi & 31));
+            }
+}
+
+__global__ void __launch_bounds__(kB) k_near_surface(float const* __restrict__ pts, int64_t n, surface_mask m,
+                                                     int8_t* __restrict__ out)
+{
+    int64_t const i = int64_t(blockIdx.x) * kB + threadIdx.x;
+    if (i < n)
+        out[i] = near_surface(m, pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]) ? 1 : 0;
+}
 
 __device__ __forceinline__ uint32_t hash32(uint64_t x)
 {
@@ -569,37 +668,48 @@ __global__ void __launch_bounds__(256, 2) k_mlp_infer(Src const src, int64_t n, 
         pred[i] = pv;
     if constexpr (ROUTE)
     {
+        bool near = false;
+        if (live)
+        {
+            float x, y, z;
+            query_at(src, i, x, y, z);
+            near = near_surface(RP.mask, x, y, z);
+        }
         float const d = fabsf(pv - 0.5f);
-        bool const band = live && !(d > RP.delta); // NaN routes too
-        bool const audit = live && !band && RP.audit_mod != 0 &&
+        bool const band = live && !near && !(d > RP.delta); // NaN routes too
+        bool const audit = live && !near && !band && RP.audit_mod != 0 &&
                            hash32(uint64_t(i) * 0x9E3779B97F4A7C15ull + RP.audit_seed) % RP.audit_mod == 0;
         if (live)
             RP.labels[i] = pv > 0.5f ? 1 : 0;
-        bool const take = band || audit;
-        uint64_t const mask = __ballot(take);
-        uint64_t const mband = __ballot(band);
-        if (mask == 0)
+        bool const take = near || band || audit;
+        uint64_t const mtake = __ballot(take);
+        if (mtake == 0)
             return;
-        int const leader = __ffsll((unsigned long long)mask) - 1;
+        uint64_t const mband = __ballot(band);
+        uint64_t const mnear = __ballot(near);
+        int const leader = __ffsll((unsigned long long)mtake) - 1;
         unsigned long long basei = 0;
         if (lane == leader)
         {
-            basei = atomicAdd(RP.counters + 2, (unsigned long long)__popcll(mask));
-            atomicAdd(RP.counters + 0, (unsigned long long)__popcll(mband));
-            if (mask != mband)
-                atomicAdd(RP.counters + 1, (unsigned long long)__popcll(mask & ~mband));
+            basei = atomicAdd(RP.counters + 3, (unsigned long long)__popcll(mtake));
+            if (mband)
+                atomicAdd(RP.counters + 0, (unsigned long long)__popcll(mband));
+            if (mnear)
+                atomicAdd(RP.counters + 1, (unsigned long long)__popcll(mnear));
+            if (mtake & ~(mband | mnear))
+                atomicAdd(RP.counters + 2, (unsigned long long)__popcll(mtake & ~(mband | mnear)));
         }
         basei = __shfl(basei, leader);
         if (take)
         {
-            int const rank = __popcll(mask & ((1ull << lane) - 1ull));
+            int const rank = __popcll(mtake & ((1ull << lane) - 1ull));
             RP.list[basei + rank] = 2 * i + (audit ? 1 : 0);
         }
     }
 }
 
 // Applies the exact labels of the routed points: w[e] is the exact GWN of list entry e. Audit entries whose
-// surrogate label disagrees are counted in counters[3].
+// surrogate label disagrees are counted in counters[4].
 __global__ void __launch_bounds__(kB) k_apply(int64_t const* __restrict__ list, int64_t m, float const* __restrict__ w,
                                               int8_t* __restrict__ labels, unsigned long long* __restrict__ counters)
 {
@@ -610,7 +720,7 @@ __global__ void __launch_bounds__(kB) k_apply(int64_t const* __restrict__ list, 
     int64_t const i = code >> 1;
     int8_t const lab = w[e] > 0.5f ? 1 : 0;
     if ((code & 1) && labels[i] != lab)
-        atomicAdd(counters + 3, 1ull);
+        atomicAdd(counters + 4, 1ull);
     labels[i] = lab;
 }
 
@@ -644,6 +754,9 @@ struct mlp_impl
     float* g = nullptr; // gradient
     __bf16* wr[kL] = {};
     __bf16* wt[kL] = {};
+    uint32_t* mask_bits = nullptr;
+    surface_mask mask{};
+    float mask_eps = 0.0f;
     int64_t step = 0;
     std::mutex mu;
 
@@ -654,6 +767,8 @@ struct mlp_impl
         for (float* p : {w, m1, m2, g})
             if (p)
                 (void)hipFree(p);
+        if (mask_bits)
+            (void)hipFree(mask_bits);
         for (int l = 0; l < kL; ++l)
         {
             if (wr[l])
@@ -1030,6 +1145,91 @@ int infer_launch(mlp_impl const& M, Src const& src, int64_t n, float* pred, rout
 
 int64_t mlp_num_params() { return kLay.total; }
 
+// Builds the near-surface mask of the network's box: voxels of edge `voxel` (<= 0: eps / 2) that come within eps of a
+// triangle of the mesh (host arrays).
+int mlp_set_surface(agwn_mlp* h, float const* verts, int64_t nv, int32_t const* idx, int64_t nidx, float eps,
+                    float voxel)
+{
+    AGWN_CHECK(h != nullptr, "mlp handle is null");
+    AGWN_CHECK(nv >= 0 && nidx >= 0 && nidx % 3 == 0, "bad mesh sizes (nv=%lld, nidx=%lld)", (long long)nv,
+               (long long)nidx);
+    AGWN_CHECK(nidx == 0 || (verts && idx), "null mesh arrays");
+    AGWN_CHECK(std::isfinite(eps) && eps >= 0.0f, "eps must be finite and >= 0");
+    for (int64_t i = 0; i < nidx; ++i)
+        AGWN_CHECK(idx[i] >= 0 && idx[i] < nv, "index %lld = %d out of range [0, %lld)", (long long)i, idx[i],
+                   (long long)nv);
+    if (!(voxel > 0.0f))
+        voxel = eps > 0.0f ? 0.5f * eps : 0.0f;
+    mlp_impl& M = h->impl;
+    surface_mask m{};
+    int64_t nvox = 1;
+    for (int a = 0; a < 3; ++a)
+    {
+        double const ext = double(M.hi[a]) - double(M.lo[a]);
+        double const cells = voxel > 0.0f ? std::ceil(ext / double(voxel)) : 1.0;
+        AGWN_CHECK(cells <= double(1 << 20), "voxel %g too small for the box", double(voxel));
+        m.n[a] = std::max(1, int(cells));
+        m.lo[a] = M.lo[a];
+        m.inv[a] = float(double(m.n[a]) / ext);
+        nvox *= m.n[a];
+    }
+    AGWN_CHECK(nvox <= (int64_t(1) << 33), "mask of %lld voxels is too large", (long long)nvox);
+    std::lock_guard<std::mutex> lk(M.mu);
+    hipStream_t s = nullptr;
+    AGWN_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    struct sguard
+    {
+        hipStream_t s;
+        ~sguard() { (void)hipStreamDestroy(s); }
+    } sg{s};
+    size_t const words = size_t((nvox + 31) / 32);
+    uint32_t* bits = nullptr;
+    AGWN_HIP(hipMalloc(&bits, words * sizeof(uint32_t)));
+    struct bguard
+    {
+        uint32_t* p;
+        ~bguard()
+        {
+            if (p)
+                (void)hipFree(p);
+        }
+    } bg{bits};
+    AGWN_HIP(hipMemsetAsync(bits, 0, words * sizeof(uint32_t), s));
+    int64_t const ntri = nidx / 3;
+    if (ntri > 0)
+    {
+        dbuf dv, di;
+        AGWN_TRY(dv.alloc(size_t(nv) * 3 * sizeof(float), s));
+        AGWN_TRY(di.alloc(size_t(nidx) * sizeof(int32_t), s));
+        AGWN_HIP(hipMemcpyAsync(dv.p, verts, size_t(nv) * 3 * sizeof(float), hipMemcpyHostToDevice, s));
+        AGWN_HIP(hipMemcpyAsync(di.p, idx, size_t(nidx) * sizeof(int32_t), hipMemcpyHostToDevice, s));
+        k_mark_surface<<<dim3(unsigned((ntri + kB - 1) / kB)), dim3(kB), 0, s>>>(dv.as<float>(), di.as<int32_t>(), ntri,
+                                                                                m, eps, bits);
+        AGWN_HIP(hipGetLastError());
+    }
+    AGWN_HIP(hipStreamSynchronize(s));
+    if (M.mask_bits)
+        AGWN_HIP(hipFree(M.mask_bits));
+    M.mask_bits = bits;
+    bg.p = nullptr;
+    m.bits = bits;
+    M.mask = m;
+    M.mask_eps = eps;
+    return ok;
+}
+
+int mlp_near_surface_dev(agwn_mlp const* h, float const* pts, int64_t n, int8_t* out, hipStream_t s)
+{
+    AGWN_CHECK(h != nullptr, "mlp handle is null");
+    AGWN_CHECK(n >= 0 && (n == 0 || (pts && out)), "bad arrays");
+    AGWN_CHECK(h->impl.mask.bits != nullptr, "the surface mask was not set");
+    if (n == 0)
+        return ok;
+    k_near_surface<<<dim3(unsigned((n + kB - 1) / kB)), dim3(kB), 0, s>>>(pts, n, h->impl.mask, out);
+    AGWN_HIP(hipGetLastError());
+    return ok;
+}
+
 int mlp_predict_dev(agwn_mlp const* h, float const* pts, int64_t n, float* out, hipStream_t s)
 {
     AGWN_CHECK(h != nullptr, "mlp handle is null");
@@ -1043,7 +1243,7 @@ int mlp_route(agwn_mlp const* h, Src const& src, int64_t n, float delta, uint32_
               int8_t* labels, int64_t* list, unsigned long long* counters, hipStream_t s)
 {
     AGWN_CHECK(h != nullptr, "mlp handle is null");
-    route_params RP{delta, audit_mod, audit_seed, labels, list, counters};
+    route_params RP{delta, audit_mod, audit_seed, labels, list, counters, h->impl.mask};
     return infer_launch<Src, true>(h->impl, src, n, nullptr, RP, s);
 }
 template int mlp_route<src_array<float>>(agwn_mlp const*, src_array<float> const&, int64_t, float, uint32_t,
@@ -1149,5 +1349,14 @@ int agwn_mlp_grad(agwn_mlp* m, float const* pts, float const* target, int64_t n,
 int agwn_mlp_predict_dev(agwn_mlp const* m, float const* pts, int64_t n, float* out, agwn_stream s)
 {
     return guarded_mlp([&] { return agwn::mlp_predict_dev(m, pts, n, out, hipStream_t(s)); });
+}
+int agwn_mlp_set_surface(agwn_mlp* m, float const* verts, int64_t nv, int32_t const* idx, int64_t nidx, float eps,
+                         float voxel)
+{
+    return guarded_mlp([&] { return agwn::mlp_set_surface(m, verts, nv, idx, nidx, eps, voxel); });
+}
+int agwn_mlp_near_surface_dev(agwn_mlp const* m, float const* pts, int64_t n, int8_t* out, agwn_stream s)
+{
+    return guarded_mlp([&] { return agwn::mlp_near_surface_dev(m, pts, n, out, hipStream_t(s)); });
 }
 } // extern "C"

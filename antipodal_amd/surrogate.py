@@ -81,6 +81,24 @@ class GwnSurrogate:
         check(lib.agwn_mlp_grad(self._h, ptr(p), ptr(y), len(y), ptr(g), ptr(pred), C.byref(loss)))
         return g, pred, float(loss.value)
 
+    def set_surface(self, verts, idx, eps: float, voxel: float = 0.0) -> None:
+        """Near-surface mask: voxels (edge `voxel`, default eps/2) within eps of a triangle are routed to the exact
+        path, as are points outside the box."""
+        v = np.ascontiguousarray(np.asarray(verts, dtype=np.float32).reshape(-1, 3))
+        i = np.ascontiguousarray(np.asarray(idx, dtype=np.int32).reshape(-1))
+        check(lib.agwn_mlp_set_surface(self._h, ptr(v), len(v), ptr(i), i.size, C.c_float(eps), C.c_float(voxel)))
+        self.eps = float(eps)
+
+    def near_surface(self, pts, out=None, stream=None):
+        """int8 flags of device points: 1 = in the near-surface mask or outside the box."""
+        import torch
+
+        n = pts.numel() // 3
+        if out is None:
+            out = torch.empty(n, dtype=torch.int8, device=pts.device)
+        check(lib.agwn_mlp_near_surface_dev(self._h, ptr(pts), n, ptr(out), _stream_ptr(stream)))
+        return out
+
     def predict(self, pts, out=None, stream=None):
         """Network output of device points (n, 3) float32."""
         import torch
@@ -100,7 +118,7 @@ class GwnSurrogate:
         if labels is None:
             labels = torch.empty(n, dtype=torch.int8, device=pts.device)
         nseg = 0 if boundary is None else boundary.numel() // 7
-        st = np.zeros(4, dtype=np.int64)
+        st = np.zeros(5, dtype=np.int64)
         check(lib.agwn_label_surrogate_dev(mesh.handle, self._h, ptr(pts), n, ptr(_x0(x0, np.float32)),
                                            ptr(boundary) if nseg else None, nseg, C.c_float(delta), int(audit_every),
                                            C.c_uint64(audit_seed), ptr(labels), ptr(st), _stream_ptr(stream)))
@@ -115,7 +133,7 @@ class GwnSurrogate:
         if labels is None:
             labels = torch.empty(n, dtype=torch.int8, device="cuda")
         vs = [np.ascontiguousarray(np.asarray(a, dtype=np.float32).reshape(3)) for a in (start, dx, dy, dz)]
-        st = np.zeros(4, dtype=np.int64)
+        st = np.zeros(5, dtype=np.int64)
         check(lib.agwn_grid_label_surrogate_dev(grid._h, self._h, *[ptr(a) for a in vs], nx, ny, nz, C.c_float(delta),
                                                 int(audit_every), C.c_uint64(audit_seed), ptr(labels), ptr(st),
                                                 _stream_ptr(stream)))
@@ -140,14 +158,15 @@ class GwnSurrogate:
 
 
 def _stats(st: np.ndarray, n: int) -> dict:
-    return {"n": n, "band": int(st[0]), "audit": int(st[1]), "routed": int(st[2]), "audit_disagree": int(st[3]),
-            "fallback_fraction": float(st[2]) / max(n, 1)}
+    return {"n": n, "band": int(st[0]), "near": int(st[1]), "audit": int(st[2]), "routed": int(st[3]),
+            "audit_disagree": int(st[4]), "fallback_fraction": float(st[3]) / max(n, 1)}
 
 
-def training_samples(verts, idx, lo, hi, n: int, near_frac: float = 0.7, band: float = 0.05, seed: int = 0):
-    """n float32 points: near_frac of them at signed normal offsets t = band * s * u^2 (s = +-1, u ~ U(0,1)) from
-    area-uniform surface points (so all within `band` of the surface, denser close to it), the rest uniform in
-    [lo, hi]."""
+def training_samples(verts, idx, lo, hi, n: int, near_frac: float = 0.7, band: float = 0.05, seed: int = 0,
+                     power: float = 2.0):
+    """n float32 points: near_frac of them at signed normal offsets t = band * s * u^power (s = +-1, u ~ U(0,1)) from
+    area-uniform surface points (so all within `band` of the surface; power > 1 puts more of them close to it), the
+    rest uniform in [lo, hi]."""
     rng = np.random.default_rng(seed)
     v = np.asarray(verts, dtype=np.float64).reshape(-1, 3)
     tri = np.asarray(idx, dtype=np.int64).reshape(-1, 3)
@@ -163,7 +182,7 @@ def training_samples(verts, idx, lo, hi, n: int, near_frac: float = 0.7, band: f
     r1 = np.sqrt(rng.random(n_near))
     r2 = rng.random(n_near)
     p = (1 - r1)[:, None] * a[k] + (r1 * (1 - r2))[:, None] * b[k] + (r1 * r2)[:, None] * c[k]
-    off = band * np.where(rng.random(n_near) < 0.5, -1.0, 1.0) * rng.random(n_near) ** 2
+    off = band * np.where(rng.random(n_near) < 0.5, -1.0, 1.0) * rng.random(n_near) ** power
     p += off[:, None] * nrm[k]
     lo = np.asarray(lo, dtype=np.float64)
     hi = np.asarray(hi, dtype=np.float64)
@@ -182,15 +201,17 @@ def calibrate_delta(pred: np.ndarray, exact: np.ndarray, factor: float = 1.5, ma
 
 def fit_surrogate(verts, idx, boundary, x0, lo, hi, n_train: int = 1 << 24, n_val: int = 1 << 20,
                   epochs: int = 8, batch: int = 16384, lr: float = 2e-3, seed: int = 0, mesh: HipMesh | None = None,
-                  max_freq: float = 128.0, jitter: float | None = None, target_clip=(-0.25, 1.25),
-                  factor: float = 1.5, margin: float = 0.02, log=None):
+                  max_freq: float = 64.0, eps: float | None = None, voxel: float = 0.0, jitter: float = 0.0,
+                  target_clip=(-0.25, 1.25), factor: float = 1.5, margin: float = 0.02, power: float = 1.0,
+                  log=None):
     """Trains a GwnSurrogate for the mesh on exact FP32 GWN values computed by the library, and picks the routing band.
 
-    The network sees each training point displaced by Gaussian noise of standard deviation `jitter` (default 0.35 %
-    of the box diagonal) while its target stays the exact GWN of the undisplaced point: the mean-squared-error optimum
-    is then the GWN blurred at that scale, so predictions fall towards 0.5 within a few `jitter` of the surface --
-    where a sharp fit would be confidently wrong about a noisy surface -- and stay near 0 or 1 elsewhere. The band
-    half-width delta is calibrated on undisplaced validation points.
+    The surrogate only ever labels points farther than `eps` from the surface (default 0.25 % of the box diagonal):
+    a conservative voxel mask of the surface's eps-neighbourhood sends everything closer -- where the GWN jumps and a
+    noisy surface can fold -- to the exact path. Off the surface the GWN is harmonic and smooth, which is what the
+    network is trained on (samples inside the mask are dropped) and calibrated on: delta = factor x the largest
+    |pred - 0.5| of a wrongly labelled unmasked validation point, plus margin. `jitter` (default 0) displaces the
+    training inputs by Gaussian noise of that standard deviation, keeping the targets of the undisplaced points.
 
     Returns (surrogate, delta, info) -- info holds the timings, per-epoch losses and validation statistics.
     """
@@ -201,25 +222,41 @@ def fit_surrogate(verts, idx, boundary, x0, lo, hi, n_train: int = 1 << 24, n_va
     own_mesh = mesh is None
     if own_mesh:
         mesh = HipMesh(np.asarray(verts, dtype=np.float32), idx, dtype=np.float32)
+    diag = float(np.linalg.norm(np.asarray(hi, dtype=np.float64) - np.asarray(lo, dtype=np.float64)))
+    if eps is None:
+        eps = 0.0025 * diag
+    net = GwnSurrogate(lo, hi, max_freq=max_freq, seed=seed)
+    net.set_surface(verts, idx, eps, voxel)
     seg = torch.from_numpy(np.ascontiguousarray(np.asarray(boundary, dtype=np.float32).reshape(-1, 7))).cuda()
-    pts_h = training_samples(verts, idx, lo, hi, n_train + n_val, seed=seed)
+    # candidates, then drop those the mask routes anyway; oversample so that enough remain
+    want = n_train + n_val
+    keep_parts, have, rnd = [], 0, 0
+    while have < want and rnd < 4:
+        cand = torch.from_numpy(training_samples(verts, idx, lo, hi, int(want * 1.6) + 1024, seed=seed + 7919 * rnd,
+                                                 power=power)).cuda()
+        ok = net.near_surface(cand) == 0
+        keep_parts.append(cand[ok])
+        have += int(ok.sum())
+        rnd += 1
+    pts = torch.cat(keep_parts)[:want].contiguous()
+    if len(pts) < want:
+        n_train = max(1, int(len(pts) * n_train / want))
+        n_val = len(pts) - n_train
     info["sample_s"] = time.perf_counter() - t0
     t1 = time.perf_counter()
-    pts = torch.from_numpy(pts_h).cuda()
-    w = torch.empty(len(pts_h), dtype=torch.float32, device="cuda")
+    w = torch.empty(len(pts), dtype=torch.float32, device="cuda")
     mesh.full(pts, x0, seg if len(seg) else None, out=w)
     torch.cuda.synchronize()
     info["label_s"] = time.perf_counter() - t1
-    if jitter is None:
-        jitter = 0.0035 * float(np.linalg.norm(np.asarray(hi, dtype=np.float64) - np.asarray(lo, dtype=np.float64)))
-    gen = torch.Generator(device="cuda").manual_seed(seed + 1)
-    tr_p = (pts[:n_train] + jitter * torch.randn((n_train, 3), device="cuda", generator=gen)).contiguous()
+    tr_p = pts[:n_train]
+    if jitter > 0:
+        gen = torch.Generator(device="cuda").manual_seed(seed + 1)
+        tr_p = tr_p + jitter * torch.randn((n_train, 3), device="cuda", generator=gen)
+    tr_p = tr_p.contiguous()
     va_p = pts[n_train:].contiguous()
     tr_y = w[:n_train].clamp(*target_clip).contiguous()
     va_y = w[n_train:]
-    info["jitter"] = jitter
-    info["max_freq"] = max_freq
-    net = GwnSurrogate(lo, hi, max_freq=max_freq, seed=seed)
+    info.update(eps=eps, jitter=jitter, max_freq=max_freq, n_train=n_train)
     t2 = time.perf_counter()
     losses = net.train(tr_p, tr_y, batch=batch, epochs=epochs, lr=lr, seed=seed)
     torch.cuda.synchronize()
@@ -229,11 +266,12 @@ def fit_surrogate(verts, idx, boundary, x0, lo, hi, n_train: int = 1 << 24, n_va
     exact = va_y.cpu().numpy()
     wrong = (pred > 0.5) != (exact > 0.5)
     delta = calibrate_delta(pred, exact, factor, margin)
+    info["val_pred"], info["val_exact"] = pred, exact
     info.update(val_points=n_val, val_wrong=int(wrong.sum()),
                 val_max_wrong=float(np.max(np.abs(pred[wrong] - 0.5))) if wrong.any() else 0.0, delta=delta,
                 val_band_fraction=float(np.mean(np.abs(pred - 0.5) <= delta)), fit_s=time.perf_counter() - t0)
     if log:
-        log(f"surrogate: {info}")
+        log("surrogate: " + str({k: v for k, v in info.items() if k not in ("val_pred", "val_exact")}))
     if own_mesh:
         mesh.close()
     return net, delta, info

@@ -6,11 +6,13 @@ Workload: seeded noisy torus R=1, r=0.35, 1353 x 473 quads (2 triangles each), s
 lattice over the bounding box expanded by 10 % (134,217,728 points, generated in-kernel); FP32.
 
   plain      one eval_grid (HipGridGwn): full GWN (fractional + integer term) of every lattice point, labels = w > 0.5
-  surrogate  a per-mesh MLP (192 Fourier features, 4 x 256 SiLU, BF16 MFMA) trained once on 2^24 exact GWN samples
-             (70 % within 0.05 of the surface, 30 % uniform in the box); each step labels the lattice with the fused
-             inference kernel, routes points with |pred - 0.5| <= delta plus a 1/256 audit sample to the exact kernels,
-             and takes their exact labels. delta comes from a held-out validation set (1.5 x the worst wrong-side
-             margin + 0.02). Training happens before the timed region and is reported (train_s).
+  surrogate  a per-mesh MLP (192 Fourier features up to 64 periods, 4 x 256 SiLU, BF16 MFMA) trained once on 2^24
+             exact GWN samples (70 % drawn within 0.05 of the surface, 30 % uniform in the box, minus those in the
+             near-surface mask); each step labels the lattice with the fused inference kernel and routes to the exact
+             kernels every point in the mask (voxels within eps = 0.15 % of the box diagonal of a triangle), every
+             point with |pred - 0.5| <= delta, and a 1/256 audit sample, which take their exact labels. delta comes
+             from a held-out validation set (1.5 x the worst wrong-side margin + 0.02). Training happens before the
+             timed region and is reported (train_s).
 The headline value is surrogate steps/s x queries; mislabels = labels that differ from the plain run's (must be 0).
 
 Reported beside it:
@@ -113,9 +115,15 @@ def main() -> int:
     ap.add_argument("--plain-steps", type=int, default=1)
     ap.add_argument("--train-samples", type=int, default=1 << 24)
     ap.add_argument("--epochs", type=int, default=16)
+    ap.add_argument("--max-freq", type=float, default=64.0)
+    ap.add_argument("--eps-frac", type=float, default=0.0015,
+                    help="near-surface mask radius as a fraction of the box diagonal")
     ap.add_argument("--batch", type=int, default=16384)
     ap.add_argument("--lr", type=float, default=3e-3)
     ap.add_argument("--audit-every", type=int, default=256)
+    ap.add_argument("--params-cache", type=str, default=None,
+                    help="npz file: load the trained surrogate (parameters, delta, fit record) from it if it exists, "
+                         "else train and save there (for profiler runs that should not retrain)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -173,9 +181,24 @@ def main() -> int:
     log(f"plain: {plain_ms:.1f} ms/step = {nq / plain_ms * 1e3 / 1e6:.2f} Mq/s")
 
     # ---- surrogate: fit (outside the timed region) -------------------------------------------------------------------
-    net, delta, fit = agwn.fit_surrogate(v32, idx, seg32, x0, lo.astype(np.float32), hi.astype(np.float32),
-                                         n_train=args.train_samples, n_val=1 << 20, epochs=args.epochs,
-                                         batch=args.batch, lr=args.lr, seed=0, mesh=mesh)
+    cache = Path(args.params_cache) if args.params_cache else None
+    if cache is not None and cache.exists():
+        with np.load(cache, allow_pickle=False) as z:
+            fit = json.loads(str(z["fit"]))
+            net = agwn.GwnSurrogate(lo.astype(np.float32), hi.astype(np.float32), max_freq=fit["max_freq"])
+            net.set_params(z["params"])
+            delta = float(z["delta"])
+        net.set_surface(v32, idx, fit["eps"])
+        log(f"surrogate loaded from {cache}")
+    else:
+        net, delta, fit = agwn.fit_surrogate(v32, idx, seg32, x0, lo.astype(np.float32), hi.astype(np.float32),
+                                             n_train=args.train_samples, n_val=1 << 20, epochs=args.epochs,
+                                             batch=args.batch, lr=args.lr, seed=0, mesh=mesh,
+                                             max_freq=args.max_freq,
+                                             eps=args.eps_frac * float(np.linalg.norm(hi - lo)))
+        fit = {k: v for k, v in fit.items() if k not in ("val_pred", "val_exact")}
+        if cache is not None:
+            np.savez(cache, params=net.get_params(), delta=delta, fit=json.dumps(fit))
     log(f"fit: {fit['fit_s']:.1f} s (train {fit['train_s']:.1f} s), losses {fit['losses'][0]:.4g} -> "
         f"{fit['losses'][-1]:.4g}, validation wrong {fit['val_wrong']}/{fit['val_points']}, delta {delta:.4f}")
 
@@ -304,8 +327,11 @@ def main() -> int:
         "mislabels": mislabels,
         "surrogate": {
             "delta": delta,
+            "eps": fit["eps"],
+            "max_freq": args.max_freq,
             "fallback_fraction": st["fallback_fraction"],
             "band": st["band"],
+            "near_surface": st["near"],
             "audit": st["audit"],
             "audit_disagree": sum(s["audit_disagree"] for s in stats),
             "train_samples": args.train_samples,

@@ -182,11 +182,19 @@ int agwn_mlp_grad(agwn_mlp* mlp, const float* pts, const float* target, int64_t 
                   float* loss);
 /* Fused inference (device arrays): out[i] = network(pts[i]). */
 int agwn_mlp_predict_dev(const agwn_mlp* mlp, const float* pts, int64_t n, float* out, agwn_stream stream);
+/* Near-surface mask of the network's box: voxels of edge `voxel` (<= 0: eps/2) within eps of a triangle of the mesh
+ * (host verts/idx), marked conservatively on the GPU. Points in marked voxels, or outside the box, are never labelled
+ * by the network. */
+int agwn_mlp_set_surface(agwn_mlp* mlp, const float* verts, int64_t nv, const int32_t* idx, int64_t nidx, float eps,
+                         float voxel);
+/* out[i] = 1 if pts[i] (device) is in the mask or outside the box, else 0 (device int8). */
+int agwn_mlp_near_surface_dev(const agwn_mlp* mlp, const float* pts, int64_t n, int8_t* out, agwn_stream stream);
 /* Surrogate + fallback labelling of points: labels[i] = (GWN(pts[i]) > 0.5). The network labels every point; points
- * with |pred - 0.5| <= delta, plus an audit sample of about 1/audit_every of the others (0: none, hashed with
- * audit_seed), go through the exact FP32 path (agwn_full_batch_f32 with this mesh, x0 and boundary) and take its label.
- * stats (host int64[4]): points routed by the band, audited, routed in all, audited points whose network label
- * disagreed with the exact one. Device pos/seg7/labels; host x0. Synchronous. */
+ * in the near-surface mask or outside the box, points with |pred - 0.5| <= delta, plus an audit sample of about
+ * 1/audit_every of the others (0: none, hashed with audit_seed), go through the exact FP32 path (agwn_full_batch_f32
+ * with this mesh, x0 and boundary) and take its label. stats (host int64[5], may be NULL): points routed by the band,
+ * by the mask, audited, routed in all, audited points whose network label disagreed with the exact one. Device
+ * pos/seg7/labels; host x0. Synchronous. */
 int agwn_label_surrogate_dev(const agwn_mesh* mesh, const agwn_mlp* mlp, const float* pos, int64_t n, const float* x0,
                              const float* seg7, int64_t nseg, float delta, int64_t audit_every, uint64_t audit_seed,
                              int8_t* labels, int64_t* stats, agwn_stream stream);

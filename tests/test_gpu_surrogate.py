@@ -101,13 +101,14 @@ def test_adam_first_step(gpu, c1):
 
 @pytest.fixture(scope="module")
 def trained_c1(gpu, c1):
-    """A network fitted to the small open mesh (icosphere cap): 2^16 training and 2^14 validation samples."""
+    """A network fitted to the small open mesh (icosphere cap): 2^18 training and 2^16 validation samples, top
+    frequency 32 (the sphere is smooth at the box scale)."""
     v, idx, lo, hi = c1
     bnd = gpu.build_boundary_segments(v.astype(np.float32), idx)
     x0 = mg.example_x0(np.float32)
     t = time.perf_counter()
-    net, delta, info = gpu.fit_surrogate(v.astype(np.float32), idx, bnd, x0, lo, hi, n_train=1 << 16, n_val=1 << 14,
-                                         epochs=40, batch=2048, lr=2e-3, seed=11)
+    net, delta, info = gpu.fit_surrogate(v.astype(np.float32), idx, bnd, x0, lo, hi, n_train=1 << 18, n_val=1 << 16,
+                                         epochs=30, batch=4096, lr=2e-3, seed=11, max_freq=32.0)
     print(f"fit {time.perf_counter() - t:.2f}s", info)
     yield net, delta, info, bnd, x0
     net.close()
@@ -149,7 +150,7 @@ def test_grid_labels_zero_mislabels(gpu, c1, trained_c1):
         assert st["fallback_fraction"] < 0.5, st
         # everything routed: the labels are the exact path's, bit for bit
         lab_all, st_all = net.label_grid(grid, start, dx, dy, dz, nres, nres, nres, float("inf"))
-        assert st_all["band"] == nres ** 3 and st_all["routed"] == nres ** 3
+        assert st_all["band"] + st_all["near"] == nres ** 3 and st_all["routed"] == nres ** 3
         assert torch.equal(lab_all, exact)
 
 
@@ -168,14 +169,47 @@ def test_point_labels_and_audit(gpu, c1, trained_c1):
         exact = (w > 0.5).to(torch.int8)
         lab, st = net.label(mesh, pts, x0, seg, delta, audit_every=32, audit_seed=9)
         assert int((lab != exact).sum()) == 0, st
-        # no band: network labels except for the audit sample, which takes the exact labels
+        # no band: network labels except for the near-surface points and the audit sample, which take the exact labels
         pred = net.predict(pts)
         netlab = (pred > 0.5).to(torch.int8)
+        near = net.near_surface(pts).bool()
         lab0, st0 = net.label(mesh, pts, x0, seg, 0.0, audit_every=8, audit_seed=2)
-        assert st0["audit"] > n // 16 and st0["band"] <= 16
-        changed = lab0 != netlab
+        assert st0["near"] == int(near.sum())
+        assert st0["audit"] > (n - st0["near"]) // 16 and st0["band"] <= 16
+        assert torch.equal(lab0[near], exact[near])
+        changed = (lab0 != netlab) & ~near
         assert torch.equal(lab0[changed], exact[changed])
         assert int(changed.sum()) == st0["audit_disagree"]
+
+
+def test_near_surface_mask_is_conservative(gpu, c1):
+    """Every point within eps of a triangle is in the mask; points outside the box are too."""
+    import torch
+
+    v, idx, lo, hi = c1
+    rng = np.random.default_rng(8)
+    tri = v[idx.reshape(-1, 3)]
+    k = rng.integers(0, len(tri), 100_000)
+    r1, r2 = np.sqrt(rng.random(len(k))), rng.random(len(k))
+    p = (1 - r1)[:, None] * tri[k, 0] + (r1 * (1 - r2))[:, None] * tri[k, 1] + (r1 * r2)[:, None] * tri[k, 2]
+    nrm = np.cross(tri[k, 1] - tri[k, 0], tri[k, 2] - tri[k, 0])
+    nrm /= np.linalg.norm(nrm, axis=1)[:, None]
+    eps = 0.02
+    d = rng.uniform(-0.999, 0.999, len(k)) * eps
+    # plus points displaced in an arbitrary direction by less than eps
+    u = rng.normal(size=(len(k), 3))
+    u /= np.linalg.norm(u, axis=1)[:, None]
+    pts = np.concatenate([p + d[:, None] * nrm, p + 0.999 * eps * rng.random(len(k))[:, None] * u])
+    outside = np.array([[hi[0] + 0.1, 0, 0], [lo[0] - 1e-3, 0, 0], [0, 0, hi[2] + 1e-3]], dtype=np.float32)
+    with gpu.GwnSurrogate(lo, hi, max_freq=32.0) as net:
+        net.set_surface(v.astype(np.float32), idx, eps)
+        flags = net.near_surface(torch.from_numpy(pts.astype(np.float32)).cuda()).cpu().numpy()
+        assert flags.all(), int((flags == 0).sum())
+        assert net.near_surface(torch.from_numpy(outside).cuda()).cpu().numpy().all()
+        far = (lo + (hi - lo) * rng.random((100_000, 3))).astype(np.float32)
+        frac = float(net.near_surface(torch.from_numpy(far).cuda()).float().mean())
+        print(f"near-surface mask: {frac:.3f} of the box (eps {eps})")
+        assert frac < 0.25
 
 
 def test_bad_arguments(gpu, c1):
