@@ -51,8 +51,24 @@ def _stream_ptr(stream) -> int | None:
     return getattr(stream, "cuda_stream", stream)
 
 
-def build_boundary_segments(vertices, indices) -> np.ndarray:
-    """Open boundary as (nseg, 7) weighted oriented segments, sorted by (min index, max index)."""
+def build_boundary_segments(vertices, indices, stream=None):
+    """Open boundary as (nseg, 7) weighted oriented segments, sorted by (min index, max index). Host numpy in ->
+    numpy out (host extraction); torch CUDA vertices (nv, 3) and int32 indices -> a device tensor (GPU extraction)."""
+    if _is_dev(vertices):
+        import torch
+
+        s = sfx(vertices.dtype)
+        if not _is_dev(indices) or indices.dtype != torch.int32:
+            raise TypeError("device extraction takes int32 device indices")
+        nidx = indices.numel()
+        if nidx % 3:
+            raise ValueError("indices.size() must be a multiple of 3")
+        out = torch.empty((max(nidx, 1), 7), dtype=vertices.dtype, device=vertices.device)
+        n = C.c_int64(0)
+        fn = getattr(lib, f"agwn_boundary_{s}_dev")
+        check(fn(ptr(vertices), vertices.numel() // 3, ptr(indices), nidx, ptr(out), nidx, C.byref(n),
+                 _stream_ptr(stream)))
+        return out[: n.value]
     v = np.ascontiguousarray(np.asarray(vertices))
     if v.dtype not in (np.float32, np.float64):
         v = v.astype(np.float64)

@@ -157,6 +157,22 @@ __device__ __forceinline__ void query_at(src_grid const& s, int64_t i, float& x,
     z = ((s.g.start[2] + s.g.dx[2] * fx) + s.g.dy[2] * fy) + s.g.dz[2] * fz;
 }
 
+// Boundary prepared for the FP32 kernel's chain path (agwn_frac.hip): when every weight is a small integer, the
+// segments are expanded to unit weight (endpoints swapped where the weight is negative) and ordered into chains, and
+// stored as a stream of points (x, y, z, flag) in the caller's frame -- flag 1: a unit segment runs from the previous
+// point to this one; flag 0: a chain starts here. Consecutive segments then share an endpoint. chain == false: the
+// weights are general and the per-segment kernel runs instead.
+struct frac_plan
+{
+    dbuf buf;           // npts float4 entries (device)
+    int64_t npts = 0;   // stream entries
+    int64_t nunit = 0;  // unit segments
+    bool chain = false;
+    float4 const* pts() const { return buf.as<float4 const>(); }
+};
+// Builds the plan from a host copy of the boundary (stream-ordered upload on `stream`; returns when it is done).
+int frac_plan_build(float const* seg7_host, int64_t nseg, frac_plan& plan, hipStream_t stream);
+
 // Fractional launchers (agwn_frac.hip). seg7/out/counts are device pointers. counts may be null (fractional
 // only); otherwise out[i] = fractional + counts[i] (the full GWN of gwn_mesh.hh:213-218).
 // Instantiated for src_array<float>, src_grid (FP32) and src_array<double> (FP64).
@@ -178,8 +194,15 @@ int mlp_route(agwn_mlp const* mlp, Src const& src, int64_t n, float delta, uint3
               int8_t* labels, int64_t* list, unsigned long long* counters, hipStream_t s);
 template <class Src>
 int gather_routed(Src const& src, int64_t const* list, int64_t m, float* pts, hipStream_t s);
+// LSD radix sort of n uint64 keys over their low `bits` bits (agwn_boundary.hip); *sorted = a or b
+int radix_sort_u64(uint64_t* a, uint64_t* b, int64_t n, int bits, hipStream_t s, uint64_t** sorted);
 int route_apply(int64_t const* list, int64_t m, float const* w, int8_t* labels, unsigned long long* counters,
                 hipStream_t s);
+
+// The FP32 launcher with a prepared plan (seg7 is still needed when the plan is not a chain plan).
+template <class Src>
+int frac_f32_planned(Src const& src, int64_t n, frac_plan const& plan, float const* seg7, int64_t nseg,
+                     double const x0[3], int32_t const* counts, float* out, hipStream_t stream);
 
 inline int frac_f32(float const* pos, int64_t n, float const* seg7, int64_t nseg, double const x0[3],
                     int32_t const* counts, float* out, hipStream_t stream)

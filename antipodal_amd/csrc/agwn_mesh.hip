@@ -723,6 +723,7 @@ struct agwn_grid
     std::unique_ptr<agwn::mesh_t<float>> mesh;
     float* seg7 = nullptr;
     int64_t nseg = 0;
+    std::unique_ptr<agwn::frac_plan> plan; // the boundary's chain plan, built once per scene
     ~agwn_grid()
     {
         if (seg7)
@@ -748,6 +749,8 @@ int grid_set_scene(agwn_grid* g, float const* verts, int64_t nv, int32_t const* 
     AGWN_TRY(ensure_device());
     auto m = std::make_unique<mesh_t<float>>();
     AGWN_TRY(m->create(verts, nv, idx, nidx));
+    auto plan = std::make_unique<frac_plan>();
+    AGWN_TRY(frac_plan_build(seg7, nseg, *plan, nullptr));
     float* d = nullptr;
     if (nseg > 0)
     {
@@ -765,6 +768,7 @@ int grid_set_scene(agwn_grid* g, float const* verts, int64_t nv, int32_t const* 
         (void)hipFree(g->seg7);
     g->seg7 = d;
     g->nseg = nseg;
+    g->plan = std::move(plan);
     return ok;
 }
 
@@ -841,7 +845,7 @@ int grid_eval_dev(agwn_grid* g, float const* start, float const* dx, float const
     dbuf cnt;
     AGWN_TRY(cnt.alloc(size_t(c.n) * sizeof(int32_t), s));
     AGWN_TRY(g->mesh->count_src(src_grid{c.gd}, c.n, c.qmax, c.x0f, cnt.as<int32_t>(), s));
-    return frac_f32_src(src_grid{c.gd}, c.n, g->seg7, g->nseg, c.x0d, cnt.as<int32_t>(), out, s);
+    return frac_f32_planned(src_grid{c.gd}, c.n, *g->plan, g->seg7, g->nseg, c.x0d, cnt.as<int32_t>(), out, s);
 }
 
 // Surrogate labelling with the exact FP32 path as fallback (see mlp_route): network labels for every query, then the
@@ -849,8 +853,8 @@ int grid_eval_dev(agwn_grid* g, float const* start, float const* dx, float const
 // coordinates as the plain evaluation) and relabelled.
 template <class Src>
 int label_surrogate(mesh_t<float>& mesh, agwn_mlp const* mlp, Src const& src, int64_t n, double qmax,
-                    float const x0f[3], float const* seg7, int64_t nseg, float delta, int64_t audit_every,
-                    uint64_t audit_seed, int8_t* labels, int64_t* stats, hipStream_t s)
+                    float const x0f[3], float const* seg7, int64_t nseg, frac_plan const* plan, float delta,
+                    int64_t audit_every, uint64_t audit_seed, int8_t* labels, int64_t* stats, hipStream_t s)
 {
     AGWN_CHECK(mlp != nullptr, "mlp handle is null");
     AGWN_CHECK(delta >= 0.0f, "delta must be >= 0 (+inf routes every point)"); // rejects NaN
@@ -871,15 +875,29 @@ int label_surrogate(mesh_t<float>& mesh, agwn_mlp const* mlp, Src const& src, in
         int64_t const m = int64_t(cnt[3]);
         if (m > 0)
         {
+            // the list was appended tile by tile in completion order: sort it back into index order, so that the
+            // exact kernels see spatially coherent runs of queries
+            dbuf tmp;
+            AGWN_TRY(tmp.alloc(size_t(m) * sizeof(uint64_t), s));
+            int bits = 1;
+            while (bits < 63 && (uint64_t(2 * n) >> bits) != 0)
+                ++bits;
+            uint64_t* sorted = nullptr;
+            AGWN_TRY(radix_sort_u64(reinterpret_cast<uint64_t*>(list.p), tmp.as<uint64_t>(), m, bits, s, &sorted));
+            int64_t const* L = reinterpret_cast<int64_t const*>(sorted);
             double const x0d[3] = {double(x0f[0]), double(x0f[1]), double(x0f[2])};
             dbuf pts, c, w;
             AGWN_TRY(pts.alloc(size_t(m) * 3 * sizeof(float), s));
             AGWN_TRY(c.alloc(size_t(m) * sizeof(int32_t), s));
             AGWN_TRY(w.alloc(size_t(m) * sizeof(float), s));
-            AGWN_TRY(gather_routed(src, list.as<int64_t>(), m, pts.as<float>(), s));
+            AGWN_TRY(gather_routed(src, L, m, pts.as<float>(), s));
             AGWN_TRY(mesh.count_src(src_array<float>{pts.as<float>()}, m, qmax, x0f, c.as<int32_t>(), s));
-            AGWN_TRY(frac_f32(pts.as<float>(), m, seg7, nseg, x0d, c.as<int32_t>(), w.as<float>(), s));
-            AGWN_TRY(route_apply(list.as<int64_t>(), m, w.as<float>(), labels, dc.as<unsigned long long>(), s));
+            if (plan)
+                AGWN_TRY(frac_f32_planned(src_array<float>{pts.as<float>()}, m, *plan, seg7, nseg, x0d,
+                                          c.as<int32_t>(), w.as<float>(), s));
+            else
+                AGWN_TRY(frac_f32(pts.as<float>(), m, seg7, nseg, x0d, c.as<int32_t>(), w.as<float>(), s));
+            AGWN_TRY(route_apply(L, m, w.as<float>(), labels, dc.as<unsigned long long>(), s));
             AGWN_HIP(hipMemcpyAsync(cnt, dc.p, sizeof(cnt), hipMemcpyDeviceToHost, s));
             AGWN_HIP(hipStreamSynchronize(s));
         }
@@ -899,8 +917,8 @@ int grid_label_dev(agwn_grid* g, agwn_mlp const* mlp, float const* start, float 
     AGWN_TRY(grid_setup(start, dx, dy, dz, nx, ny, nz, c));
     std::lock_guard<std::mutex> lk(g->mu);
     AGWN_CHECK(g->mesh != nullptr, "set_scene was not called");
-    return label_surrogate(*g->mesh, mlp, src_grid{c.gd}, c.n, c.qmax, c.x0f, g->seg7, g->nseg, delta, audit_every,
-                           audit_seed, labels, stats, s);
+    return label_surrogate(*g->mesh, mlp, src_grid{c.gd}, c.n, c.qmax, c.x0f, g->seg7, g->nseg, g->plan.get(), delta,
+                           audit_every, audit_seed, labels, stats, s);
 }
 
 int grid_eval_host(agwn_grid* g, float const* start, float const* dx, float const* dy, float const* dz, int nx, int ny,
@@ -975,8 +993,8 @@ int agwn_label_surrogate_dev(agwn_mesh const* m, agwn_mlp const* mlp, float cons
         double qmax = 0.0;
         if (n > 0)
             AGWN_TRY(mesh_t<float>::max_query_norm(pos, n, st, qmax));
-        return label_surrogate(mesh, mlp, src_array<float>{pos}, n, qmax, x0, seg7, nseg, delta, audit_every,
-                               audit_seed, labels, stats, st);
+        return label_surrogate(mesh, mlp, src_array<float>{pos}, n, qmax, x0, seg7, nseg, nullptr, delta,
+                               audit_every, audit_seed, labels, stats, st);
     });
 }
 } // extern "C"

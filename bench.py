@@ -94,6 +94,41 @@ def lattice_points(start, dx, dy, dz, nx, ny, nz, idx: np.ndarray) -> np.ndarray
     return (((s[None, :] + a[None, :] * ix[:, None]) + b[None, :] * iy[:, None]) + c[None, :] * iz[:, None]).astype(f)
 
 
+def lattice_torch(start, dx, dy, dz, n: int, dev):
+    """All n^3 lattice points on the device, in the kernels' order and float32 arithmetic (no contraction: torch's
+    elementwise kernels round each operation)."""
+    import torch
+
+    f = torch.float32
+    s, a, b, c = (torch.tensor(np.asarray(v, dtype=np.float32), device=dev) for v in (start, dx, dy, dz))
+    r = torch.arange(n, device=dev, dtype=f)
+    P = torch.empty((n, n, n, 3), dtype=f, device=dev)
+    P[...] = s
+    P += a * r[None, None, :, None]
+    P += b * r[None, :, None, None]
+    P += c * r[:, None, None, None]
+    return P.reshape(-1, 3)
+
+
+def audit_hash(idx, audit_every: int, seed: int):
+    """The routing kernel's audit choice with audit seed `seed`: hash32(i * 0x9E3779B97F4A7C15 + seed) % m == 0, on
+    int64 device tensors (wrapping multiplication; logical shifts by masking)."""
+
+    def s64(c: int) -> int:
+        return c - (1 << 64) if c >= 1 << 63 else c
+
+    def lsr33(x):
+        return (x >> 33) & ((1 << 31) - 1)
+
+    x = idx * s64(0x9E3779B97F4A7C15) + seed
+    x = x ^ lsr33(x)
+    x = x * s64(0xFF51AFD7ED558CCD)
+    x = x ^ lsr33(x)
+    x = x * s64(0xC4CEB9FE1A85EC53)
+    x = x ^ lsr33(x)
+    return (x & 0xFFFFFFFF) % audit_every == 0
+
+
 def copy_calibration() -> int:
     import torch
 
@@ -225,37 +260,45 @@ def main() -> int:
         f"speedup {plain_ms / ms_per_step:.2f}x, routed {st['fallback_fraction']:.4f}, mislabels {mislabels}")
 
     # ---- dominant kernel: the step's launches, re-run alone on the same inputs with HIP events on their stream ------
+    # The last step's routed set is rebuilt exactly: the lattice points (same float32 formula as the kernels), the
+    # mask flags and network outputs of the same kernels, the band, and the audit hash; its size must match the
+    # step's own count.
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    sample_idx = np.sort(np.random.default_rng(0).choice(nq, size=min(nq, 1 << 22), replace=False))
-    pts_s = torch.from_numpy(lattice_points(start, dx, dy, dz, nres, nres, nres, sample_idx)).to(dev)
-    pred_s = net.predict(pts_s)
+    P = lattice_torch(start, dx, dy, dz, nres, dev)
+    near = net.near_surface(P).bool()
+    pred = torch.empty(nq, dtype=torch.float32, device=dev)
+    net.predict(P, out=pred, stream=stream)
     ev0.record(stream)
-    net.predict(pts_s, out=pred_s, stream=stream)
+    net.predict(P, out=pred, stream=stream)
     ev1.record(stream)
     ev1.synchronize()
-    infer_ms = ev0.elapsed_time(ev1) * nq / len(sample_idx)
-    # the routed points of the last step: its band (same inference, same coordinates) -- audit points are a hashed
-    # 1/audit_every sample, so the band plus that many arbitrary non-band points reproduce the launch size
-    routed = st["routed"]
-    n_frac = max(1, routed)
-    band_idx = sample_idx[(pred_s - 0.5).abs().cpu().numpy() <= delta]
-    reps = int(np.ceil(n_frac / max(len(band_idx), 1)))
-    rp = np.tile(band_idx, reps)[:n_frac] if len(band_idx) else sample_idx[:n_frac]
-    pts_r = torch.from_numpy(lattice_points(start, dx, dy, dz, nres, nres, nres, rp)).to(dev)
+    infer_ms = ev0.elapsed_time(ev1)  # the route variant also writes labels and the list: a lower bound
+    band = ~near & ((pred - 0.5).abs() <= delta)
+    routed_mask = near | band
+    if args.audit_every:
+        ids = torch.arange(nq, device=dev, dtype=torch.int64)
+        routed_mask |= ~routed_mask & audit_hash(ids, args.audit_every, args.steps - 1)
+        del ids
+    rp_t = torch.nonzero(routed_mask).flatten()
+    if len(rp_t) != st["routed"]:
+        log(f"warning: rebuilt routed set has {len(rp_t)} points, the step routed {st['routed']}")
+    pts_r = P[rp_t].contiguous()
+    del P
     seg_t = torch.from_numpy(seg32).to(dev)
-    frac_r = torch.empty(len(rp), dtype=torch.float32, device=dev)
+    frac_r = torch.empty(len(rp_t), dtype=torch.float32, device=dev)
     agwn.eval_gwnr_mesh_batch_fractional(pts_r, x0, seg_t, out=frac_r, stream=stream)
     ev0.record(stream)
     agwn.eval_gwnr_mesh_batch_fractional(pts_r, x0, seg_t, out=frac_r, stream=stream)
     ev1.record(stream)
     ev1.synchronize()
     frac_ms = ev0.elapsed_time(ev1)
-    cnt_r = torch.empty(len(rp), dtype=torch.int32, device=dev)
+    cnt_r = torch.empty(len(rp_t), dtype=torch.int32, device=dev)
     ev0.record(stream)
     agwn.eval_gwnr_mesh_batch_integer(mesh, pts_r, x0, out=cnt_r, stream=stream)
     ev1.record(stream)
     ev1.synchronize()
     int_ms = ev0.elapsed_time(ev1)
+    rp = rp_t
     # sanity on the same points: full = fractional + integer
     full_r = torch.empty_like(frac_r)
     agwn.eval_gwnr_mesh_batch(mesh, pts_r, x0, seg_t, out=full_r, stream=stream)
@@ -266,10 +309,10 @@ def main() -> int:
         f"on {len(rp)} routed points")
     if frac_ms >= infer_ms:
         achieved = len(rp) * nseg * FLOP_PER_PAIR / (frac_ms * 1e-3) / 1e12
-        roof = {"bound": "valu", "kernel": "k_frac_f32", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
+        roof = {"bound": "valu", "kernel": "k_frac_f32_chain", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                 "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "kernel_ms": frac_ms,
                 "flop_per_launch": len(rp) * nseg * FLOP_PER_PAIR, "launch_points": len(rp)}
-        traffic_key = "k_frac_f32"
+        traffic_key = "k_frac_f32_chain"
     else:
         achieved = nq * MLP_FLOP_PER_POINT / (infer_ms * 1e-3) / 1e12
         roof = {"bound": "mfma", "kernel": "k_mlp_infer", "achieved": achieved, "peak": PEAK_BF16_TFLOPS,

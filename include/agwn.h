@@ -151,6 +151,12 @@ int agwn_boundary_f32(const float* verts, int64_t nv, const int32_t* idx, int64_
                       int64_t* count);
 int agwn_boundary_f64(const double* verts, int64_t nv, const int32_t* idx, int64_t nidx, double* out7, int64_t cap,
                       int64_t* count);
+/* The same on the GPU (device verts/idx/out7, host count; returns when done): half-edge keys radix-sorted, net
+ * direction per undirected edge, nonzero edges compacted -- element for element the host functions' output. */
+int agwn_boundary_f32_dev(const float* verts, int64_t nv, const int32_t* idx, int64_t nidx, float* out7, int64_t cap,
+                          int64_t* count, agwn_stream stream);
+int agwn_boundary_f64_dev(const double* verts, int64_t nv, const int32_t* idx, int64_t nidx, double* out7,
+                          int64_t cap, int64_t* count, agwn_stream stream);
 
 /* ---------------------------------------------------------------------------------------------------------------
  * Learned inside/outside surrogate (no counterpart in the reference: an accelerator in front of the exact path, which
