@@ -14,6 +14,7 @@ from ._lib import AgwnError, device_available, last_error
 from .api import (
     HipGridGwn,
     HipMesh,
+    HipMeshBatch,
     RobustMeshGwn,
     eval_gwnr_mesh_batch_robust,
     build_boundary_segments,
@@ -28,6 +29,7 @@ __all__ = [
     "GwnSurrogate",
     "HipGridGwn",
     "HipMesh",
+    "HipMeshBatch",
     "RobustMeshGwn",
     "eval_gwnr_mesh_batch_robust",
     "build_boundary_segments",

@@ -66,6 +66,12 @@ _SIG = {
     "agwn_boundary_f64": (_int, [_p, _i64, _p, _i64, _p, _i64, C.POINTER(_i64)]),
     "agwn_boundary_f32_dev": (_int, [_p, _i64, _p, _i64, _p, _i64, C.POINTER(_i64), _p]),
     "agwn_boundary_f64_dev": (_int, [_p, _i64, _p, _i64, _p, _i64, C.POINTER(_i64), _p]),
+    "agwn_batch_create": (_int, [_i64, _p, _p, _p, _p, _p, _p, C.POINTER(_p)]),
+    "agwn_batch_destroy": (_int, [_p]),
+    "agwn_batch_tune": (_int, [_p, _p, _p, _p, _p, _p]),
+    "agwn_batch_eval_dev": (_int, [_p, _p, _p, _p, _p, _p]),
+    "agwn_batch_info": (_int, [_p, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_int),
+                               C.POINTER(_int), _p]),
     "agwn_mlp_num_params": (_i64, []),
     "agwn_mlp_create": (_int, [_p, _p, C.c_float, C.c_uint64, C.POINTER(_p)]),
     "agwn_mlp_destroy": (_int, [_p]),

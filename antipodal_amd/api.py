@@ -344,3 +344,79 @@ class HipGridGwn:
 
     def __exit__(self, *a):
         self.close()
+
+
+class HipMeshBatch:
+    """Many meshes with uneven query counts, evaluated together (agwn_batch_*): eval(positions, q_off, x0) returns the
+    full GWN (eval_gwnr_mesh_batch, gwn_mesh.hh:201-219) of each mesh's own queries, in two persistent launches
+    captured in a HIP graph. meshes: a list of (vertices, indices, boundary) with float32 data and boundary weights
+    that are small integers (as build_boundary_segments returns)."""
+
+    def __init__(self, meshes):
+        vs, ids, segs = [], [], []
+        voff, ioff, soff = [0], [0], [0]
+        for v, i, b in meshes:
+            v = _host(v, np.float32, 3)
+            i = _host(i, np.int32).reshape(-1)
+            b = _host(b if b is not None else np.zeros((0, 7)), np.float32, 7)
+            vs.append(v)
+            ids.append(i)
+            segs.append(b)
+            voff.append(voff[-1] + len(v))
+            ioff.append(ioff[-1] + i.size)
+            soff.append(soff[-1] + len(b))
+        self.k = len(meshes)
+        cat = lambda parts, dt, cols: (np.ascontiguousarray(np.concatenate(parts).astype(dt))
+                                       if parts else np.zeros((0, cols), dt))
+        v = cat(vs, np.float32, 3)
+        i = cat(ids, np.int32, 1).reshape(-1)
+        b = cat(segs, np.float32, 7)
+        self._offs = [np.asarray(o, dtype=np.int64) for o in (voff, ioff, soff)]
+        h = C.c_void_p()
+        check(lib.agwn_batch_create(self.k, ptr(v) if v.size else None, ptr(self._offs[0]), ptr(i) if i.size else None,
+                                    ptr(self._offs[1]), ptr(b) if b.size else None, ptr(self._offs[2]), C.byref(h)))
+        self._h = h
+
+    def tune(self, positions, q_off, x0, out, stream=None) -> dict:
+        q = np.ascontiguousarray(np.asarray(q_off, dtype=np.int64))
+        check(lib.agwn_batch_tune(self._h, ptr(positions), ptr(q), ptr(_x0(x0, np.float32)), ptr(out),
+                                  _stream_ptr(stream)))
+        return self.info()
+
+    def eval(self, positions, q_off, x0, out=None, stream=None):
+        """positions: torch CUDA float32 (n, 3), each mesh's queries contiguous in mesh order; q_off: k + 1 offsets."""
+        import torch
+
+        q = np.ascontiguousarray(np.asarray(q_off, dtype=np.int64))
+        if len(q) != self.k + 1 or q[0] != 0 or positions.numel() != 3 * q[-1]:
+            raise ValueError("q_off must have k + 1 offsets from 0 to the number of positions")
+        if out is None:
+            out = torch.empty(int(q[-1]), dtype=torch.float32, device=positions.device)
+        check(lib.agwn_batch_eval_dev(self._h, ptr(positions), ptr(q), ptr(_x0(x0, np.float32)), ptr(out),
+                                      _stream_ptr(stream)))
+        return out
+
+    def info(self) -> dict:
+        k, t, u = C.c_int64(), C.c_int64(), C.c_int64()
+        fq, fb = C.c_int(), C.c_int()
+        ms = np.zeros(8, dtype=np.float64)
+        check(lib.agwn_batch_info(self._h, C.byref(k), C.byref(t), C.byref(u), C.byref(fq), C.byref(fb), ptr(ms)))
+        return {"meshes": k.value, "triangles": t.value, "unit_segments": u.value, "queries_per_lane": fq.value,
+                "block": fb.value, "tune_ms": [float(x) for x in ms if x > 0]}
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib.agwn_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

@@ -1,32 +1,38 @@
 // Fractional GWN term on gfx950: the boundary sum of gwn_mesh.hh:34-46 / math/common.hh:181-189, batched.
 //
 // Work decomposition. One lane owns Q query points; a 256-lane workgroup owns 256*Q consecutive queries and walks the
-// whole boundary. The boundary is pre-rotated once per call into the pole frame, packed 32 B per segment, and read
-// with wave-uniform addresses, so each segment arrives in SGPRs through one scalar load and feeds the VALU of all 64
-// lanes: no LDS traffic and no bank conflicts in the inner loop.
+// whole boundary in tiles staged through LDS (one coalesced load per lane, then broadcast reads of the same address
+// by every lane).
 //
 // Pole frame: rows e1 = x1/|x1| (x1 = -x0), e2, e3 orthonormal and right-handed; coordinates (X, y, z) of
 // v = endpoint - query. With s = |x1| -- NOT assumed to be 1: a normalised float x0 is off unit length by ~1e-8, and
 // the reference's formula is not invariant to |x1| (near the pole the difference reaches 1e-4 rad per pair), so the
-// reference's function is evaluated exactly as written:
-//   num = x1 . (v0 x v1)                             = s (y0 z1 - z0 y1)
-//   den = l0 l1 + l1 x1.v0 + l0 x1.v1 + v0.v1        = l0 l1 + s (l1 X0 + l0 X1) + X0 X1 + y0 y1 + z0 z1
+// reference's function is evaluated as written, in an exactly equivalent arrangement:
+//   num = x1 . (v0 x v1)                       = s (y0 z1 - z0 y1)
+//   den = l0 l1 + l1 x1.v0 + l0 x1.v1 + v0.v1  = (l0 + s X0)(l1 + s X1) + (1 - s^2) X0 X1 + y0 y1 + z0 z1
+//   l + s X = (X >= 0 ? l + X : (y^2 + z^2) / (l - X)) + (s - 1) X
 //   area += w * atan2(num, den)
-// FP32 kernel: the pair is evaluated in FP32 and accumulated in FP64 (one convert + one FP64 FMA per pair; cheaper
-// than FP32 two-sum compensation and strictly more accurate). A pair whose (num, den) is small against l0*l1 is
-// ill-conditioned -- the query is near the segment, or an endpoint is seen almost along the pole -- and is
-// re-evaluated in FP64 from the FP64 copy of the segment, with the cancellation-free
-//   l + s X = (X >= 0 ? l + X : (y^2 + z^2) / (l - X)) + (s - 1) X,
-//   den = (l0 + s X0)(l1 + s X1) + (1 - s^2) X0 X1 + y0 y1 + z0 z1,
-// then scaled into FP32 range and finished with the FP32 atan2 (the accurate num and den are what matters).
-// FP64 kernel: every pair in FP64 with that cancellation-free form and two-sum compensated accumulation.
+// The second form has no cancellation when an endpoint lies along the pole (X ~ -l), which the first suffers for
+// every segment seen near the pole direction: in FP32 those pairs would carry errors of ~eps * l0 l1 / |den| each,
+// and a boundary of 4e5 segments accumulates them past 1e-5.
+// FP32 kernels: the pair in FP32, accumulated in FP64 (one convert + one add per term; cheaper than FP32 two-sum
+// compensation and more accurate). Coordinates: the rotation is done in FP64, and every pole-frame coordinate of an
+// endpoint and of a query is kept as a float pair (hi, lo) with hi + lo = the FP64 value to ~48 bits; v is formed as
+// (a_hi - p_hi) + (a_lo - p_lo), so each component is accurate to ~2 eps of itself. Rounding the rotated coordinates
+// to one float instead would leave an absolute error of ~eps |a| in v, and a segment at distance l from the query
+// would carry a relative error of ~eps |a| / l: 1e-4 rad at l = 1e-3 on a unit-size mesh. A pair with |num| + |den|
+// small against the scale of its products (lp0 lp1 + (|y0| + |z0|)(|y1| + |z1|)) is ill-conditioned -- the query
+// is near the segment, or the segment passes near the pole direction -- and is re-evaluated in FP64, then scaled
+// into FP32 range and finished with the FP32 atan2 (accurate num and den are what matters). So is every pair with
+// den <= 0 (there atan2 sits on its branch cut, and the sign of num decides between +pi and -pi).
+// FP64 kernel: every pair in FP64 in the same form, with two-sum compensated accumulation.
 //
 // FP32 chain kernel (boundaries whose weights are small integers -- build_boundary_segments' are +-1): the boundary
 // is a stream of points in which consecutive unit segments share endpoints, so each query computes v = P - q and
-// l = |v| once per point rather than twice per segment; and the atan2 of two consecutive well-conditioned pairs is
+// l + s X once per point rather than twice per segment; and the atan2 of two consecutive well-conditioned pairs is
 // taken once, of the product of their complex numbers (den + i num): with den > 0 each angle lies in (-pi/2, pi/2),
-// so the product's argument is the exact sum of the two with no 2 pi wrap. Pairs with den <= 0 or ill-conditioned
-// ones take their own atan2 (FP32) or the FP64 path, as above, and enter the product as 1.
+// so the product's argument is the exact sum of the two with no 2 pi wrap. Pairs that go to FP64 enter the product
+// as 1.
 
 #include "agwn_internal.hpp"
 
@@ -86,7 +92,8 @@ __global__ void __launch_bounds__(kBlock) k_prep_segments(T const* __restrict__ 
     if (i >= nseg)
     {
         if (s32)
-            s32[i] = seg_f32{1.0e6f, 0.0f, 0.0f, 1.0e6f, 0.0f, 0.0f, 0.0f, 0.0f};
+            s32[i] = seg_f32{1.0e6f, 0.0f, 0.0f, 1.0e6f, 0.0f, 0.0f, 0.0f, 0.0f,
+                             0.0f,   0.0f, 0.0f, 0.0f,   0.0f, 0.0f, 0.0f, 0.0f};
         s64[i] = seg_f64{1.0e6, 0.0, 0.0, 1.0e6, 0.0, 0.0, 0.0, 0.0};
         return;
     }
@@ -101,7 +108,13 @@ __global__ void __launch_bounds__(kBlock) k_prep_segments(T const* __restrict__ 
     double const rby = R.r[3] * bx + R.r[4] * by + R.r[5] * bz;
     double const rbz = R.r[6] * bx + R.r[7] * by + R.r[8] * bz;
     if (s32)
-        s32[i] = seg_f32{float(rax), float(ray), float(raz), float(rbx), float(rby), float(rbz), float(w), 0.0f};
+    {
+        float const hax = float(rax), hay = float(ray), haz = float(raz);
+        float const hbx = float(rbx), hby = float(rby), hbz = float(rbz);
+        s32[i] = seg_f32{hax, hay, haz, hbx, hby, hbz, float(w), 0.0f,
+                         float(rax - double(hax)), float(ray - double(hay)), float(raz - double(haz)),
+                         float(rbx - double(hbx)), float(rby - double(hby)), float(rbz - double(hbz)), 0.0f, 0.0f};
+    }
     s64[i] = seg_f64{rax, ray, raz, rbx, rby, rbz, w, 0.0};
 }
 
@@ -156,6 +169,15 @@ __device__ __forceinline__ float atan2_f32(float y, float x)
     return __builtin_copysignf(a, y);
 }
 
+// l + s X of one endpoint (v = endpoint - query in the pole frame) without cancellation:
+// (X >= 0 ? l + X : (y^2 + z^2) / (l - X)) + (s - 1) X, with l = |v|.
+__device__ __forceinline__ float lplus_f32(float x, float yz, float l2, float sm1)
+{
+    float const l = __builtin_amdgcn_sqrtf(l2);
+    float const t = x >= 0.0f ? l + x : yz * __builtin_amdgcn_rcpf(l - x);
+    return t + sm1 * x;
+}
+
 // The FP32 kernels' ill-conditioned pair: (num, den) formed in FP64 without cancellation (as pair_f64), scaled by a
 // power of two into FP32 range and finished with the FP32 atan2. The conditioning problem is in forming num and den,
 // not in the arctangent of accurate values: rounding them to FP32 moves the angle by at most ~6e-8 rad, and the
@@ -196,8 +218,23 @@ __device__ __forceinline__ void query_f64(Src const& src, int64_t i, int64_t n, 
     z = R.r[6] * px + R.r[7] * py + R.r[8] * pz;
 }
 
+// A query's pole-frame coordinates as float pairs: h = float(FP64 value), l = float(remainder).
+template <class Src>
+__device__ __forceinline__ void query_split(Src const& src, int64_t i, int64_t n, frame const& R, float h[3],
+                                            float l[3])
+{
+    double c[3];
+    query_f64(src, i, n, R, c[0], c[1], c[2]);
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+    {
+        h[k] = float(c[k]);
+        l[k] = float(c[k] - double(h[k]));
+    }
+}
+
 constexpr int kChunk = 4;          // the boundary is padded to a multiple of this (zero-weight segments)
-constexpr int kTile = kBlock;      // segments staged in LDS per pass: one 32-byte segment per lane
+constexpr int kTile = kBlock;      // segments staged in LDS per pass: one 64-byte segment per lane
 
 template <int Q, class Src>
 __global__ void __launch_bounds__(kBlock)
@@ -207,19 +244,15 @@ __global__ void __launch_bounds__(kBlock)
 {
     __shared__ seg_f32 tile[kTile];
     int64_t const base = int64_t(blockIdx.x) * (kBlock * Q) + threadIdx.x;
-    float px[Q], py[Q], pz[Q];
+    float ph[Q][3], pl[Q][3];
     double acc[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q)
     {
-        double x, y, z;
-        query_f64(src, base + int64_t(q) * kBlock, n, R, x, y, z);
-        px[q] = float(x);
-        py[q] = float(y);
-        pz[q] = float(z);
+        query_split(src, base + int64_t(q) * kBlock, n, R, ph[q], pl[q]);
         acc[q] = 0.0;
     }
-    float const sf = float(P.s);
+    float const sf = float(P.s), sm1 = float(P.sm1), cf = float(P.c);
 
     for (int64_t t0 = 0; t0 < nseg_padded; t0 += kTile)
     {
@@ -236,14 +269,22 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
             for (int q = 0; q < Q; ++q)
             {
-                float const v0x = g.ax - px[q], v0y = g.ay - py[q], v0z = g.az - pz[q];
-                float const v1x = g.bx - px[q], v1y = g.by - py[q], v1z = g.bz - pz[q];
-                float const l0 = __builtin_amdgcn_sqrtf(v0x * v0x + v0y * v0y + v0z * v0z);
-                float const l1 = __builtin_amdgcn_sqrtf(v1x * v1x + v1y * v1y + v1z * v1z);
+                float const v0x = (g.ax - ph[q][0]) + (g.lax - pl[q][0]);
+                float const v0y = (g.ay - ph[q][1]) + (g.lay - pl[q][1]);
+                float const v0z = (g.az - ph[q][2]) + (g.laz - pl[q][2]);
+                float const v1x = (g.bx - ph[q][0]) + (g.lbx - pl[q][0]);
+                float const v1y = (g.by - ph[q][1]) + (g.lby - pl[q][1]);
+                float const v1z = (g.bz - ph[q][2]) + (g.lbz - pl[q][2]);
+                float const yz0 = v0y * v0y + v0z * v0z, yz1 = v1y * v1y + v1z * v1z;
+                float const lp0 = lplus_f32(v0x, yz0, v0x * v0x + yz0, sm1);
+                float const lp1 = lplus_f32(v1x, yz1, v1x * v1x + yz1, sm1);
                 float const num = sf * (v0y * v1z - v0z * v1y);
-                float const den = l0 * l1 + sf * (l1 * v0x + l0 * v1x) + (v0x * v1x + v0y * v1y + v0z * v1z);
+                float const pp = lp0 * lp1;
+                float const den = pp + (cf * (v0x * v1x) + (v0y * v1y + v0z * v1z));
+                float const scale = pp + (__builtin_fabsf(v0y) + __builtin_fabsf(v0z)) *
+                                             (__builtin_fabsf(v1y) + __builtin_fabsf(v1z));
                 double a;
-                if (__builtin_expect(__builtin_fabsf(num) + __builtin_fabsf(den) < tau * (l0 * l1), 0))
+                if (__builtin_expect(__builtin_fabsf(num) + __builtin_fabsf(den) < tau * scale || !(den > 0.0f), 0))
                 {
                     // ill-conditioned pair: FP64 from the FP64 copies of the segment and the query
                     seg_f64 const h = seg64[t0 + j];
@@ -274,9 +315,9 @@ __global__ void __launch_bounds__(kBlock)
     }
 }
 
-// Rotates the point stream into the pole frame: FP32 (flag kept in .w) and FP64 copies.
+// Rotates the point stream into the pole frame, two float4 per point: (hi x, y, z, flag) and (lo x, y, z, 0).
 __global__ void __launch_bounds__(kBlock) k_prep_stream(float4 const* __restrict__ in, int64_t n, frame R,
-                                                       float4* __restrict__ out32, double4* __restrict__ out64)
+                                                       float4* __restrict__ out)
 {
     int64_t const i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
     if (i >= n)
@@ -286,75 +327,93 @@ __global__ void __launch_bounds__(kBlock) k_prep_stream(float4 const* __restrict
     double const rx = R.r[0] * x + R.r[1] * y + R.r[2] * z;
     double const ry = R.r[3] * x + R.r[4] * y + R.r[5] * z;
     double const rz = R.r[6] * x + R.r[7] * y + R.r[8] * z;
-    out32[i] = make_float4(float(rx), float(ry), float(rz), g.w);
-    out64[i] = make_double4(rx, ry, rz, 0.0);
+    float const hx = float(rx), hy = float(ry), hz = float(rz);
+    out[2 * i] = make_float4(hx, hy, hz, g.w);
+    out[2 * i + 1] = make_float4(float(rx - double(hx)), float(ry - double(hy)), float(rz - double(hz)), 0.0f);
 }
 
-template <int Q, class Src>
-__global__ void __launch_bounds__(kBlock)
-    k_frac_f32_chain(Src const src, int64_t n, float4 const* __restrict__ pts, double4 const* __restrict__ pts64,
-                     int64_t npts, frame R, pole P, float tau, int32_t const* __restrict__ counts,
-                     float* __restrict__ out)
+// The chain kernel's work for one block of B lanes x Q queries: queries base + q*B (lane's share), valid below n;
+// the point stream pts[0, 2 npts) (pole frame, hi/lo float4 per point) staged through `tile` (2B float4 of LDS).
+// Per stream point a lane keeps v = P - q (X, y, z), lp = l + s X and |y| + |z|; per unit segment
+//   num = s (y0 z1 - z0 y1),  den = lp0 lp1 + (1 - s^2) X0 X1 + (y0 y1 + z0 z1)
+// -- the reference's den, rearranged exactly so that no term cancels when an endpoint lies along the pole. A pair is
+// ill-conditioned when |num| + |den| < tau (lp0 lp1 + (|y0| + |z0|)(|y1| + |z1|)) (the scale of every product in
+// num and den: the query is near the segment, or the segment passes near the pole direction) and goes to FP64 (the
+// coordinates hi + lo, exact in double); so do pairs with den <= 0, where atan2 sits on its branch cut and only an
+// accurate num gives the sign.
+template <int Q, int B, class Src>
+__device__ __forceinline__ void chain_block(Src const& src, int64_t base, int64_t n, float4 const* __restrict__ pts,
+                                            int64_t npts, frame const& R, pole const& P, float tau,
+                                            int32_t const* __restrict__ counts, float* __restrict__ out, float4* tile)
 {
-    __shared__ float4 tile[kTile];
-    int64_t const base = int64_t(blockIdx.x) * (kBlock * Q) + threadIdx.x;
-    float px[Q], py[Q], pz[Q];
-    float vx[Q], vy[Q], vz[Q], vl[Q]; // v and |v| of the previous stream point
-    float pn[Q], pd[Q];               // the pending pair's (num, den), when `pending`
+    float ph[Q][3], pl[Q][3];                // the query, hi and lo
+    float vx[Q], vy[Q], vz[Q], vp[Q], va[Q]; // the previous stream point: v, l + s X, |y| + |z|
+    float pn[Q], pd[Q];                      // the pending pair's (num, den), when `pending`
     double acc[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q)
     {
-        double x, y, z;
-        query_f64(src, base + int64_t(q) * kBlock, n, R, x, y, z);
-        px[q] = float(x);
-        py[q] = float(y);
-        pz[q] = float(z);
+        query_split(src, base + int64_t(q) * B, n, R, ph[q], pl[q]);
         acc[q] = 0.0;
-        vx[q] = vy[q] = vz[q] = vl[q] = 0.0f;
+        vx[q] = vy[q] = vz[q] = vp[q] = va[q] = 0.0f;
         pn[q] = 0.0f;
         pd[q] = 1.0f;
     }
-    float const sf = float(P.s);
+    float const sf = float(P.s), sm1 = float(P.sm1), cf = float(P.c);
     bool pending = false; // uniform: the segment structure is the same for every lane
+    int64_t const nf = 2 * npts;
 
-    for (int64_t t0 = 0; t0 < npts; t0 += kTile)
+    for (int64_t t0 = 0; t0 < npts; t0 += B)
     {
         __syncthreads(); // the previous tile has been consumed by every wave
-        if (t0 + threadIdx.x < npts)
-            tile[threadIdx.x] = pts[t0 + threadIdx.x];
+        int64_t const f0 = 2 * t0;
+        if (f0 + threadIdx.x < nf)
+            tile[threadIdx.x] = pts[f0 + threadIdx.x];
+        if (f0 + B + threadIdx.x < nf)
+            tile[B + threadIdx.x] = pts[f0 + B + threadIdx.x];
         __syncthreads();
-        int const cnt = int(npts - t0 < kTile ? npts - t0 : kTile);
+        int const cnt = int(npts - t0 < B ? npts - t0 : B);
         for (int j = 0; j < cnt; ++j)
         {
-            float4 const g = tile[j]; // same address in every lane: an LDS broadcast
+            float4 const g = tile[2 * j], gl = tile[2 * j + 1]; // same address in every lane: LDS broadcasts
             bool const seg = __builtin_amdgcn_readfirstlane(__float_as_int(g.w)) != 0;
 #pragma unroll
             for (int q = 0; q < Q; ++q)
             {
-                float const nx = g.x - px[q], ny = g.y - py[q], nz = g.z - pz[q];
-                float const nl = __builtin_amdgcn_sqrtf(nx * nx + ny * ny + nz * nz);
+                float const nx = (g.x - ph[q][0]) + (gl.x - pl[q][0]);
+                float const ny = (g.y - ph[q][1]) + (gl.y - pl[q][1]);
+                float const nz = (g.z - ph[q][2]) + (gl.z - pl[q][2]);
+                float const yz = ny * ny + nz * nz;
+                float const np = lplus_f32(nx, yz, nx * nx + yz, sm1);
+                float const na = __builtin_fabsf(ny) + __builtin_fabsf(nz);
                 if (seg)
                 {
                     float num = sf * (vy[q] * nz - vz[q] * ny);
-                    float den = vl[q] * nl + sf * (nl * vx[q] + vl[q] * nx) + (vx[q] * nx + vy[q] * ny + vz[q] * nz);
-                    bool const ill = __builtin_fabsf(num) + __builtin_fabsf(den) < tau * (vl[q] * nl);
+                    float const pp = vp[q] * np;
+                    float den = pp + (cf * (vx[q] * nx) + (vy[q] * ny + vz[q] * nz));
+                    bool const ill = __builtin_fabsf(num) + __builtin_fabsf(den) < tau * (pp + va[q] * na);
                     if (__builtin_expect(ill || !(den > 0.0f), 0))
                     {
-                        double a;
-                        if (ill)
+                        // the segment's start point: in this tile, or (j == 0) the last point of the previous one --
+                        // a flagged point is never the first of a stream, so then t0 > 0
+                        float4 h0, l0;
+                        if (j > 0)
                         {
-                            double4 const h0 = pts64[t0 + j - 1], h1 = pts64[t0 + j];
-                            double dx, dy, dz;
-                            query_f64(src, base + int64_t(q) * kBlock, n, R, dx, dy, dz);
-                            a = double(pair_f64_atan_f32(h0.x - dx, h0.y - dy, h0.z - dz, h1.x - dx, h1.y - dy,
-                                                         h1.z - dz, P));
+                            h0 = tile[2 * j - 2];
+                            l0 = tile[2 * j - 1];
                         }
                         else
                         {
-                            a = double(atan2_f32(num, den));
+                            h0 = pts[f0 - 2];
+                            l0 = pts[f0 - 1];
                         }
-                        acc[q] += a;
+                        double const qx = double(ph[q][0]) + double(pl[q][0]);
+                        double const qy = double(ph[q][1]) + double(pl[q][1]);
+                        double const qz = double(ph[q][2]) + double(pl[q][2]);
+                        acc[q] += double(pair_f64_atan_f32(
+                            (double(h0.x) + double(l0.x)) - qx, (double(h0.y) + double(l0.y)) - qy,
+                            (double(h0.z) + double(l0.z)) - qz, (double(g.x) + double(gl.x)) - qx,
+                            (double(g.y) + double(gl.y)) - qy, (double(g.z) + double(gl.z)) - qz, P));
                         num = 0.0f;
                         den = 1.0f;
                     }
@@ -373,7 +432,8 @@ __global__ void __launch_bounds__(kBlock)
                 vx[q] = nx;
                 vy[q] = ny;
                 vz[q] = nz;
-                vl[q] = nl;
+                vp[q] = np;
+                va[q] = na;
             }
             if (seg)
                 pending = !pending;
@@ -385,7 +445,7 @@ __global__ void __launch_bounds__(kBlock)
     {
         if (pending)
             acc[q] += double(atan2_f32(pn[q], pd[q]));
-        int64_t const i = base + int64_t(q) * kBlock;
+        int64_t const i = base + int64_t(q) * B;
         if (i < n)
         {
             double r = acc[q] / kTwoPi;
@@ -393,6 +453,43 @@ __global__ void __launch_bounds__(kBlock)
                 r += double(counts[i]);
             out[i] = float(r);
         }
+    }
+}
+
+template <int Q, class Src>
+__global__ void __launch_bounds__(kBlock)
+    k_frac_f32_chain(Src const src, int64_t n, float4 const* __restrict__ pts, int64_t npts, frame R, pole P,
+                     float tau, int32_t const* __restrict__ counts, float* __restrict__ out)
+{
+    __shared__ float4 tile[2 * kBlock];
+    int64_t const base = int64_t(blockIdx.x) * (kBlock * Q) + threadIdx.x;
+    chain_block<Q, kBlock>(src, base, n, pts, npts, R, P, tau, counts, out, tile);
+}
+
+// Persistent chain kernel over a batch of meshes: workgroups take work items (mesh, first query of a B x Q tile)
+// from an atomic counter until none are left; each mesh has its own point stream and query range, and counts (the
+// integer term) are added per query. Every workgroup ends at the same test.
+template <int Q, int B>
+__global__ void __launch_bounds__(B)
+    k_frac_f32_chain_batch(float const* __restrict__ pos, batch_frac_desc const* __restrict__ meshes,
+                           int32_t const* __restrict__ item_mesh, int64_t const* __restrict__ item_q0, int64_t nitems,
+                           unsigned long long* __restrict__ counter, float4 const* __restrict__ pts, frame R, pole P,
+                           float tau, int32_t const* __restrict__ counts, float* __restrict__ out)
+{
+    __shared__ float4 tile[2 * B];
+    __shared__ unsigned long long cur;
+    while (true)
+    {
+        if (threadIdx.x == 0)
+            cur = atomicAdd(counter, 1ull);
+        __syncthreads();
+        int64_t const it = int64_t(cur);
+        __syncthreads();
+        if (it >= nitems)
+            break;
+        batch_frac_desc const D = meshes[item_mesh[it]];
+        chain_block<Q, B>(src_array<float>{pos}, item_q0[it] + threadIdx.x, D.q_end, pts + 2 * D.pt0, D.npts, R, P,
+                          tau, counts, out, tile);
     }
 }
 
@@ -576,17 +673,16 @@ int frac_f32_planned(Src const& pos, int64_t n, frac_plan const& plan, float con
     AGWN_CHECK(blocks < (int64_t(1) << 31), "frac_f32: too many queries (%lld)", (long long)n);
     if (plan.chain)
     {
-        dbuf p32, p64;
-        AGWN_TRY(p32.alloc(size_t(std::max<int64_t>(plan.npts, 1)) * sizeof(float4), stream));
-        AGWN_TRY(p64.alloc(size_t(std::max<int64_t>(plan.npts, 1)) * sizeof(double4), stream));
+        dbuf p32;
+        AGWN_TRY(p32.alloc(size_t(std::max<int64_t>(plan.npts, 1)) * 2 * sizeof(float4), stream));
         if (plan.npts > 0)
         {
             k_prep_stream<<<dim3(unsigned((plan.npts + kBlock - 1) / kBlock)), dim3(kBlock), 0, stream>>>(
-                plan.pts(), plan.npts, R, p32.as<float4>(), p64.as<double4>());
+                plan.pts(), plan.npts, R, p32.as<float4>());
             AGWN_HIP(hipGetLastError());
         }
         k_frac_f32_chain<kQ, Src><<<dim3(unsigned(blocks)), dim3(kBlock), 0, stream>>>(
-            pos, n, p32.as<float4>(), p64.as<double4>(), plan.npts, R, P, kTau, counts, out);
+            pos, n, p32.as<float4>(), plan.npts, R, P, kTau, counts, out);
         AGWN_HIP(hipGetLastError());
         return ok;
     }
@@ -645,6 +741,46 @@ int frac_f64_src(Src const& pos, int64_t n, double const* seg7, int64_t nseg, do
     pole const P = make_pole(x0[0], x0[1], x0[2]);
     k_frac_f64<Src><<<dim3(unsigned(blocks)), dim3(kBlock), 0, stream>>>(pos, n, s64.as<seg_f64>(), nseg, R, P,
                                                                          counts, out);
+    AGWN_HIP(hipGetLastError());
+    return ok;
+}
+
+// (queries per lane, block) shapes of the batched kernel offered to the autotuner
+constexpr int kBatchShapes[][2] = {{1, 256}, {2, 256}, {4, 256}, {2, 128}, {4, 128}};
+
+int frac_batch_variants() { return int(sizeof(kBatchShapes) / sizeof(kBatchShapes[0])); }
+
+void frac_batch_shape(int v, int& q, int& block)
+{
+    q = kBatchShapes[v][0];
+    block = kBatchShapes[v][1];
+}
+
+int frac_stream_rotate(float4 const* in, int64_t n, double const x0[3], float4* out, hipStream_t s)
+{
+    if (n == 0)
+        return ok;
+    frame const R = make_frame_impl(x0[0], x0[1], x0[2]);
+    k_prep_stream<<<dim3(unsigned((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s>>>(in, n, R, out);
+    AGWN_HIP(hipGetLastError());
+    return ok;
+}
+
+int frac_batch_launch(int v, int blocks, float const* pos, batch_frac_desc const* meshes, int32_t const* item_mesh,
+                      int64_t const* item_q0, int64_t nitems, unsigned long long* counter, float4 const* pts,
+                      double const x0[3], int32_t const* counts, float* out, hipStream_t s)
+{
+    AGWN_CHECK(v >= 0 && v < frac_batch_variants(), "unknown batch kernel shape %d", v);
+    frame const R = make_frame_impl(x0[0], x0[1], x0[2]);
+    pole const P = make_pole(x0[0], x0[1], x0[2]);
+    dim3 const g = dim3(static_cast<unsigned>(blocks));
+#define AGWN_BATCH_CASE(QQ, BB)                                                                                        \
+    if (kBatchShapes[v][0] == QQ && kBatchShapes[v][1] == BB)                                                          \
+        k_frac_f32_chain_batch<QQ, BB><<<g, dim3(BB), 0, s>>>(pos, meshes, item_mesh, item_q0, nitems, counter,        \
+                                                              pts, R, P, kTau, counts, out);
+    AGWN_BATCH_CASE(1, 256)
+    else AGWN_BATCH_CASE(2, 256) else AGWN_BATCH_CASE(4, 256) else AGWN_BATCH_CASE(2, 128) else AGWN_BATCH_CASE(4, 128)
+#undef AGWN_BATCH_CASE
     AGWN_HIP(hipGetLastError());
     return ok;
 }

@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <string>
+#include <vector>
 
 struct agwn_mlp; // surrogate network handle (agwn_mlp.hip)
 
@@ -104,10 +105,12 @@ struct frame
 };
 frame make_pole_frame(double x0x, double x0y, double x0z);
 
-// Packed, rotated boundary segment: endpoints a, b in the pole frame, weight w. 32 B (one scalar dwordx8 load).
-struct alignas(32) seg_f32
+// Packed, rotated boundary segment: endpoints a, b in the pole frame as float pairs (hi = the FP64 coordinate rounded
+// to float, lo = the rounded remainder; hi + lo carries ~48 bits), weight w. 64 B.
+struct alignas(64) seg_f32
 {
-    float ax, ay, az, bx, by, bz, w, pad;
+    float ax, ay, az, bx, by, bz, w, pad0;
+    float lax, lay, laz, lbx, lby, lbz, pad1, pad2;
 };
 // The same segment in double, for the FP64 path and the FP32 kernel's ill-conditioned-pair fallback. 64 B.
 struct alignas(64) seg_f64
@@ -198,6 +201,21 @@ int gather_routed(Src const& src, int64_t const* list, int64_t m, float* pts, hi
 int radix_sort_u64(uint64_t* a, uint64_t* b, int64_t n, int bits, hipStream_t s, uint64_t** sorted);
 int route_apply(int64_t const* list, int64_t m, float const* w, int8_t* labels, unsigned long long* counters,
                 hipStream_t s);
+
+// Batched evaluation (agwn_frac.hip): the persistent chain kernel over many meshes' point streams.
+struct batch_frac_desc
+{
+    int64_t pt0, npts; // the mesh's point stream within the concatenated stream
+    int64_t q_end;     // one past the mesh's last query in the concatenated query array
+};
+bool chain_stream(float const* seg7_host, int64_t nseg, std::vector<float4>& stream_pts, int64_t& nunit);
+int frac_batch_variants();                         // number of (queries per lane, block) shapes
+void frac_batch_shape(int variant, int& q, int& block);
+// rotates the concatenated stream into the pole frame of x0: two float4 per point, (hi, flag) and (lo, 0)
+int frac_stream_rotate(float4 const* in, int64_t n, double const x0[3], float4* out, hipStream_t s);
+int frac_batch_launch(int variant, int blocks, float const* pos, batch_frac_desc const* meshes,
+                      int32_t const* item_mesh, int64_t const* item_q0, int64_t nitems, unsigned long long* counter,
+                      float4 const* pts, double const x0[3], int32_t const* counts, float* out, hipStream_t s);
 
 // The FP32 launcher with a prepared plan (seg7 is still needed when the plan is not a chain plan).
 template <class Src>

@@ -226,18 +226,14 @@ __device__ __forceinline__ float inv_dir(double d)
     return f != 0.0f ? 1.0f / f : (signbit(f) ? -1e30f : 1e30f);
 }
 
-template <class T, class Src>
-__global__ void __launch_bounds__(kTB)
-    k_count(Src const src, int64_t n, T dx, T dy, T dz, bvh_node<float> const* __restrict__ nodes,
-            leaf_rec<T> const* __restrict__ rec, int32_t const* __restrict__ brute, int32_t const* __restrict__ nbrute_p,
-            int use_bvh, int32_t* __restrict__ out)
+// Signed crossing count of one ray (p, d) against a prepared mesh: LBVH short-stack traversal (stack column `lane` of
+// an LDS array [kMaxStack][kTB]) with the Moeller-Trumbore replica at the leaves, plus the exhaustively tested
+// triangles.
+template <class T>
+__device__ __forceinline__ int count_one(T px, T py, T pz, T dx, T dy, T dz, bvh_node<float> const* __restrict__ nodes,
+                                         leaf_rec<T> const* __restrict__ rec, int32_t const* __restrict__ brute,
+                                         int32_t nb, int use_bvh, int32_t (*stack)[kTB], int lane)
 {
-    __shared__ int32_t stack[kMaxStack][kTB];
-    int64_t const i = int64_t(blockIdx.x) * kTB + threadIdx.x;
-    if (i >= n)
-        return;
-    T px, py, pz;
-    query_at(src, i, px, py, pz);
     int sum = 0;
     if (use_bvh)
     {
@@ -277,7 +273,7 @@ __global__ void __launch_bounds__(kTB)
                 else if (have)
                 {
                     if (sp < kMaxStack) // depth <= kMaxStack - 2 is checked at build
-                        stack[sp++][threadIdx.x] = nd.c1;
+                        stack[sp++][lane] = nd.c1;
                 }
                 else
                 {
@@ -293,14 +289,67 @@ __global__ void __launch_bounds__(kTB)
             {
                 if (sp == 0)
                     break;
-                node = stack[--sp][threadIdx.x];
+                node = stack[--sp][lane];
             }
         }
     }
-    int32_t const nb = *nbrute_p;
     for (int32_t b = 0; b < nb; ++b)
         sum += mt_test(rec[brute[b]], px, py, pz, dx, dy, dz);
-    out[i] = sum;
+    return sum;
+}
+
+template <class T, class Src>
+__global__ void __launch_bounds__(kTB)
+    k_count(Src const src, int64_t n, T dx, T dy, T dz, bvh_node<float> const* __restrict__ nodes,
+            leaf_rec<T> const* __restrict__ rec, int32_t const* __restrict__ brute, int32_t const* __restrict__ nbrute_p,
+            int use_bvh, int32_t* __restrict__ out)
+{
+    __shared__ int32_t stack[kMaxStack][kTB];
+    int64_t const i = int64_t(blockIdx.x) * kTB + threadIdx.x;
+    if (i >= n)
+        return;
+    T px, py, pz;
+    query_at(src, i, px, py, pz);
+    out[i] = count_one(px, py, pz, dx, dy, dz, nodes, rec, brute, *nbrute_p, use_bvh, stack, int(threadIdx.x));
+}
+
+// One prepared mesh of a batch, for the persistent counting kernel.
+struct batch_mesh_desc
+{
+    bvh_node<float> const* nodes;
+    leaf_rec<float> const* rec;
+    int32_t const* brute;
+    int32_t const* nbrute;
+    int32_t use_bvh;
+    int32_t pad;
+    int64_t q_end; // one past the mesh's last query in the concatenated query array
+};
+
+// Persistent counting kernel over a batch: workgroups take work items (mesh, first query; kTB queries at most, not
+// past the mesh's end) from an atomic counter until none are left -- every workgroup ends at the same test.
+__global__ void __launch_bounds__(kTB)
+    k_count_batch(float const* __restrict__ pos, batch_mesh_desc const* __restrict__ meshes,
+                  int32_t const* __restrict__ item_mesh, int64_t const* __restrict__ item_q0, int64_t nitems,
+                  unsigned long long* __restrict__ counter, float dx, float dy, float dz, int32_t* __restrict__ out)
+{
+    __shared__ int32_t stack[kMaxStack][kTB];
+    __shared__ unsigned long long cur;
+    while (true)
+    {
+        if (threadIdx.x == 0)
+            cur = atomicAdd(counter, 1ull);
+        __syncthreads();
+        int64_t const it = int64_t(cur);
+        __syncthreads();
+        if (it >= nitems)
+            break;
+        int32_t const m = item_mesh[it];
+        batch_mesh_desc const D = meshes[m];
+        int64_t const q = item_q0[it] + threadIdx.x;
+        if (q < D.q_end)
+            out[q] = count_one(pos[3 * q], pos[3 * q + 1], pos[3 * q + 2], dx, dy, dz, D.nodes, D.rec, D.brute,
+                               D.nbrute ? *D.nbrute : 0, D.use_bvh, stack, int(threadIdx.x));
+    }
 }
 
 // Per-direction prepared state, cached on the mesh (rebuilt when the direction changes or the queries reach farther).
@@ -995,6 +1044,450 @@ int agwn_label_surrogate_dev(agwn_mesh const* m, agwn_mlp const* mlp, float cons
             AGWN_TRY(mesh_t<float>::max_query_norm(pos, n, st, qmax));
         return label_surrogate(mesh, mlp, src_array<float>{pos}, n, qmax, x0, seg7, nseg, nullptr, delta,
                                audit_every, audit_seed, labels, stats, st);
+    });
+}
+} // extern "C"
+
+// =====================================================================================================================
+// Batched evaluation of many meshes with uneven query counts: the full GWN of every query of every mesh, in two
+// persistent launches (the integer term, then the fractional term plus the counts) captured once in a HIP graph and
+// replayed while the inputs stay the same. Each mesh keeps its own LBVH and chain-ordered boundary stream; work items
+// (mesh, query tile) are ordered longest mesh first and taken from atomic counters. The fractional kernel's tile shape
+// (queries per lane x block) is chosen by an autotuner that times every shape on a sample of the meshes. Per query the
+// arithmetic is exactly the single-mesh path's (k_count, k_frac_f32_chain with x0 shared by the batch).
+// =====================================================================================================================
+struct agwn_batch
+{
+    std::mutex mu;
+    int64_t k = 0;
+    std::vector<std::unique_ptr<agwn::mesh_t<float>>> meshes;
+    std::vector<int64_t> pt_off; // k + 1 offsets into the point stream
+    std::vector<int64_t> nunit;  // unit boundary segments per mesh
+    agwn::dbuf stream;           // the concatenated point streams, caller frame (persistent)
+    agwn::dbuf zero;             // an int32 zero: the exhaustive-test count of meshes without a BVH state
+    hipStream_t bs = nullptr;    // the batch's own stream (graph capture and replay)
+    int variant = -1;            // tuned fractional kernel shape
+    double tune_ms[8] = {};
+    int ncu = 0;
+    // the cached launch: inputs it was built for, its buffers and graph
+    float const* g_pos = nullptr;
+    float* g_out = nullptr;
+    float g_x0[3] = {0, 0, 0};
+    std::vector<int64_t> g_qoff;
+    int g_variant = -1;
+    std::vector<std::shared_ptr<agwn::dir_state<float>>> states;
+    agwn::dbuf cdesc, fdesc, citem_m, citem_q, fitem_m, fitem_q, counters, counts, p32;
+    int64_t ncitems = 0, nfitems = 0;
+    hipGraphExec_t exec = nullptr;
+    ~agwn_batch()
+    {
+        if (exec)
+            (void)hipGraphExecDestroy(exec);
+        if (bs)
+        {
+            (void)hipStreamSynchronize(bs);
+            (void)hipStreamDestroy(bs);
+        }
+    }
+};
+
+namespace agwn
+{
+namespace
+{
+__global__ void __launch_bounds__(kB) k_zero_u64(unsigned long long* p, int n)
+{
+    if (int(threadIdx.x) < n)
+        p[threadIdx.x] = 0ull;
+}
+
+// per-mesh largest query norm (as float bits: non-negative floats order like their bit patterns)
+__global__ void __launch_bounds__(kB) k_seg_max_norm(float const* __restrict__ pos, int64_t n,
+                                                     int64_t const* __restrict__ qoff, int64_t k,
+                                                     unsigned int* __restrict__ out)
+{
+    int64_t const i = int64_t(blockIdx.x) * kB + threadIdx.x;
+    if (i >= n)
+        return;
+    // mesh of query i: the last offset <= i (binary search over k + 1 offsets)
+    int64_t lo = 0, hi = k;
+    while (hi - lo > 1)
+    {
+        int64_t const mid = (lo + hi) / 2;
+        if (qoff[mid] <= i)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    double const x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
+    float const r = float(sqrt(x * x + y * y + z * z)) * 1.00001f + 1e-30f;
+    atomicMax(out + lo, __float_as_uint(r));
+}
+
+int batch_create(int64_t k, float const* verts, int64_t const* voff, int32_t const* idx, int64_t const* ioff,
+                 float const* seg7, int64_t const* soff, agwn_batch** out)
+{
+    AGWN_CHECK(out != nullptr, "out is null");
+    *out = nullptr;
+    AGWN_CHECK(k > 0 && voff && ioff && soff, "a batch needs k > 0 meshes and their offset arrays");
+    AGWN_CHECK(voff[0] == 0 && ioff[0] == 0 && soff[0] == 0, "offsets must start at 0");
+    for (int64_t m = 0; m < k; ++m)
+    {
+        AGWN_CHECK(voff[m + 1] >= voff[m] && ioff[m + 1] >= ioff[m] && soff[m + 1] >= soff[m],
+                   "offsets must be non-decreasing (mesh %lld)", (long long)m);
+        AGWN_CHECK((ioff[m + 1] - ioff[m]) % 3 == 0, "mesh %lld: index count not a multiple of 3", (long long)m);
+    }
+    AGWN_CHECK(voff[k] == 0 || verts, "verts is null");
+    AGWN_CHECK(ioff[k] == 0 || idx, "idx is null");
+    AGWN_CHECK(soff[k] == 0 || seg7, "seg7 is null");
+    AGWN_TRY(ensure_device());
+    auto b = std::make_unique<agwn_batch>();
+    b->k = k;
+    b->pt_off.assign(size_t(k + 1), 0);
+    b->nunit.assign(size_t(k), 0);
+    std::vector<float4> all;
+    for (int64_t m = 0; m < k; ++m)
+    {
+        int64_t const nv = voff[m + 1] - voff[m];
+        int64_t const ni = ioff[m + 1] - ioff[m];
+        int32_t const* im = idx + ioff[m];
+        for (int64_t i = 0; i < ni; ++i)
+            AGWN_CHECK(im[i] >= 0 && im[i] < nv, "mesh %lld: index %lld = %d out of range [0, %lld)", (long long)m,
+                       (long long)i, im[i], (long long)nv);
+        auto mesh = std::make_unique<mesh_t<float>>();
+        AGWN_TRY(mesh->create(verts + 3 * voff[m], nv, im, ni));
+        b->meshes.push_back(std::move(mesh));
+        std::vector<float4> st;
+        int64_t nu = 0;
+        AGWN_CHECK(chain_stream(seg7 + 7 * soff[m], soff[m + 1] - soff[m], st, nu),
+                   "mesh %lld: batched evaluation needs integer boundary weights (|w| <= 16)", (long long)m);
+        all.insert(all.end(), st.begin(), st.end());
+        b->pt_off[size_t(m + 1)] = int64_t(all.size());
+        b->nunit[size_t(m)] = nu;
+    }
+    AGWN_HIP(hipStreamCreateWithFlags(&b->bs, hipStreamNonBlocking));
+    AGWN_TRY(b->stream.alloc(std::max<size_t>(all.size(), 1) * sizeof(float4), nullptr));
+    if (!all.empty())
+        AGWN_HIP(hipMemcpy(b->stream.p, all.data(), all.size() * sizeof(float4), hipMemcpyHostToDevice));
+    AGWN_TRY(b->zero.alloc(sizeof(int32_t), nullptr));
+    AGWN_HIP(hipMemset(b->zero.p, 0, sizeof(int32_t)));
+    int dev = 0;
+    AGWN_HIP(hipGetDevice(&dev));
+    AGWN_HIP(hipDeviceGetAttribute(&b->ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    AGWN_HIP(hipDeviceSynchronize());
+    *out = b.release();
+    return ok;
+}
+
+// Prepares the per-direction state of every mesh and the work lists for `variant` (meshes listed in `which`, all
+// when empty) into the batch's buffers. Synchronous.
+// per-mesh bound on the query norms (synchronous)
+int batch_qmax(agwn_batch& B, float const* pos, int64_t const* qoff, std::vector<float>& qmax)
+{
+    hipStream_t const s = B.bs;
+    int64_t const k = B.k;
+    int64_t const n = qoff[k];
+    qmax.assign(size_t(k), 0.0f);
+    if (n == 0)
+        return ok;
+    dbuf dq, mx;
+    AGWN_TRY(dq.alloc(size_t(k + 1) * sizeof(int64_t), s));
+    AGWN_TRY(mx.alloc(size_t(k) * sizeof(unsigned), s));
+    AGWN_HIP(hipMemcpyAsync(dq.p, qoff, size_t(k + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
+    AGWN_HIP(hipMemsetAsync(mx.p, 0, size_t(k) * sizeof(unsigned), s));
+    k_seg_max_norm<<<dim3(unsigned((n + kB - 1) / kB)), dim3(kB), 0, s>>>(pos, n, dq.as<int64_t>(), k,
+                                                                          mx.as<unsigned>());
+    AGWN_HIP(hipGetLastError());
+    AGWN_HIP(hipMemcpyAsync(qmax.data(), mx.p, size_t(k) * sizeof(float), hipMemcpyDeviceToHost, s));
+    AGWN_HIP(hipStreamSynchronize(s));
+    return ok;
+}
+
+int batch_prepare(agwn_batch& B, float const* pos, int64_t const* qoff, float const x0f[3], int variant,
+                  std::vector<int64_t> const& which)
+{
+    hipStream_t const s = B.bs;
+    int64_t const k = B.k;
+    int64_t const n = qoff[k];
+    std::vector<float> qmax;
+    AGWN_TRY(batch_qmax(B, pos, qoff, qmax));
+    std::vector<char> use(size_t(k), which.empty() ? 1 : 0);
+    for (int64_t m : which)
+        use[size_t(m)] = 1;
+    std::vector<batch_mesh_desc> cd(static_cast<size_t>(k));
+    std::vector<batch_frac_desc> fd(static_cast<size_t>(k));
+    B.states.assign(size_t(k), nullptr);
+    for (int64_t m = 0; m < k; ++m)
+    {
+        mesh_t<float>& M = *B.meshes[size_t(m)];
+        batch_mesh_desc d{};
+        d.nbrute = B.zero.as<int32_t>();
+        d.q_end = qoff[m + 1];
+        if (use[size_t(m)] && qoff[m + 1] > qoff[m] && M.ntri > 0)
+        {
+            std::shared_ptr<dir_state<float>> st;
+            AGWN_TRY(M.prepare(x0f, double(qmax[size_t(m)]), s, st));
+            d.nodes = st->nodes;
+            d.rec = st->rec;
+            d.brute = st->brute;
+            d.nbrute = st->nbrute;
+            d.use_bvh = M.ntri > 1 ? 1 : 0;
+            B.states[size_t(m)] = st;
+        }
+        cd[size_t(m)] = d;
+        fd[size_t(m)] = batch_frac_desc{B.pt_off[size_t(m)], B.pt_off[size_t(m + 1)] - B.pt_off[size_t(m)], qoff[m + 1]};
+    }
+    // work items, longest first: counting by triangle count, the fractional term by stream length
+    int fq = 0, fb = 0;
+    frac_batch_shape(variant, fq, fb);
+    std::vector<int64_t> order(static_cast<size_t>(k));
+    for (int64_t m = 0; m < k; ++m)
+        order[size_t(m)] = m;
+    std::vector<int32_t> cim, fim;
+    std::vector<int64_t> ciq, fiq;
+    std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
+        return B.meshes[size_t(a)]->ntri > B.meshes[size_t(b)]->ntri;
+    });
+    for (int64_t m : order) // meshes without triangles too: their counts must be written (as 0)
+        if (use[size_t(m)])
+            for (int64_t q = qoff[m]; q < qoff[m + 1]; q += kTB)
+            {
+                cim.push_back(int32_t(m));
+                ciq.push_back(q);
+            }
+    std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
+        return B.pt_off[size_t(a) + 1] - B.pt_off[size_t(a)] > B.pt_off[size_t(b) + 1] - B.pt_off[size_t(b)];
+    });
+    for (int64_t m : order)
+        if (use[size_t(m)])
+            for (int64_t q = qoff[m]; q < qoff[m + 1]; q += int64_t(fq) * fb)
+            {
+                fim.push_back(int32_t(m));
+                fiq.push_back(q);
+            }
+    B.ncitems = int64_t(cim.size());
+    B.nfitems = int64_t(fim.size());
+    auto up = [&](dbuf& d, void const* h, size_t bytes) -> int {
+        AGWN_TRY(d.alloc(std::max<size_t>(bytes, 16), nullptr));
+        if (bytes)
+            AGWN_HIP(hipMemcpyAsync(d.p, h, bytes, hipMemcpyHostToDevice, s));
+        return ok;
+    };
+    AGWN_TRY(up(B.cdesc, cd.data(), cd.size() * sizeof(batch_mesh_desc)));
+    AGWN_TRY(up(B.fdesc, fd.data(), fd.size() * sizeof(batch_frac_desc)));
+    AGWN_TRY(up(B.citem_m, cim.data(), cim.size() * sizeof(int32_t)));
+    AGWN_TRY(up(B.citem_q, ciq.data(), ciq.size() * sizeof(int64_t)));
+    AGWN_TRY(up(B.fitem_m, fim.data(), fim.size() * sizeof(int32_t)));
+    AGWN_TRY(up(B.fitem_q, fiq.data(), fiq.size() * sizeof(int64_t)));
+    AGWN_TRY(B.counters.alloc(2 * sizeof(unsigned long long), nullptr));
+    AGWN_TRY(B.counts.alloc(size_t(std::max<int64_t>(n, 1)) * sizeof(int32_t), nullptr));
+    int64_t const npts = B.pt_off[size_t(k)];
+    AGWN_TRY(B.p32.alloc(size_t(std::max<int64_t>(npts, 1)) * 2 * sizeof(float4), nullptr));
+    AGWN_HIP(hipStreamSynchronize(s));
+    return ok;
+}
+
+// The batch's launches for the prepared lists: stream rotation, counter reset, counting, fractional + counts.
+int batch_launches(agwn_batch& B, float const* pos, float const x0f[3], int variant, float* out)
+{
+    hipStream_t const s = B.bs;
+    double const x0d[3] = {double(x0f[0]), double(x0f[1]), double(x0f[2])};
+    int64_t const npts = B.pt_off[size_t(B.k)];
+    AGWN_TRY(frac_stream_rotate(B.stream.as<float4 const>(), npts, x0d, B.p32.as<float4>(), s));
+    k_zero_u64<<<dim3(1), dim3(kB), 0, s>>>(B.counters.as<unsigned long long>(), 2);
+    AGWN_HIP(hipGetLastError());
+    if (B.ncitems > 0)
+    {
+        int const blocks = int(std::min<int64_t>(B.ncitems, int64_t(B.ncu) * 4));
+        k_count_batch<<<dim3(unsigned(blocks)), dim3(kTB), 0, s>>>(
+            pos, B.cdesc.as<batch_mesh_desc>(), B.citem_m.as<int32_t>(), B.citem_q.as<int64_t>(), B.ncitems,
+            B.counters.as<unsigned long long>(), x0f[0], x0f[1], x0f[2], B.counts.as<int32_t>());
+        AGWN_HIP(hipGetLastError());
+    }
+    if (B.nfitems > 0)
+    {
+        int fq = 0, fb = 0;
+        frac_batch_shape(variant, fq, fb);
+        int const per_cu = fb == 256 ? 8 : 16;
+        int const blocks = int(std::min<int64_t>(B.nfitems, int64_t(B.ncu) * per_cu));
+        AGWN_TRY(frac_batch_launch(variant, blocks, pos, B.fdesc.as<batch_frac_desc>(), B.fitem_m.as<int32_t>(),
+                                   B.fitem_q.as<int64_t>(), B.nfitems, B.counters.as<unsigned long long>() + 1,
+                                   B.p32.as<float4>(), x0d, B.counts.as<int32_t>(), out, s));
+    }
+    return ok;
+}
+
+// Autotuner: times every fractional kernel shape on every 8th mesh (same work for each shape) and keeps the fastest.
+int batch_tune(agwn_batch& B, float const* pos, int64_t const* qoff, float const x0f[3], float* out)
+{
+    std::vector<int64_t> sample;
+    for (int64_t m = 0; m < B.k; m += 8)
+        sample.push_back(m);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    AGWN_HIP(hipEventCreate(&e0));
+    AGWN_HIP(hipEventCreate(&e1));
+    struct eguard
+    {
+        hipEvent_t a, b;
+        ~eguard()
+        {
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+        }
+    } eg{e0, e1};
+    int best = 0;
+    double best_ms = 1e300;
+    for (int v = 0; v < frac_batch_variants(); ++v)
+    {
+        AGWN_TRY(batch_prepare(B, pos, qoff, x0f, v, sample));
+        AGWN_TRY(batch_launches(B, pos, x0f, v, out)); // warm-up
+        AGWN_HIP(hipEventRecord(e0, B.bs));
+        AGWN_TRY(batch_launches(B, pos, x0f, v, out));
+        AGWN_HIP(hipEventRecord(e1, B.bs));
+        AGWN_HIP(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        AGWN_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (v < 8)
+            B.tune_ms[v] = ms;
+        if (ms < best_ms)
+        {
+            best_ms = ms;
+            best = v;
+        }
+    }
+    B.variant = best;
+    B.g_pos = nullptr; // the lists now describe the sample only
+    return ok;
+}
+
+int batch_eval(agwn_batch* b, float const* pos, int64_t const* qoff, float const* x0, float* out, hipStream_t s)
+{
+    AGWN_CHECK(b != nullptr, "batch handle is null");
+    AGWN_CHECK(qoff && x0, "null argument");
+    AGWN_CHECK(qoff[0] == 0, "query offsets must start at 0");
+    for (int64_t m = 0; m < b->k; ++m)
+        AGWN_CHECK(qoff[m + 1] >= qoff[m], "query offsets must be non-decreasing (mesh %lld)", (long long)m);
+    int64_t const n = qoff[b->k];
+    AGWN_CHECK(n == 0 || (pos && out), "null query or output array");
+    std::lock_guard<std::mutex> lk(b->mu);
+    agwn_batch& B = *b;
+    // order the batch's stream after the caller's work on `s`
+    hipEvent_t ev = nullptr;
+    AGWN_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    struct evguard
+    {
+        hipEvent_t e;
+        ~evguard() { (void)hipEventDestroy(e); }
+    } eg{ev};
+    AGWN_HIP(hipEventRecord(ev, s));
+    AGWN_HIP(hipStreamWaitEvent(B.bs, ev, 0));
+    if (n == 0)
+        return ok;
+    if (B.variant < 0)
+        AGWN_TRY(batch_tune(B, pos, qoff, x0, out));
+    std::vector<int64_t> q(qoff, qoff + B.k + 1);
+    bool same = B.exec && B.g_pos == pos && B.g_out == out && B.g_x0[0] == x0[0] && B.g_x0[1] == x0[1] &&
+                B.g_x0[2] == x0[2] && B.g_qoff == q && B.g_variant == B.variant;
+    if (same)
+    {
+        // the graph's prepared BVH states must still cover the queries (their contents may have changed in place)
+        std::vector<float> qmax;
+        AGWN_TRY(batch_qmax(B, pos, qoff, qmax));
+        for (int64_t m = 0; m < B.k && same; ++m)
+        {
+            mesh_t<float> const& M = *B.meshes[size_t(m)];
+            if (qoff[m + 1] > qoff[m] && M.ntri > 0)
+                same = B.states[size_t(m)] && double(qmax[size_t(m)]) + M.vmax <= B.states[size_t(m)]->dsum;
+        }
+    }
+    if (!same)
+    {
+        if (B.exec)
+        {
+            AGWN_HIP(hipGraphExecDestroy(B.exec));
+            B.exec = nullptr;
+        }
+        AGWN_TRY(batch_prepare(B, pos, qoff, x0, B.variant, {}));
+        hipGraph_t graph = nullptr;
+        AGWN_HIP(hipStreamBeginCapture(B.bs, hipStreamCaptureModeThreadLocal));
+        int const st = batch_launches(B, pos, x0, B.variant, out);
+        hipError_t const ce = hipStreamEndCapture(B.bs, &graph);
+        AGWN_TRY(st);
+        AGWN_HIP(ce);
+        hipError_t const ie = hipGraphInstantiate(&B.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        AGWN_HIP(ie);
+        B.g_pos = pos;
+        B.g_out = out;
+        B.g_x0[0] = x0[0];
+        B.g_x0[1] = x0[1];
+        B.g_x0[2] = x0[2];
+        B.g_qoff = q;
+        B.g_variant = B.variant;
+    }
+    AGWN_HIP(hipGraphLaunch(B.exec, B.bs));
+    AGWN_HIP(hipStreamSynchronize(B.bs));
+    return ok;
+}
+} // namespace
+} // namespace agwn
+
+extern "C" {
+int agwn_batch_create(int64_t k, float const* verts, int64_t const* vert_off, int32_t const* idx,
+                      int64_t const* idx_off, float const* seg7, int64_t const* seg_off, agwn_batch** out)
+{
+    return guarded2([&] { return batch_create(k, verts, vert_off, idx, idx_off, seg7, seg_off, out); });
+}
+int agwn_batch_destroy(agwn_batch* b)
+{
+    clear_error();
+    delete b;
+    return ok;
+}
+int agwn_batch_tune(agwn_batch* b, float const* pos, int64_t const* q_off, float const* x0, float* out,
+                    agwn_stream s)
+{
+    return guarded2([&]() -> int {
+        AGWN_CHECK(b != nullptr, "batch handle is null");
+        AGWN_CHECK(q_off && x0, "null argument");
+        std::lock_guard<std::mutex> lk(b->mu);
+        AGWN_HIP(hipStreamSynchronize(hipStream_t(s)));
+        return batch_tune(*b, pos, q_off, x0, out);
+    });
+}
+int agwn_batch_eval_dev(agwn_batch* b, float const* pos, int64_t const* q_off, float const* x0, float* out,
+                        agwn_stream s)
+{
+    return guarded2([&] { return batch_eval(b, pos, q_off, x0, out, hipStream_t(s)); });
+}
+int agwn_batch_info(agwn_batch const* b, int64_t* k, int64_t* ntri, int64_t* nunit, int* frac_q, int* frac_block,
+                    double* tune_ms)
+{
+    return guarded2([&]() -> int {
+        AGWN_CHECK(b != nullptr, "batch handle is null");
+        int64_t t = 0, u = 0;
+        for (int64_t m = 0; m < b->k; ++m)
+        {
+            t += b->meshes[size_t(m)]->ntri;
+            u += b->nunit[size_t(m)];
+        }
+        if (k)
+            *k = b->k;
+        if (ntri)
+            *ntri = t;
+        if (nunit)
+            *nunit = u;
+        int q = 0, blk = 0;
+        if (b->variant >= 0)
+            frac_batch_shape(b->variant, q, blk);
+        if (frac_q)
+            *frac_q = q;
+        if (frac_block)
+            *frac_block = blk;
+        if (tune_ms)
+            for (int v = 0; v < frac_batch_variants() && v < 8; ++v)
+                tune_ms[v] = b->tune_ms[v];
+        return ok;
     });
 }
 } // extern "C"

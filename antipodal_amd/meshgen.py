@@ -15,6 +15,10 @@ Workloads (sizes as named in the benchmark plan):
                         peaks near hole radius 0.074, at about 31k segments.)
   grid_points()         a dense n^3 lattice over a bounding box expanded by 10 % (5 % each side)
   uniform_points()      uniform points in a box
+  surface_points()      area-uniform points on a mesh with their unit face normals
+  precision_stress()    the precision-stress configuration: the torus family at 3040 x 1063 quads with 5000 holes of
+                        radius 0.02 (4,038,497 triangles, 429,869 boundary segments), queries half uniform in the
+                        10 %-expanded box and half at |N(0, 1e-3)| along the outward normal of surface points
 """
 from __future__ import annotations
 
@@ -148,6 +152,84 @@ def holed_torus_mesh(seed: int = 1, nu: int = 1353, nv: int = 473, R: float = 1.
 
         d, _ = cKDTree(centres).query(cent, k=1)
         tri = tri[d > hole_radius]
+    return _compact(verts, tri.reshape(-1))
+
+
+def surface_points(verts, idx, n: int, seed: int, stream: int = 10):
+    """n area-uniform points on the triangles (seeded) and their unit face normals (right-hand winding)."""
+    v = np.asarray(verts, dtype=np.float64).reshape(-1, 3)
+    tri = np.asarray(idx, dtype=np.int64).reshape(-1, 3)
+    a, b, c = v[tri[:, 0]], v[tri[:, 1]], v[tri[:, 2]]
+    nrm = np.cross(b - a, c - a)
+    area = np.linalg.norm(nrm, axis=1)
+    ok = area > 0
+    nrm[ok] /= area[ok, None]
+    cdf = np.cumsum(area)
+    k = np.minimum(np.searchsorted(cdf, uniform01(seed, stream, n) * cdf[-1], side="right"), len(tri) - 1)
+    r1 = np.sqrt(uniform01(seed, stream + 1, n))
+    r2 = uniform01(seed, stream + 2, n)
+    p = (1 - r1)[:, None] * a[k] + (r1 * (1 - r2))[:, None] * b[k] + (r1 * r2)[:, None] * c[k]
+    return p, nrm[k]
+
+
+def precision_stress(n_queries: int = 10_000_000, seed: int = 1):
+    """(verts, idx, queries float64 (n, 3)) of the precision-stress configuration."""
+    verts, idx = holed_torus_mesh(seed=seed, nu=3040, nv=1063, holes=5000, hole_radius=0.02)
+    lo, hi = expanded_bbox(verts)
+    nu = n_queries // 2
+    q_uniform = uniform_points(seed, nu, lo, hi, stream=20)
+    p, nrm = surface_points(verts, idx, n_queries - nu, seed, stream=30)
+    off = np.abs(normal(seed, 40, n_queries - nu)) * 1e-3
+    q_near = p + off[:, None] * nrm
+    return verts, idx, np.concatenate([q_uniform, q_near])
+
+
+def uv_sphere_mesh(nu: int, nv: int, radius: float = 1.0):
+    """Latitude-longitude sphere: nu longitudes x nv latitude bands, poles as single vertices (2 nu (nv - 1) tris)."""
+    nu, nv = max(nu, 3), max(nv, 2)
+    th = np.pi * np.arange(1, nv) / nv
+    ph = 2.0 * np.pi * np.arange(nu) / nu
+    tt, pp = np.meshgrid(th, ph, indexing="ij")  # (nv - 1, nu)
+    ring = np.stack([np.sin(tt) * np.cos(pp), np.sin(tt) * np.sin(pp), np.cos(tt)], -1).reshape(-1, 3)
+    verts = np.concatenate([[[0.0, 0.0, 1.0]], ring, [[0.0, 0.0, -1.0]]]) * radius
+    ring_id = lambda i, j: 1 + i * nu + (j % nu)
+    tris = []
+    j = np.arange(nu)
+    tris.append(np.stack([np.zeros(nu, np.int64), ring_id(0, j), ring_id(0, j + 1)], -1))
+    for i in range(nv - 2):
+        a, b, c, d = ring_id(i, j), ring_id(i + 1, j), ring_id(i + 1, j + 1), ring_id(i, j + 1)
+        tris.append(np.stack([a, b, c], -1))
+        tris.append(np.stack([a, c, d], -1))
+    last = len(verts) - 1
+    tris.append(np.stack([np.full(nu, last), ring_id(nv - 2, j + 1), ring_id(nv - 2, j)], -1))
+    return verts, np.concatenate(tris).astype(np.int32).reshape(-1)
+
+
+def open_mesh(kind: str, ntri: int, holes: int, seed: int, noise: float = 0.01):
+    """A noisy sphere or torus of about `ntri` triangles with `holes` disk holes (radius scaled so that the holes cover
+    about a quarter of the surface at most), seeded."""
+    if kind == "sphere":
+        nv = max(2, int(round(math.sqrt(ntri / 4.0))))
+        nu = max(3, int(round(ntri / (2.0 * max(nv - 1, 1)))))
+        verts, idx = uv_sphere_mesh(nu, nv)
+        scale, area = 1.0, 4.0 * math.pi
+    else:
+        nvv = max(3, int(round(math.sqrt(ntri / 2.0 * 0.35))))
+        nuu = max(3, int(round(ntri / (2.0 * nvv))))
+        verts, idx = torus_mesh(nuu, nvv)
+        scale, area = 0.35, 4.0 * math.pi ** 2 * 0.35
+    nvtx = len(verts)
+    verts = verts + (noise * scale) * normal(seed, 1, 3 * nvtx).reshape(nvtx, 3)
+    tri = idx.reshape(-1, 3)
+    cent = verts[tri].mean(axis=1)
+    if holes > 0:
+        rad = math.sqrt(0.25 * area / (math.pi * holes))
+        rad = min(rad, 0.5 * scale)
+        pick = (uniform01(seed, 2, holes) * len(tri)).astype(np.int64)
+        from scipy.spatial import cKDTree
+
+        d, _ = cKDTree(cent[pick]).query(cent, k=1)
+        tri = tri[d > rad]
     return _compact(verts, tri.reshape(-1))
 
 

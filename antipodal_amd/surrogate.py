@@ -256,7 +256,7 @@ def fit_surrogate(verts, idx, boundary, x0, lo, hi, n_train: int = 1 << 24, n_va
     va_p = pts[n_train:].contiguous()
     tr_y = w[:n_train].clamp(*target_clip).contiguous()
     va_y = w[n_train:]
-    info.update(eps=eps, jitter=jitter, max_freq=max_freq, n_train=n_train)
+    info.update(eps=eps, voxel=voxel, jitter=jitter, max_freq=max_freq, n_train=n_train)
     t2 = time.perf_counter()
     losses = net.train(tr_p, tr_y, batch=batch, epochs=epochs, lr=lr, seed=seed)
     torch.cuda.synchronize()

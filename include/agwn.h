@@ -159,6 +159,32 @@ int agwn_boundary_f64_dev(const double* verts, int64_t nv, const int32_t* idx, i
                           int64_t cap, int64_t* count, agwn_stream stream);
 
 /* ---------------------------------------------------------------------------------------------------------------
+ * Batched evaluation of many meshes with uneven query counts: eval_gwnr_mesh_batch (gwn_mesh.hh:201-219) of every
+ * mesh's own queries against that mesh, for the whole batch in two persistent launches (integer term, then
+ * fractional term + counts) captured in a HIP graph and replayed while the inputs stay the same. Per query the
+ * arithmetic is the single-mesh path's (agwn_full_batch_f32 with the same x0), so results are identical.
+ * Meshes are concatenated: verts (host float[3 * vert_off[k]]), idx (host int32, indices local to each mesh,
+ * idx_off[k + 1], multiples of 3), boundaries seg7 (host float[7 * seg_off[k]], integer weights with |w| <= 16, e.g.
+ * build_boundary_segments output). Queries (device float[3 * q_off[k]]) are concatenated per mesh in the same order,
+ * q_off host int64[k + 1]; out device float[q_off[k]]; x0 host float[3] (unit), shared by the batch.
+ * --------------------------------------------------------------------------------------------------------------- */
+typedef struct agwn_batch agwn_batch;
+int agwn_batch_create(int64_t k, const float* verts, const int64_t* vert_off, const int32_t* idx,
+                      const int64_t* idx_off, const float* seg7, const int64_t* seg_off, agwn_batch** out);
+int agwn_batch_destroy(agwn_batch* batch);
+/* Autotuner: times each fractional kernel tile shape (queries per lane x workgroup size) on every 8th mesh and keeps
+ * the fastest (run by the first agwn_batch_eval_dev if never called). `out` is used as scratch. Synchronous. */
+int agwn_batch_tune(agwn_batch* batch, const float* pos, const int64_t* q_off, const float* x0, float* out,
+                    agwn_stream stream);
+/* out[i] = full GWN of query i against its mesh. Ordered after prior work on `stream`; returns when done. */
+int agwn_batch_eval_dev(agwn_batch* batch, const float* pos, const int64_t* q_off, const float* x0, float* out,
+                        agwn_stream stream);
+/* Mesh count, triangles, unit boundary segments, the tuned shape (0, 0 before tuning) and the tuner's timings (ms,
+ * up to 8 shapes, in shape order); any output may be NULL. */
+int agwn_batch_info(const agwn_batch* batch, int64_t* k, int64_t* ntri, int64_t* nunit, int* frac_q, int* frac_block,
+                    double* tune_ms);
+
+/* ---------------------------------------------------------------------------------------------------------------
  * Learned inside/outside surrogate (no counterpart in the reference: an accelerator in front of the exact path, which
  * stays the judge). A per-mesh MLP -- 192 Fourier features of the position normalised to a box (32 frequencies
  * max_freq^(k/31) per axis, sin and cos), 4 hidden layers x 256 SiLU, linear output -- with BF16 weights/activations and
