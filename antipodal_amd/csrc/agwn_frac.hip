@@ -200,6 +200,14 @@ __device__ __forceinline__ float pair_f64_atan_f32(double v0x, double v0y, doubl
     return atan2_f32(float(ldexp(num, -e)), float(ldexp(den, -e)));
 }
 
+// x unchanged, but opaque to the optimiser: work that depends on it stays where it is written (used to keep the
+// FP64 conversions of the rare ill-conditioned branch from being hoisted onto the common path). Emits no instruction.
+__device__ __forceinline__ float opaque(float x)
+{
+    asm volatile("" : "+v"(x));
+    return x;
+}
+
 template <class Src>
 __device__ __forceinline__ void query_f64(Src const& src, int64_t i, int64_t n, frame const& R, double& x, double& y,
                                           double& z)
@@ -373,8 +381,9 @@ __device__ __forceinline__ void chain_block(Src const& src, int64_t base, int64_
             tile[B + threadIdx.x] = pts[f0 + B + threadIdx.x];
         __syncthreads();
         int const cnt = int(npts - t0 < B ? npts - t0 : B);
-        for (int j = 0; j < cnt; ++j)
-        {
+        // one stream point; the loop below takes two per iteration, so that the previous point's registers are
+        // renamed rather than copied
+        auto step = [&](int const j) __attribute__((always_inline)) {
             float4 const g = tile[2 * j], gl = tile[2 * j + 1]; // same address in every lane: LDS broadcasts
             bool const seg = __builtin_amdgcn_readfirstlane(__float_as_int(g.w)) != 0;
 #pragma unroll
@@ -407,13 +416,13 @@ __device__ __forceinline__ void chain_block(Src const& src, int64_t base, int64_
                             h0 = pts[f0 - 2];
                             l0 = pts[f0 - 1];
                         }
-                        double const qx = double(ph[q][0]) + double(pl[q][0]);
-                        double const qy = double(ph[q][1]) + double(pl[q][1]);
-                        double const qz = double(ph[q][2]) + double(pl[q][2]);
-                        acc[q] += double(pair_f64_atan_f32(
-                            (double(h0.x) + double(l0.x)) - qx, (double(h0.y) + double(l0.y)) - qy,
-                            (double(h0.z) + double(l0.z)) - qz, (double(g.x) + double(gl.x)) - qx,
-                            (double(g.y) + double(gl.y)) - qy, (double(g.z) + double(gl.z)) - qz, P));
+                        // hi + lo is exact in double
+                        auto d = [](float h, float l) { return double(opaque(h)) + double(opaque(l)); };
+                        double const qx = d(ph[q][0], pl[q][0]), qy = d(ph[q][1], pl[q][1]);
+                        double const qz = d(ph[q][2], pl[q][2]);
+                        acc[q] += double(pair_f64_atan_f32(d(h0.x, l0.x) - qx, d(h0.y, l0.y) - qy,
+                                                           d(h0.z, l0.z) - qz, d(g.x, gl.x) - qx, d(g.y, gl.y) - qy,
+                                                           d(g.z, gl.z) - qz, P));
                         num = 0.0f;
                         den = 1.0f;
                     }
@@ -437,7 +446,15 @@ __device__ __forceinline__ void chain_block(Src const& src, int64_t base, int64_
             }
             if (seg)
                 pending = !pending;
+        };
+        int j = 0;
+        for (; j + 1 < cnt; j += 2)
+        {
+            step(j);
+            step(j + 1);
         }
+        if (j < cnt)
+            step(j);
     }
 
 #pragma unroll
@@ -543,7 +560,7 @@ pole make_pole(double x0x, double x0y, double x0z)
 
 // ill-conditioning threshold of the FP32 kernel: pairs with |num| + |den| < tau * l0 * l1 go to FP64
 constexpr float kTau = 1.0f / 32.0f;
-constexpr int kQ = 2;
+constexpr int kQ = 1;
 } // namespace
 
 frame make_pole_frame(double x0x, double x0y, double x0z) { return make_frame_impl(x0x, x0y, x0z); }
