@@ -1101,27 +1101,60 @@ __global__ void __launch_bounds__(kB) k_zero_u64(unsigned long long* p, int n)
         p[threadIdx.x] = 0ull;
 }
 
-// per-mesh largest query norm (as float bits: non-negative floats order like their bit patterns)
+// per-mesh largest query norm (as float bits: non-negative floats order like their bit patterns). Each lane takes
+// kNormPerLane queries strided by the block width (coalesced) and keeps a running maximum for the mesh it is in,
+// flushing it with one atomic when the queries move on to a later mesh; at the end the lanes of a wave that all
+// ended in the same mesh (the common case: a mesh's queries are contiguous) reduce first and issue one atomic.
+constexpr int kNormPerLane = 16;
 __global__ void __launch_bounds__(kB) k_seg_max_norm(float const* __restrict__ pos, int64_t n,
                                                      int64_t const* __restrict__ qoff, int64_t k,
                                                      unsigned int* __restrict__ out)
 {
-    int64_t const i = int64_t(blockIdx.x) * kB + threadIdx.x;
-    if (i >= n)
-        return;
-    // mesh of query i: the last offset <= i (binary search over k + 1 offsets)
-    int64_t lo = 0, hi = k;
-    while (hi - lo > 1)
+    int64_t const i0 = int64_t(blockIdx.x) * (kB * kNormPerLane) + threadIdx.x;
+    int64_t m = -1;   // mesh of the running maximum
+    unsigned int r = 0; // running maximum; 0 = none (every norm below is >= 1e-30)
+    if (i0 < n)
     {
-        int64_t const mid = (lo + hi) / 2;
-        if (qoff[mid] <= i)
-            lo = mid;
-        else
-            hi = mid;
+        // mesh of query i0: the last offset <= i0 (binary search over the k + 1 offsets)
+        int64_t lo = 0, hi = k;
+        while (hi - lo > 1)
+        {
+            int64_t const mid = (lo + hi) / 2;
+            if (qoff[mid] <= i0)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        m = lo;
+        for (int j = 0; j < kNormPerLane; ++j)
+        {
+            int64_t const i = i0 + int64_t(j) * kB;
+            if (i >= n)
+                break;
+            while (qoff[m + 1] <= i) // a later mesh (qoff[k] = n > i ends this; meshes without queries are passed)
+            {
+                if (r)
+                    atomicMax(out + m, r);
+                r = 0;
+                ++m;
+            }
+            double const x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
+            float const nrm = float(sqrt(x * x + y * y + z * z)) * 1.00001f + 1e-30f;
+            r = max(r, __float_as_uint(nrm));
+        }
     }
-    double const x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
-    float const r = float(sqrt(x * x + y * y + z * z)) * 1.00001f + 1e-30f;
-    atomicMax(out + lo, __float_as_uint(r));
+    // every lane of the (full) block reaches this point: the wave-wide vote and shuffles see all 64 lanes
+    int const mi = int(m);
+    int const m0 = __shfl(mi, 0);
+    if (__all(mi == m0))
+    {
+        for (int off = 32; off > 0; off >>= 1)
+            r = max(r, __shfl_xor(r, off));
+        if ((threadIdx.x & 63) == 0 && m0 >= 0 && r)
+            atomicMax(out + m0, r);
+    }
+    else if (m >= 0 && r)
+        atomicMax(out + m, r);
 }
 
 int batch_create(int64_t k, float const* verts, int64_t const* voff, int32_t const* idx, int64_t const* ioff,
@@ -1130,6 +1163,7 @@ int batch_create(int64_t k, float const* verts, int64_t const* voff, int32_t con
     AGWN_CHECK(out != nullptr, "out is null");
     *out = nullptr;
     AGWN_CHECK(k > 0 && voff && ioff && soff, "a batch needs k > 0 meshes and their offset arrays");
+    AGWN_CHECK(k < (int64_t(1) << 31), "a batch holds at most 2^31 - 1 meshes (%lld)", (long long)k);
     AGWN_CHECK(voff[0] == 0 && ioff[0] == 0 && soff[0] == 0, "offsets must start at 0");
     for (int64_t m = 0; m < k; ++m)
     {
@@ -1195,8 +1229,9 @@ int batch_qmax(agwn_batch& B, float const* pos, int64_t const* qoff, std::vector
     AGWN_TRY(mx.alloc(size_t(k) * sizeof(unsigned), s));
     AGWN_HIP(hipMemcpyAsync(dq.p, qoff, size_t(k + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
     AGWN_HIP(hipMemsetAsync(mx.p, 0, size_t(k) * sizeof(unsigned), s));
-    k_seg_max_norm<<<dim3(unsigned((n + kB - 1) / kB)), dim3(kB), 0, s>>>(pos, n, dq.as<int64_t>(), k,
-                                                                          mx.as<unsigned>());
+    int64_t const per_block = int64_t(kB) * kNormPerLane;
+    k_seg_max_norm<<<dim3(unsigned((n + per_block - 1) / per_block)), dim3(kB), 0, s>>>(pos, n, dq.as<int64_t>(), k,
+                                                                                         mx.as<unsigned>());
     AGWN_HIP(hipGetLastError());
     AGWN_HIP(hipMemcpyAsync(qmax.data(), mx.p, size_t(k) * sizeof(float), hipMemcpyDeviceToHost, s));
     AGWN_HIP(hipStreamSynchronize(s));

@@ -162,6 +162,8 @@ def bench_batched(args, agwn, mg, torch) -> int:
         qn.append(int(math.exp(rng.uniform(math.log(1e3), math.log(1e6)))))
         lo, hi = mg.expanded_bbox(v)
         boxes.append((lo, hi))
+        if (m + 1) % 512 == 0:
+            log(f"generated {m + 1}/{k} meshes ({time.perf_counter() - t:.1f} s)")
     gen_s = time.perf_counter() - t
     q_off = np.concatenate([[0], np.cumsum(qn)]).astype(np.int64)
     nq = int(q_off[-1])
