@@ -11,17 +11,17 @@ lattice over the bounding box expanded by 10 % (134,217,728 points, generated in
              near-surface mask); each step labels the lattice with the fused inference kernel and routes to the exact
              kernels every point in the mask (voxels of edge eps/4 within eps = 0.10 % of the box diagonal of a
              triangle), every point with |pred - 0.5| <= delta, and a 1/1024 audit sample, which take their exact
-             labels. delta comes
-             from a held-out validation set (1.5 x the worst wrong-side margin + 0.02). Training happens before the
-             timed region and is reported (train_s).
+             labels. delta comes from a held-out validation set (1.5 x the worst wrong-side margin + 0.02). Training
+             happens before the timed region and is reported (train_s).
 The headline value is surrogate steps/s x queries; mislabels = labels that differ from the plain run's (must be 0).
 
 Reported beside it:
   roofline      the dominant kernel of a surrogate step (the exact fractional sum over the routed points, k_frac_f32,
                 unless inference takes longer): its launch on exactly the routed points, timed with HIP events on the
-                launch stream; algorithmic FLOP per launch = routed x segments x 56 (per-pair count of the design
-                notes) against MI355X FP32 vector peak 157.3 TFLOP/s (or 492,032 FLOP per point against the 2.5 PFLOP/s
-                BF16 MFMA peak for inference)
+                launch stream; algorithmic FLOP per launch = routed x segments x 56 (per pair: the 53 adds and
+                multiplies of the reference's pair expression, FMA = 2, plus 2 sqrt and 1 atan2) against MI355X FP32
+                vector peak 157.3 TFLOP/s (or 492,032 FLOP per point against the 2.5 PFLOP/s BF16 MFMA peak for
+                inference)
   cpu_baseline  the reference's own eval_gwnr_mesh_batch_fractional<float> (oracle/_ref, compiled from its headers)
                 with its OpenMPDispatcher on this host's cores, on a bounded seeded sample of the grid's queries,
                 scaled to queries/s; the integer term is counted as free on the CPU (no Embree here), which favours
@@ -244,8 +244,8 @@ def main() -> int:
         return copy_calibration()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path (queries shard trivially by "
-              "z-slab; see the design notes). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU only: there is no multi-GPU path (queries are independent, so the grid "
+              "would shard by z-slab with the mesh state replicated per GPU). Run with --gpus 1.", file=sys.stderr)
         return 2
 
     import torch
@@ -413,7 +413,7 @@ def main() -> int:
     if not args.no_cpu:
         rng = np.random.default_rng(0)
         sample = lattice_points(start, dx, dy, dz, nres, nres, nres,
-                                np.sort(rng.choice(nq, size=min(nq, 200_000), replace=False)))
+                                np.sort(rng.choice(nq, size=min(nq, 1_000_000), replace=False)))
         log("cpu baseline ...")
         cpu = cpu_baseline(sample, seg32, x0, args.cpu_seconds, f"{nres}^3 grid")
         if cpu:
