@@ -165,16 +165,43 @@ __device__ __forceinline__ void query_at(src_grid const& s, int64_t i, float& x,
 // stored as a stream of points (x, y, z, flag) in the caller's frame -- flag 1: a unit segment runs from the previous
 // point to this one; flag 0: a chain starts here. Consecutive segments then share an endpoint. chain == false: the
 // weights are general and the per-segment kernel runs instead.
+//
+// Every walk of a boundary that is a 1-cycle (build_boundary_segments' always is: the boundary of a triangle chain)
+// ends where it starts, so the walks are closed loops. For each, in double and the caller's frame: a bounding ball
+// (centre c = mean of its points, radius rho), the vector area a = sum over the fan from c of (1/2) u x w, the area of
+// that fan (a spanning surface inside the ball) and its first moment M_ij = sum of (triangle centroid - c)_i (n dA)_j.
+// They bound the loop's fractional contribution at a query outside the ball (frac_decide). loops_ok is false when
+// some walk is open (then there is no decision stage).
+struct loop_host
+{
+    double c[3];
+    double rho;
+    double a[3];
+    double area;
+    double m[9];
+    int64_t start, count; // stream entries [start, start + count)
+};
 struct frac_plan
 {
     dbuf buf;           // npts float4 entries (device)
     int64_t npts = 0;   // stream entries
     int64_t nunit = 0;  // unit segments
     bool chain = false;
+    std::vector<loop_host> loops;
+    bool loops_ok = false;
     float4 const* pts() const { return buf.as<float4 const>(); }
 };
 // Builds the plan from a host copy of the boundary (stream-ordered upload on `stream`; returns when it is done).
 int frac_plan_build(float const* seg7_host, int64_t nseg, frac_plan& plan, hipStream_t stream);
+
+// Inside/outside decision without the full boundary sum, for m query points (device, FP32 xyz) with their integer
+// terms `counts` (needs plan.chain && plan.loops_ok): per query, loops whose bound is small are replaced by a
+// second-order far-field expansion with a rigorous remainder bound, the others are summed exactly (FP32 chain
+// arithmetic, FP64 for ill-conditioned pairs). Where count + estimate is farther from 0.5 than the bound plus a margin
+// that covers the FP32 evaluation's own error, val[i] = that estimate (on the same side of 0.5 as the full GWN);
+// otherwise i is appended to `undecided` (device, m entries) and val[i] is left unwritten. nund: how many (host).
+int frac_decide(float const* pos, int64_t m, int32_t const* counts, frac_plan const& plan, double const x0[3],
+                float* val, int64_t* undecided, int64_t& nund, hipStream_t s);
 
 // Fractional launchers (agwn_frac.hip). seg7/out/counts are device pointers. counts may be null (fractional
 // only); otherwise out[i] = fractional + counts[i] (the full GWN of gwn_mesh.hh:213-218).

@@ -897,9 +897,33 @@ int grid_eval_dev(agwn_grid* g, float const* start, float const* dx, float const
     return frac_f32_planned(src_grid{c.gd}, c.n, *g->plan, g->seg7, g->nseg, c.x0d, cnt.as<int32_t>(), out, s);
 }
 
+// Gather / scatter of the points the far-field decision left open (list = indices into the routed arrays).
+__global__ void __launch_bounds__(kB) k_gather_undecided(float const* __restrict__ pts, int32_t const* __restrict__ cnt,
+                                                         int64_t const* __restrict__ list, int64_t n,
+                                                         float* __restrict__ opts, int32_t* __restrict__ ocnt)
+{
+    int64_t const k = int64_t(blockIdx.x) * kB + threadIdx.x;
+    if (k >= n)
+        return;
+    int64_t const i = list[k];
+    opts[3 * k] = pts[3 * i];
+    opts[3 * k + 1] = pts[3 * i + 1];
+    opts[3 * k + 2] = pts[3 * i + 2];
+    ocnt[k] = cnt[i];
+}
+__global__ void __launch_bounds__(kB) k_scatter_values(int64_t const* __restrict__ list, int64_t n,
+                                                       float const* __restrict__ v, float* __restrict__ out)
+{
+    int64_t const k = int64_t(blockIdx.x) * kB + threadIdx.x;
+    if (k < n)
+        out[list[k]] = v[k];
+}
+
 // Surrogate labelling with the exact FP32 path as fallback (see mlp_route): network labels for every query, then the
-// band and audit points are gathered, evaluated exactly (counts + fractional with the same kernels, x0 and query
-// coordinates as the plain evaluation) and relabelled.
+// mask, band and audit points are gathered and labelled exactly: their integer terms (the same kernel, x0 and query
+// coordinates as the plain evaluation), then the far-field decision (frac_decide) where the boundary is made of
+// closed loops, and the full FP32 boundary sum for the points it leaves open (or for all, otherwise). A decided
+// point's value is an estimate of its GWN proven to lie on the same side of 0.5 as the full evaluation.
 template <class Src>
 int label_surrogate(mesh_t<float>& mesh, agwn_mlp const* mlp, Src const& src, int64_t n, double qmax,
                     float const x0f[3], float const* seg7, int64_t nseg, frac_plan const* plan, float delta,
@@ -941,11 +965,47 @@ int label_surrogate(mesh_t<float>& mesh, agwn_mlp const* mlp, Src const& src, in
             AGWN_TRY(w.alloc(size_t(m) * sizeof(float), s));
             AGWN_TRY(gather_routed(src, L, m, pts.as<float>(), s));
             AGWN_TRY(mesh.count_src(src_array<float>{pts.as<float>()}, m, qmax, x0f, c.as<int32_t>(), s));
-            if (plan)
-                AGWN_TRY(frac_f32_planned(src_array<float>{pts.as<float>()}, m, *plan, seg7, nseg, x0d,
-                                          c.as<int32_t>(), w.as<float>(), s));
+            frac_plan local;
+            frac_plan const* pl = plan;
+            if (!pl)
+            {
+                std::vector<float> h(size_t(nseg) * 7);
+                if (nseg > 0)
+                {
+                    AGWN_HIP(hipMemcpyAsync(h.data(), seg7, h.size() * sizeof(float), hipMemcpyDeviceToHost, s));
+                    AGWN_HIP(hipStreamSynchronize(s));
+                }
+                AGWN_TRY(frac_plan_build(h.data(), nseg, local, s));
+                pl = &local;
+            }
+            if (pl->chain && pl->loops_ok)
+            {
+                // far-field decision first; only the points it cannot decide take the full boundary sum
+                dbuf und;
+                AGWN_TRY(und.alloc(size_t(m) * sizeof(int64_t), s));
+                int64_t nu = 0;
+                AGWN_TRY(frac_decide(pts.as<float>(), m, c.as<int32_t>(), *pl, x0d, w.as<float>(), und.as<int64_t>(),
+                                     nu, s));
+                if (nu > 0)
+                {
+                    dbuf up, uc, uw;
+                    AGWN_TRY(up.alloc(size_t(nu) * 3 * sizeof(float), s));
+                    AGWN_TRY(uc.alloc(size_t(nu) * sizeof(int32_t), s));
+                    AGWN_TRY(uw.alloc(size_t(nu) * sizeof(float), s));
+                    unsigned const gb = unsigned((nu + kB - 1) / kB);
+                    k_gather_undecided<<<dim3(gb), dim3(kB), 0, s>>>(pts.as<float>(), c.as<int32_t>(),
+                                                                     und.as<int64_t>(), nu, up.as<float>(),
+                                                                     uc.as<int32_t>());
+                    AGWN_HIP(hipGetLastError());
+                    AGWN_TRY(frac_f32_planned(src_array<float>{up.as<float>()}, nu, *pl, seg7, nseg, x0d,
+                                              uc.as<int32_t>(), uw.as<float>(), s));
+                    k_scatter_values<<<dim3(gb), dim3(kB), 0, s>>>(und.as<int64_t>(), nu, uw.as<float>(), w.as<float>());
+                    AGWN_HIP(hipGetLastError());
+                }
+            }
             else
-                AGWN_TRY(frac_f32(pts.as<float>(), m, seg7, nseg, x0d, c.as<int32_t>(), w.as<float>(), s));
+                AGWN_TRY(frac_f32_planned(src_array<float>{pts.as<float>()}, m, *pl, seg7, nseg, x0d, c.as<int32_t>(),
+                                          w.as<float>(), s));
             AGWN_TRY(route_apply(L, m, w.as<float>(), labels, dc.as<unsigned long long>(), s));
             AGWN_HIP(hipMemcpyAsync(cnt, dc.p, sizeof(cnt), hipMemcpyDeviceToHost, s));
             AGWN_HIP(hipStreamSynchronize(s));

@@ -11,17 +11,18 @@ lattice over the bounding box expanded by 10 % (134,217,728 points, generated in
              near-surface mask); each step labels the lattice with the fused inference kernel and routes to the exact
              kernels every point in the mask (voxels of edge eps/4 within eps = 0.10 % of the box diagonal of a
              triangle), every point with |pred - 0.5| <= delta, and a 1/1024 audit sample, which take their exact
-             labels. delta comes from a held-out validation set (1.5 x the worst wrong-side margin + 0.02). Training
-             happens before the timed region and is reported (train_s).
+             labels: integer term, then a far-field decision per boundary loop (second-order expansion with a
+             rigorous remainder bound; loops too close are summed exactly), and the full boundary sum only for the
+             points the decision leaves open. delta comes from a held-out validation set (1.5 x the worst wrong-side
+             margin + 0.02). Training happens before the timed region and is reported (train_s).
 The headline value is surrogate steps/s x queries; mislabels = labels that differ from the plain run's (must be 0).
 
 Reported beside it:
-  roofline      the dominant kernel of a surrogate step (the exact fractional sum over the routed points, k_frac_f32,
-                unless inference takes longer): its launch on exactly the routed points, timed with HIP events on the
-                launch stream; algorithmic FLOP per launch = routed x segments x 56 (per pair: the 53 adds and
-                multiplies of the reference's pair expression, FMA = 2, plus 2 sqrt and 1 atan2) against MI355X FP32
-                vector peak 157.3 TFLOP/s (or 492,032 FLOP per point against the 2.5 PFLOP/s BF16 MFMA peak for
-                inference)
+  roofline      the dominant kernel of a surrogate step, re-launched alone on the same inputs and timed with HIP
+                events on its stream: the fused inference (492,032 FLOP per point against the 2.5 PFLOP/s BF16 MFMA
+                peak) unless the exact stage's decision takes longer, else the full boundary sum over the routed
+                points (routed x segments x 56 FLOP: the 53 adds and multiplies of the reference's pair expression,
+                FMA = 2, plus 2 sqrt and 1 atan2, against the 157.3 TFLOP/s FP32 vector peak)
   cpu_baseline  the reference's own eval_gwnr_mesh_batch_fractional<float> (oracle/_ref, compiled from its headers)
                 with its OpenMPDispatcher on this host's cores, on a bounded seeded sample of the grid's queries,
                 scaled to queries/s; the integer term is counted as free on the CPU (no Embree here), which favours
@@ -382,11 +383,26 @@ def main() -> int:
     chk = (full_r - frac_r - cnt_r.to(torch.float32)).abs().max().item()
     if not chk < 1e-4:
         raise SystemExit(f"full != fractional + integer (max diff {chk})")
-    log(f"kernels per step: inference {infer_ms:.2f} ms, fractional {frac_ms:.2f} ms and integer {int_ms:.2f} ms "
-        f"on {len(rp)} routed points")
-    if frac_ms >= infer_ms:
+    # the step's exact stage on exactly the routed points: integer term, far-field decision, full boundary sum for the
+    # points it leaves open (the point-labelling entry with delta = +inf routes every point; it also runs inference
+    # on them, 5 % of the grid's, and its host synchronisations are inside the interval: an upper bound)
+    lab_r = torch.empty(len(rp_t), dtype=torch.int8, device=dev)
+    net.label(mesh, pts_r, x0, seg_t, float("inf"), labels=lab_r, stream=stream)
+    ev0.record(stream)
+    net.label(mesh, pts_r, x0, seg_t, float("inf"), labels=lab_r, stream=stream)
+    ev1.record(stream)
+    ev1.synchronize()
+    stage_ms = ev0.elapsed_time(ev1)
+    if not torch.equal(lab_r, exact[rp_t]):
+        raise SystemExit("exact-stage labels of the routed points differ from the plain run's")
+    decide_ms = stage_ms - int_ms  # decision + full sum of the undecided points (+ the small extras above)
+    log(f"kernels per step: inference {infer_ms:.2f} ms; on the {len(rp)} routed points: exact stage {stage_ms:.2f} ms "
+        f"(integer {int_ms:.2f} ms, decision + undecided {decide_ms:.2f} ms); the full boundary sum on all of them "
+        f"would take {frac_ms:.2f} ms")
+    if decide_ms > infer_ms:
         achieved = len(rp) * nseg * FLOP_PER_PAIR / (frac_ms * 1e-3) / 1e12
-        roof = {"bound": "valu", "kernel": "k_frac_f32_chain", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
+        roof = {"bound": "valu", "kernel": "k_frac_f32_chain (full boundary sum on the routed points)",
+                "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                 "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "kernel_ms": frac_ms,
                 "flop_per_launch": len(rp) * nseg * FLOP_PER_PAIR, "launch_points": len(rp)}
         traffic_key = "k_frac_f32_chain"
@@ -463,7 +479,9 @@ def main() -> int:
             "val_wrong": fit["val_wrong"],
             "val_points": fit["val_points"],
         },
-        "kernels_ms": {"k_mlp_infer": infer_ms, "k_frac_f32": frac_ms, "k_count": int_ms},
+        "kernels_ms": {"k_mlp_infer": infer_ms, "exact_stage_on_routed": stage_ms, "k_count_on_routed": int_ms,
+                       "decision_and_undecided_on_routed": decide_ms,
+                       "full_boundary_sum_on_routed_for_comparison": frac_ms},
         "roofline": roof,
         "build_ms": build_s * 1e3,
         "cpu_baseline": cpu,
