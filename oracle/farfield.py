@@ -1,0 +1,96 @@
+"""Far-field bound of one closed boundary loop's fractional contribution -- CPU restatement (test infrastructure only:
+imported by tests/, never by the library).
+
+No reference counterpart: this is the mathematics behind the library's far-field decision (the labelling stage in
+csrc/agwn_frac.hip), restated in float64 numpy so that the CPU tests can check it against the oracle's own fractional
+sum (Oracle.frac, which follows math/common.hh:181-189 and gwn_mesh.hh:34-46 and is pinned against the reference).
+
+A closed loop L (points P_0..P_m, P_m = P_0) spanned by the fan D from its centroid c lies in the ball (c, rho). For a
+query q outside the ball such that the line through q along x0 misses the ball, the loop's fractional term is
+    frac_L(q) = -Omega_D(q) / 4 pi,   -Omega_D(q) = a.u / d^2 + (tr M - 3 u.M.u) / d^3 + R,
+    |R| <= 3 rho^2 A / (d - rho)^4,
+with x = c - q, d = |x|, u = x / d, a = the loop's vector area (1/2) sum u_i x u_i+1 (u_i = P_i - c), A the fan's area
+and M = sum over fan triangles of (centroid - c) (x) (n dA).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def unit_loops(seg7: np.ndarray):
+    """Greedy walks over the unit-weight expansion of a boundary (negative weights reversed), in input order; returns
+    the walks as point arrays and whether every walk is closed."""
+    seg7 = np.asarray(seg7, dtype=np.float64).reshape(-1, 7)
+    units = []
+    for s in seg7:
+        w = int(round(s[6]))
+        a, b = tuple(s[0:3]), tuple(s[3:6])
+        if w < 0:
+            a, b, w = b, a, -w
+        units += [(a, b)] * w
+    out: dict = {}
+    for i, (a, _) in enumerate(units):
+        out.setdefault(a, []).append(i)
+    used = np.zeros(len(units), dtype=bool)
+    walks = []
+    for i in range(len(units)):
+        if used[i]:
+            continue
+        pts = [units[i][0]]
+        cur = i
+        while True:
+            used[cur] = True
+            e = units[cur][1]
+            pts.append(e)
+            nxt = [j for j in out.get(e, []) if not used[j]]
+            if not nxt:
+                break
+            cur = nxt[0]
+        walks.append(np.array(pts, dtype=np.float64))
+    closed = all(np.array_equal(w[0], w[-1]) for w in walks)
+    return walks, closed
+
+
+def loop_table(points: np.ndarray) -> dict:
+    """Centre, radius, vector area, fan area and first moment of one closed loop (last point = first)."""
+    p = np.asarray(points, dtype=np.float64)
+    c = p[:-1].mean(axis=0)
+    u = p[:-1] - c
+    w = p[1:] - c
+    t = 0.5 * np.cross(u, w)
+    g = (u + w) / 3.0
+    return {
+        "c": c,
+        "rho": float(np.sqrt((u * u).sum(axis=1)).max()),
+        "a": t.sum(axis=0),
+        "area": float(np.sqrt((t * t).sum(axis=1)).sum()),
+        "m": (g[:, :, None] * t[:, None, :]).sum(axis=0),
+    }
+
+
+def loop_segments(points: np.ndarray) -> np.ndarray:
+    """The loop as unit-weight weighted_segment3 rows (p0, p1, w = 1)."""
+    p = np.asarray(points, dtype=np.float64)
+    return np.concatenate([p[:-1], p[1:], np.ones((len(p) - 1, 1))], axis=1)
+
+
+def applicable(tab: dict, q: np.ndarray, x0: np.ndarray) -> np.ndarray:
+    """Queries (n, 3) for which the expansion holds: outside the ball, and the line along x0 misses it."""
+    e = np.asarray(x0, dtype=np.float64) / np.linalg.norm(x0)
+    x = tab["c"][None, :] - np.asarray(q, dtype=np.float64)
+    d = np.linalg.norm(x, axis=1)
+    line = np.linalg.norm(np.cross(x, e[None, :]), axis=1)
+    return (d > tab["rho"]) & (line > tab["rho"])
+
+
+def estimate(tab: dict, q: np.ndarray, sign: float = 1.0):
+    """(estimate of frac_L, remainder bound) at queries q (n, 3), both in units of the fractional term. `sign` exists
+    only so that tests can plant a sign error and see the bound check catch it."""
+    x = tab["c"][None, :] - np.asarray(q, dtype=np.float64)
+    d = np.linalg.norm(x, axis=1)
+    u = x / d[:, None]
+    m = tab["m"]
+    mq = np.einsum("ni,ij,nj->n", u, m, u)
+    om = sign * (u @ tab["a"]) / d ** 2 + (np.trace(m) - 3.0 * mq) / d ** 3
+    bound = 3.0 * tab["rho"] ** 2 * tab["area"] / (d - tab["rho"]) ** 4
+    return om / (4.0 * np.pi), bound / (4.0 * np.pi)

@@ -1,0 +1,89 @@
+"""CPU: the far-field bound behind the labelling decision (oracle/farfield.py), checked against the oracle's own
+fractional sum (Oracle.frac: math/common.hh:181-189, gwn_mesh.hh:34-46, pinned against the reference).
+
+Bars: for every closed loop and every query where the expansion applies, |frac_L - estimate| <= bound (measured
+worst ratio ~1e-2); a planted sign error in the dipole term breaks it; on the axis of a planar 4096-gon the oracle's
+loop sum equals the disk's closed-form solid angle / 4 pi (to 1e-5) and the estimate approaches it as 1/z^4.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from antipodal_amd import meshgen as mg
+from oracle import farfield as ff
+
+
+def worst_ratio(oracle, walks, x0, q, sign=1.0):
+    worst, n = 0.0, 0
+    for w in walks:
+        tab = ff.loop_table(w)
+        ok = ff.applicable(tab, q, x0)
+        if not ok.any():
+            continue
+        qq = q[ok]
+        exact = oracle.frac(qq, ff.loop_segments(w), x0)
+        est, b = ff.estimate(tab, qq, sign)
+        worst = max(worst, float((np.abs(exact - est) / b).max()))
+        n += len(qq)
+    return worst, n
+
+
+@pytest.fixture(scope="module")
+def torus_walks(agwn):
+    v, idx = mg.holed_torus_mesh()
+    seg = agwn.build_boundary_segments(v.astype(np.float32), idx)
+    walks, closed = ff.unit_loops(seg)
+    assert closed and len(walks) == 459
+    rng = np.random.default_rng(1)
+    sp, nrm = mg.surface_points(v, idx, 24, seed=2)
+    lo, hi = mg.expanded_bbox(v)
+    q = np.concatenate([sp + nrm * rng.normal(0, 0.01, 24)[:, None], lo + (hi - lo) * rng.random((24, 3))])
+    return walks, q
+
+
+def test_bound_holds_on_the_torus_loops(oracle, torus_walks):
+    walks, q = torus_walks
+    x0 = mg.example_x0(np.float64)
+    worst, n = worst_ratio(oracle, walks, x0, q)
+    print(f"torus: {n} (query, loop) cases, worst |frac_L - estimate| / bound = {worst:.3e}")
+    assert n > 5000
+    assert worst <= 1.0
+
+
+def test_planted_sign_error_is_caught(oracle, torus_walks):
+    walks, q = torus_walks
+    x0 = mg.example_x0(np.float64)
+    worst, _ = worst_ratio(oracle, walks, x0, q, sign=-1.0)
+    assert worst > 1.0
+
+
+def test_bound_holds_on_the_c1_cap(oracle, agwn):
+    v, idx = mg.icosphere_cap_mesh()
+    walks, closed = ff.unit_loops(agwn.build_boundary_segments(v, idx))
+    assert closed and len(walks) == 1
+    rng = np.random.default_rng(3)
+    q = rng.uniform(-4.0, 4.0, (400, 3))
+    worst, n = worst_ratio(oracle, walks, mg.example_x0(np.float64), q)
+    assert n > 100 and worst <= 1.0
+
+
+def test_disk_axis_closed_form(oracle):
+    k = 4096
+    t = 2.0 * np.pi * np.arange(k + 1) / k
+    loop = np.stack([np.cos(t), np.sin(t), np.zeros_like(t)], axis=1)
+    loop[-1] = loop[0]
+    tab = ff.loop_table(loop)
+    x0 = np.array([1.0, 0.0, 0.0])  # the line through (0, 0, z) along x misses the unit ball for z > 1
+    z = np.array([1.5, 2.0, 4.0, 8.0, 16.0])
+    q = np.stack([np.zeros_like(z), np.zeros_like(z), z], axis=1)
+    assert ff.applicable(tab, q, x0).all()
+    exact = oracle.frac(q, ff.loop_segments(loop), x0)
+    disk = 2.0 * np.pi * (1.0 - z / np.sqrt(z * z + 1.0)) / (4.0 * np.pi)
+    np.testing.assert_allclose(np.abs(exact), disk, rtol=1e-5, atol=0)
+    est, b = ff.estimate(tab, q)
+    err = np.abs(exact - est)
+    assert (err <= b).all()
+    assert np.all(np.sign(est) == np.sign(exact))
+    # the next term of the axis expansion is O(z^-4): doubling z cuts the error ~16x
+    assert err[-1] < err[-2] / 10.0 and err[-2] < err[-3] / 10.0
