@@ -948,16 +948,12 @@ int label_surrogate(mesh_t<float>& mesh, agwn_mlp const* mlp, Src const& src, in
         int64_t const m = int64_t(cnt[3]);
         if (m > 0)
         {
-            // the list was appended tile by tile in completion order: sort it back into index order, so that the
-            // exact kernels see spatially coherent runs of queries
+            // the list was appended tile by tile in completion order: order it spatially, so that the exact
+            // kernels' waves see coherent runs of queries
             dbuf tmp;
             AGWN_TRY(tmp.alloc(size_t(m) * sizeof(uint64_t), s));
-            int bits = 1;
-            while (bits < 63 && (uint64_t(2 * n) >> bits) != 0)
-                ++bits;
-            uint64_t* sorted = nullptr;
-            AGWN_TRY(radix_sort_u64(reinterpret_cast<uint64_t*>(list.p), tmp.as<uint64_t>(), m, bits, s, &sorted));
-            int64_t const* L = reinterpret_cast<int64_t const*>(sorted);
+            int64_t const* L = nullptr;
+            AGWN_TRY(route_order(mlp, src, n, list.as<int64_t>(), m, tmp.as<uint64_t>(), s, &L));
             double const x0d[3] = {double(x0f[0]), double(x0f[1]), double(x0f[2])};
             dbuf pts, c, w;
             AGWN_TRY(pts.alloc(size_t(m) * 3 * sizeof(float), s));

@@ -1261,6 +1261,88 @@ int route_apply(int64_t const* list, int64_t m, float const* w, int8_t* labels, 
     return ok;
 }
 
+// Routed-list order: spatially coherent runs make the exact kernels' waves coherent (the integer term's traversals
+// and the far-field decision's exactly summed loops). Key = 30-bit Morton code of the query quantised to 1024 cells
+// per axis of the network's box (points outside clamp to the border cells), above the entry's own bits.
+__device__ __forceinline__ uint64_t spread3_10(uint32_t v)
+{
+    uint64_t x = v & 0x3ffu;
+    x = (x | (x << 16)) & 0x030000FFull;
+    x = (x | (x << 8)) & 0x0300F00Full;
+    x = (x | (x << 4)) & 0x030C30C3ull;
+    x = (x | (x << 2)) & 0x09249249ull;
+    return x;
+}
+struct key_box
+{
+    float lo[3], scale[3]; // cell = (p - lo) * scale, clamped to [0, 1023]
+};
+template <class Src>
+__global__ void __launch_bounds__(kB) k_route_keys(Src const src, uint64_t* __restrict__ list, int64_t m, key_box K,
+                                                   int ebits)
+{
+    int64_t const k = int64_t(blockIdx.x) * kB + threadIdx.x;
+    if (k >= m)
+        return;
+    uint64_t const e = list[k];
+    float p[3];
+    query_at(src, int64_t(e >> 1), p[0], p[1], p[2]);
+    uint64_t code = 0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+    {
+        float const c = fminf(fmaxf((p[a] - K.lo[a]) * K.scale[a], 0.0f), 1023.0f); // NaN -> 0
+        code |= spread3_10(uint32_t(c)) << a;
+    }
+    list[k] = (code << ebits) | e;
+}
+__global__ void __launch_bounds__(kB) k_strip_keys(uint64_t* __restrict__ list, int64_t m, uint64_t mask)
+{
+    int64_t const k = int64_t(blockIdx.x) * kB + threadIdx.x;
+    if (k < m)
+        list[k] &= mask;
+}
+
+template <class Src>
+int route_order(agwn_mlp const* h, Src const& src, int64_t n, int64_t* list, int64_t m, uint64_t* tmp, hipStream_t s,
+                int64_t const** ordered)
+{
+    AGWN_CHECK(h != nullptr, "mlp handle is null");
+    *ordered = list;
+    if (m == 0)
+        return ok;
+    int ebits = 1;
+    while (ebits < 63 && (uint64_t(2 * n) >> ebits) != 0)
+        ++ebits;
+    uint64_t* sorted = nullptr;
+    if (ebits + 30 > 64) // index order only
+    {
+        AGWN_TRY(radix_sort_u64(reinterpret_cast<uint64_t*>(list), tmp, m, ebits, s, &sorted));
+        *ordered = reinterpret_cast<int64_t const*>(sorted);
+        return ok;
+    }
+    mlp_impl const& M = h->impl;
+    key_box K{};
+    for (int a = 0; a < 3; ++a)
+    {
+        double const ext = double(M.hi[a]) - double(M.lo[a]);
+        K.lo[a] = M.lo[a];
+        K.scale[a] = ext > 0.0 ? float(1024.0 / ext) : 0.0f;
+    }
+    unsigned const g = unsigned((m + kB - 1) / kB);
+    k_route_keys<Src><<<dim3(g), dim3(kB), 0, s>>>(src, reinterpret_cast<uint64_t*>(list), m, K, ebits);
+    AGWN_HIP(hipGetLastError());
+    AGWN_TRY(radix_sort_u64(reinterpret_cast<uint64_t*>(list), tmp, m, ebits + 30, s, &sorted));
+    k_strip_keys<<<dim3(g), dim3(kB), 0, s>>>(sorted, m, (uint64_t(1) << ebits) - 1);
+    AGWN_HIP(hipGetLastError());
+    *ordered = reinterpret_cast<int64_t const*>(sorted);
+    return ok;
+}
+template int route_order<src_array<float>>(agwn_mlp const*, src_array<float> const&, int64_t, int64_t*, int64_t,
+                                           uint64_t*, hipStream_t, int64_t const**);
+template int route_order<src_grid>(agwn_mlp const*, src_grid const&, int64_t, int64_t*, int64_t, uint64_t*,
+                                   hipStream_t, int64_t const**);
+
 template <class Src>
 int gather_routed(Src const& src, int64_t const* list, int64_t m, float* pts, hipStream_t s)
 {
