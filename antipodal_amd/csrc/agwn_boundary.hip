@@ -85,10 +85,66 @@ __global__ void __launch_bounds__(1024) k_scan1024(uint32_t* __restrict__ a, int
     }
 }
 
+// Digit-pass offsets in two parallel levels: each digit's row of per-block counts (hist[d * nblocks + b]) is scanned by
+// its own block in coalesced chunks of kB (wave shuffles, then the waves' sums), rowsum[d] = the row's total; then
+// the 256 row totals are scanned by one block. Offset of (d, b) = hist[d * nblocks + b] + rowsum[d].
+__device__ __forceinline__ uint32_t block_scan_excl(uint32_t v, uint32_t* ws, uint32_t& total)
+{
+    int const lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t x = v;
+    for (int o = 1; o < 64; o <<= 1)
+    {
+        uint32_t const y = __shfl_up(x, o);
+        if (lane >= o)
+            x += y;
+    }
+    if (lane == 63)
+        ws[wave] = x;
+    __syncthreads();
+    uint32_t base = 0;
+    total = 0;
+    for (int w = 0; w < kB / 64; ++w)
+    {
+        uint32_t const t = ws[w];
+        if (w < wave)
+            base += t;
+        total += t;
+    }
+    __syncthreads(); // ws may be reused
+    return base + x - v;
+}
+__global__ void __launch_bounds__(kB) k_scan_rows(uint32_t* __restrict__ hist, int nblocks,
+                                                  uint32_t* __restrict__ rowsum)
+{
+    __shared__ uint32_t ws[kB / 64];
+    uint32_t* row = hist + int64_t(blockIdx.x) * nblocks;
+    uint32_t carry = 0;
+    for (int c0 = 0; c0 < nblocks; c0 += kB)
+    {
+        int const i = c0 + int(threadIdx.x);
+        uint32_t const v = i < nblocks ? row[i] : 0u;
+        uint32_t total = 0;
+        uint32_t const e = block_scan_excl(v, ws, total);
+        if (i < nblocks)
+            row[i] = carry + e;
+        carry += total;
+    }
+    if (threadIdx.x == 0)
+        rowsum[blockIdx.x] = carry;
+}
+__global__ void __launch_bounds__(kB) k_scan_digits(uint32_t* __restrict__ rowsum)
+{
+    __shared__ uint32_t ws[kB / 64];
+    uint32_t total = 0;
+    uint32_t const e = block_scan_excl(rowsum[threadIdx.x], ws, total);
+    rowsum[threadIdx.x] = e;
+}
+
 // Stable scatter of one digit pass: within a tile row, lanes with equal digits are ranked by a ballot match; waves
 // of the block are ordered through per-wave digit counts in LDS.
 __global__ void __launch_bounds__(kB) k_scatter64(uint64_t const* __restrict__ kin, int64_t n, int shift,
-                                                  uint32_t const* __restrict__ offs, int nblocks,
+                                                  uint32_t const* __restrict__ offs,
+                                                  uint32_t const* __restrict__ rowbase, int nblocks,
                                                   uint64_t* __restrict__ kout)
 {
     constexpr int kWaves = kB / 64;
@@ -96,7 +152,7 @@ __global__ void __launch_bounds__(kB) k_scatter64(uint64_t const* __restrict__ k
     __shared__ uint32_t wcnt[kWaves][256];
     int const lane = threadIdx.x & 63;
     int const wave = threadIdx.x >> 6;
-    run[threadIdx.x] = offs[int64_t(threadIdx.x) * nblocks + blockIdx.x];
+    run[threadIdx.x] = offs[int64_t(threadIdx.x) * nblocks + blockIdx.x] + rowbase[threadIdx.x];
     uint64_t const lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     int64_t const base = int64_t(blockIdx.x) * kTileN;
     for (int r = 0; r < kItems; ++r)
@@ -193,11 +249,9 @@ int boundary_dev(T const* verts, int64_t nv, int32_t const* idx, int64_t nidx, T
         return ok;
     AGWN_TRY(ensure_device());
     int64_t const n = nidx; // half-edges
-    int const nblocks = int((n + kTileN - 1) / kTileN);
-    dbuf k0, k1, hist, flag, net, bad;
+    dbuf k0, k1, flag, net, bad;
     AGWN_TRY(k0.alloc(size_t(n) * sizeof(uint64_t), s));
     AGWN_TRY(k1.alloc(size_t(n) * sizeof(uint64_t), s));
-    AGWN_TRY(hist.alloc(size_t(256) * nblocks * sizeof(uint32_t), s));
     AGWN_TRY(flag.alloc(size_t(n + 1) * sizeof(uint32_t), s));
     AGWN_TRY(net.alloc(size_t(n) * sizeof(int32_t), s));
     AGWN_TRY(bad.alloc(sizeof(int), s));
@@ -209,17 +263,9 @@ int boundary_dev(T const* verts, int64_t nv, int32_t const* idx, int64_t nidx, T
     AGWN_HIP(hipMemcpyAsync(&hbad, bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
     AGWN_HIP(hipStreamSynchronize(s));
     AGWN_CHECK(hbad == 0, "a triangle index is out of range [0, %lld)", (long long)nv);
-    // the key's significant bits: 2 * 31 + 1 = 63 (indices < 2^31); all 8 digits keep the passes even
-    uint64_t* ka = k0.as<uint64_t>();
-    uint64_t* kb = k1.as<uint64_t>();
-    for (int shift = 0; shift < 64; shift += 8)
-    {
-        k_hist64<<<dim3(unsigned(nblocks)), dim3(kB), 0, s>>>(ka, n, shift, hist.as<uint32_t>(), nblocks);
-        k_scan1024<<<dim3(1), dim3(1024), 0, s>>>(hist.as<uint32_t>(), int64_t(256) * nblocks);
-        k_scatter64<<<dim3(unsigned(nblocks)), dim3(kB), 0, s>>>(ka, n, shift, hist.as<uint32_t>(), nblocks, kb);
-        AGWN_HIP(hipGetLastError());
-        std::swap(ka, kb);
-    }
+    // the key's significant bits: 2 * 31 + 1 = 63 (indices < 2^31)
+    uint64_t* ka = nullptr;
+    AGWN_TRY(radix_sort_u64(k0.as<uint64_t>(), k1.as<uint64_t>(), n, 64, s, &ka));
     // ka holds the sorted keys; flags -> exclusive positions (flag[n] = total after the scan)
     uint32_t* fl = flag.as<uint32_t>();
     k_edge_net<<<dim3(g), dim3(kB), 0, s>>>(ka, n, fl, net.as<int32_t>());
@@ -252,13 +298,16 @@ int radix_sort_u64(uint64_t* a, uint64_t* b, int64_t n, int bits, hipStream_t s,
         return ok;
     AGWN_CHECK(n < (int64_t(1) << 32), "radix sort of %lld keys", (long long)n);
     int const nblocks = int((n + kTileN - 1) / kTileN);
-    dbuf hist;
+    dbuf hist, rows;
     AGWN_TRY(hist.alloc(size_t(256) * nblocks * sizeof(uint32_t), s));
+    AGWN_TRY(rows.alloc(256 * sizeof(uint32_t), s));
     for (int shift = 0; shift < bits; shift += 8)
     {
         k_hist64<<<dim3(unsigned(nblocks)), dim3(kB), 0, s>>>(a, n, shift, hist.as<uint32_t>(), nblocks);
-        k_scan1024<<<dim3(1), dim3(1024), 0, s>>>(hist.as<uint32_t>(), int64_t(256) * nblocks);
-        k_scatter64<<<dim3(unsigned(nblocks)), dim3(kB), 0, s>>>(a, n, shift, hist.as<uint32_t>(), nblocks, b);
+        k_scan_rows<<<dim3(256), dim3(kB), 0, s>>>(hist.as<uint32_t>(), nblocks, rows.as<uint32_t>());
+        k_scan_digits<<<dim3(1), dim3(kB), 0, s>>>(rows.as<uint32_t>());
+        k_scatter64<<<dim3(unsigned(nblocks)), dim3(kB), 0, s>>>(a, n, shift, hist.as<uint32_t>(),
+                                                                 rows.as<uint32_t const>(), nblocks, b);
         AGWN_HIP(hipGetLastError());
         std::swap(a, b);
     }
