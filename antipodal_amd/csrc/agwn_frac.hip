@@ -817,10 +817,60 @@ bool chain_stream(float const* h, int64_t nseg, std::vector<float4>& stream_pts,
     return true;
 }
 
-// The stream's walks as closed loops with their far-field data (frac_plan::loops); false if some walk is open.
-bool loops_of_stream(std::vector<float4> const& p, std::vector<loop_host>& loops)
+// Far-field data of one closed loop (points p[0..n), p[n-1] = p[0]): centroid of the distinct points, radius, vector
+// area, fan area and first moment of the fan from the centroid (frac_plan::loops).
+loop_host loop_data(float4 const* p, int64_t n)
+{
+    loop_host L{};
+    int64_t const m = n - 1; // distinct points
+    for (int64_t i = 0; i < m; ++i)
+    {
+        L.c[0] += double(p[i].x);
+        L.c[1] += double(p[i].y);
+        L.c[2] += double(p[i].z);
+    }
+    for (double& v : L.c)
+        v /= double(m);
+    for (int64_t i = 0; i < m; ++i)
+    {
+        double const u[3] = {double(p[i].x) - L.c[0], double(p[i].y) - L.c[1], double(p[i].z) - L.c[2]};
+        double const w[3] = {double(p[i + 1].x) - L.c[0], double(p[i + 1].y) - L.c[1], double(p[i + 1].z) - L.c[2]};
+        double const t[3] = {0.5 * (u[1] * w[2] - u[2] * w[1]), 0.5 * (u[2] * w[0] - u[0] * w[2]),
+                             0.5 * (u[0] * w[1] - u[1] * w[0])}; // n dA of the fan triangle (c, p_i, p_i+1)
+        L.rho = std::max(L.rho, std::sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]));
+        L.area += std::sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+        for (int r = 0; r < 3; ++r)
+        {
+            L.a[r] += t[r];
+            double const g = (u[r] + w[r]) / 3.0; // triangle centroid - c
+            for (int k = 0; k < 3; ++k)
+                L.m[3 * r + k] += g * t[k];
+        }
+    }
+    return L;
+}
+
+// A walk longer than kArc segments is split into arcs of kArc segments, each closed by a chord back to its first
+// point, plus the skeleton loop through the split points, whose chords run the other way: the chords cancel, so the
+// pieces sum to the walk exactly, and every arc is compact (small radius and fan area: a small far-field bound) while
+// the skeleton has few points. Long loops would otherwise be summed exactly for most queries.
+constexpr int64_t kArc = 32;
+
+// The stream's walks as closed loops, split as above, into a decision stream of their own (x, y, z, flag: 0 starts a
+// loop) with their far-field data; false if some walk is open.
+bool loops_of_stream(std::vector<float4> const& p, std::vector<loop_host>& loops, std::vector<float4>& dec)
 {
     loops.clear();
+    dec.clear();
+    std::vector<float4> piece;
+    auto emit = [&]() {
+        loop_host L = loop_data(piece.data(), int64_t(piece.size()));
+        L.start = int64_t(dec.size());
+        L.count = int64_t(piece.size());
+        for (size_t k = 0; k < piece.size(); ++k)
+            dec.push_back(make_float4(piece[k].x, piece[k].y, piece[k].z, k == 0 ? 0.0f : 1.0f));
+        loops.push_back(L);
+    };
     int64_t const n = int64_t(p.size());
     for (int64_t s0 = 0; s0 < n;)
     {
@@ -831,37 +881,29 @@ bool loops_of_stream(std::vector<float4> const& p, std::vector<loop_host>& loops
         float4 const& b = p[size_t(e - 1)];
         if (e - s0 < 2 || a.x != b.x || a.y != b.y || a.z != b.z)
             return false;
-        loop_host L{};
-        L.start = s0;
-        L.count = e - s0;
-        int64_t const m = e - 1 - s0; // distinct points (the last repeats the first)
-        for (int64_t i = s0; i < e - 1; ++i)
+        int64_t const m = e - 1 - s0; // segments
+        if (m <= kArc)
         {
-            L.c[0] += double(p[size_t(i)].x);
-            L.c[1] += double(p[size_t(i)].y);
-            L.c[2] += double(p[size_t(i)].z);
+            piece.assign(p.begin() + s0, p.begin() + e);
+            emit();
         }
-        for (double& v : L.c)
-            v /= double(m);
-        for (int64_t i = s0; i < e - 1; ++i)
+        else
         {
-            double const u[3] = {double(p[size_t(i)].x) - L.c[0], double(p[size_t(i)].y) - L.c[1],
-                                 double(p[size_t(i)].z) - L.c[2]};
-            double const w[3] = {double(p[size_t(i + 1)].x) - L.c[0], double(p[size_t(i + 1)].y) - L.c[1],
-                                 double(p[size_t(i + 1)].z) - L.c[2]};
-            double const t[3] = {0.5 * (u[1] * w[2] - u[2] * w[1]), 0.5 * (u[2] * w[0] - u[0] * w[2]),
-                                 0.5 * (u[0] * w[1] - u[1] * w[0])}; // n dA of the fan triangle (c, P_i, P_i+1)
-            L.rho = std::max(L.rho, std::sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]));
-            L.area += std::sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
-            for (int r = 0; r < 3; ++r)
+            std::vector<int64_t> ks;
+            for (int64_t k = 0; k < m; k += kArc)
+                ks.push_back(k);
+            ks.push_back(m);
+            for (size_t j = 0; j + 1 < ks.size(); ++j)
             {
-                L.a[r] += t[r];
-                double const g = (u[r] + w[r]) / 3.0; // triangle centroid - c
-                for (int k = 0; k < 3; ++k)
-                    L.m[3 * r + k] += g * t[k];
+                piece.assign(p.begin() + s0 + ks[j], p.begin() + s0 + ks[j + 1] + 1);
+                piece.push_back(p[size_t(s0 + ks[j])]); // chord back to the arc's first point
+                emit();
             }
+            piece.clear();
+            for (int64_t k : ks)
+                piece.push_back(p[size_t(s0 + k)]);
+            emit(); // skeleton: first point, split points, first point again
         }
-        loops.push_back(L);
         s0 = e;
     }
     return true;
@@ -873,15 +915,25 @@ int frac_plan_build(float const* h, int64_t nseg, frac_plan& plan, hipStream_t s
     plan.npts = plan.nunit = 0;
     plan.loops.clear();
     plan.loops_ok = false;
+    plan.dec_npts = 0;
     std::vector<float4> stream_pts;
     int64_t nunit = 0;
     if (!chain_stream(h, nseg, stream_pts, nunit))
         return ok;
     plan.npts = int64_t(stream_pts.size());
     plan.nunit = nunit;
-    plan.loops_ok = loops_of_stream(stream_pts, plan.loops);
+    std::vector<float4> dec;
+    plan.loops_ok = loops_of_stream(stream_pts, plan.loops, dec);
     if (!plan.loops_ok)
         plan.loops.clear();
+    else
+    {
+        plan.dec_npts = int64_t(dec.size());
+        AGWN_TRY(plan.dec.alloc(std::max<size_t>(dec.size(), 1) * sizeof(float4), stream));
+        if (!dec.empty())
+            AGWN_HIP(hipMemcpyAsync(plan.dec.p, dec.data(), dec.size() * sizeof(float4), hipMemcpyHostToDevice,
+                                    stream));
+    }
     AGWN_TRY(plan.buf.alloc(std::max<size_t>(stream_pts.size(), 1) * sizeof(float4), stream));
     if (!stream_pts.empty())
         AGWN_HIP(hipMemcpyAsync(plan.buf.p, stream_pts.data(), stream_pts.size() * sizeof(float4),
@@ -903,7 +955,7 @@ int frac_decide(float const* pos, int64_t m, int32_t const* counts, frac_plan co
     if (m == 0)
         return ok;
     AGWN_CHECK(plan.chain && plan.loops_ok, "the far-field decision needs a boundary of closed unit-weight loops");
-    AGWN_CHECK(plan.loops.size() < size_t(INT32_MAX) && plan.npts < INT32_MAX, "boundary too large for the decision");
+    AGWN_CHECK(plan.loops.size() < size_t(INT32_MAX) && plan.dec_npts < INT32_MAX, "boundary too large for the decision");
     frame const R = make_frame_impl(x0[0], x0[1], x0[2]);
     pole const P = make_pole(x0[0], x0[1], x0[2]);
     std::vector<loop_rec> tab(plan.loops.size());
@@ -945,11 +997,11 @@ int frac_decide(float const* pos, int64_t m, int32_t const* counts, frac_plan co
     AGWN_TRY(dt.alloc(std::max<size_t>(tab.size(), 1) * sizeof(loop_rec), s));
     if (!tab.empty())
         AGWN_HIP(hipMemcpyAsync(dt.p, tab.data(), tab.size() * sizeof(loop_rec), hipMemcpyHostToDevice, s));
-    AGWN_TRY(p32.alloc(size_t(std::max<int64_t>(plan.npts, 1)) * 2 * sizeof(float4), s));
-    if (plan.npts > 0)
+    AGWN_TRY(p32.alloc(size_t(std::max<int64_t>(plan.dec_npts, 1)) * 2 * sizeof(float4), s));
+    if (plan.dec_npts > 0)
     {
-        k_prep_stream<<<dim3(unsigned((plan.npts + kBlock - 1) / kBlock)), dim3(kBlock), 0, s>>>(plan.pts(), plan.npts,
-                                                                                              R, p32.as<float4>());
+        k_prep_stream<<<dim3(unsigned((plan.dec_npts + kBlock - 1) / kBlock)), dim3(kBlock), 0, s>>>(
+            plan.dec.as<float4 const>(), plan.dec_npts, R, p32.as<float4>());
         AGWN_HIP(hipGetLastError());
     }
     AGWN_TRY(dn.alloc(sizeof(unsigned long long), s));

@@ -167,7 +167,8 @@ __device__ __forceinline__ void query_at(src_grid const& s, int64_t i, float& x,
 // weights are general and the per-segment kernel runs instead.
 //
 // Every walk of a boundary that is a 1-cycle (build_boundary_segments' always is: the boundary of a triangle chain)
-// ends where it starts, so the walks are closed loops. For each, in double and the caller's frame: a bounding ball
+// ends where it starts, so the walks are closed loops (long ones split into closed arcs plus a skeleton loop whose
+// chords cancel the arcs' closing chords). For each, in double and the caller's frame: a bounding ball
 // (centre c = mean of its points, radius rho), the vector area a = sum over the fan from c of (1/2) u x w, the area of
 // that fan (a spanning surface inside the ball) and its first moment M_ij = sum of (triangle centroid - c)_i (n dA)_j.
 // They bound the loop's fractional contribution at a query outside the ball (frac_decide). loops_ok is false when
@@ -179,7 +180,7 @@ struct loop_host
     double a[3];
     double area;
     double m[9];
-    int64_t start, count; // stream entries [start, start + count)
+    int64_t start, count; // decision-stream entries [start, start + count)
 };
 struct frac_plan
 {
@@ -187,8 +188,10 @@ struct frac_plan
     int64_t npts = 0;   // stream entries
     int64_t nunit = 0;  // unit segments
     bool chain = false;
-    std::vector<loop_host> loops;
+    std::vector<loop_host> loops; // closed loops (long walks split into arcs + a skeleton), in the decision stream
     bool loops_ok = false;
+    dbuf dec;                     // decision stream: the loops' points, each loop contiguous (flag 0 at its start)
+    int64_t dec_npts = 0;
     float4 const* pts() const { return buf.as<float4 const>(); }
 };
 // Builds the plan from a host copy of the boundary (stream-ordered upload on `stream`; returns when it is done).

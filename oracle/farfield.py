@@ -51,6 +51,19 @@ def unit_loops(seg7: np.ndarray):
     return walks, closed
 
 
+def split_loop(points: np.ndarray, arc: int = 32):
+    """The library's split of a long closed loop (more than `arc` segments): arcs of `arc` segments, each closed by a
+    chord back to its first point, plus the skeleton loop through the split points (its chords cancel the arcs')."""
+    p = np.asarray(points, dtype=np.float64)
+    m = len(p) - 1
+    if m <= arc:
+        return [p]
+    ks = list(range(0, m, arc)) + [m]
+    pieces = [np.concatenate([p[ks[j]: ks[j + 1] + 1], p[ks[j]: ks[j] + 1]]) for j in range(len(ks) - 1)]
+    pieces.append(p[ks])
+    return pieces
+
+
 def loop_table(points: np.ndarray) -> dict:
     """Centre, radius, vector area, fan area and first moment of one closed loop (last point = first)."""
     p = np.asarray(points, dtype=np.float64)

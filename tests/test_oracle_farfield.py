@@ -2,7 +2,8 @@
 fractional sum (Oracle.frac: math/common.hh:181-189, gwn_mesh.hh:34-46, pinned against the reference).
 
 Bars: for every closed loop and every query where the expansion applies, |frac_L - estimate| <= bound (measured
-worst ratio ~1e-2); a planted sign error in the dipole term breaks it; on the axis of a planar 4096-gon the oracle's
+worst ratio ~1e-1); the split of a long loop into chord-closed arcs plus a skeleton sums to the loop (1e-9) and each
+piece keeps the bound; a planted sign error in the dipole term breaks it; on the axis of a planar 4096-gon the oracle's
 loop sum equals the disk's closed-form solid angle / 4 pi (to 1e-5) and the estimate approaches it as 1/z^4.
 """
 from __future__ import annotations
@@ -49,6 +50,21 @@ def test_bound_holds_on_the_torus_loops(oracle, torus_walks):
     print(f"torus: {n} (query, loop) cases, worst |frac_L - estimate| / bound = {worst:.3e}")
     assert n > 5000
     assert worst <= 1.0
+
+
+def test_split_pieces_sum_to_the_loop_and_keep_the_bound(oracle, torus_walks):
+    walks, q = torus_walks
+    x0 = mg.example_x0(np.float64)
+    longw = [w for w in walks if len(w) > 200][:6]
+    assert longw
+    for w in longw:
+        pieces = ff.split_loop(w, 32)
+        assert len(pieces) > 2 and all(np.array_equal(p[0], p[-1]) for p in pieces)
+        whole = oracle.frac(q, ff.loop_segments(w), x0)
+        parts = sum(oracle.frac(q, ff.loop_segments(p), x0) for p in pieces)
+        np.testing.assert_allclose(parts, whole, rtol=0, atol=1e-9)
+    worst, n = worst_ratio(oracle, [p for w in longw for p in ff.split_loop(w, 32)], x0, q)
+    assert n > 100 and worst <= 1.0
 
 
 def test_planted_sign_error_is_caught(oracle, torus_walks):
