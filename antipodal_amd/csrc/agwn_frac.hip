@@ -943,17 +943,16 @@ int frac_plan_build(float const* h, int64_t nseg, frac_plan& plan, hipStream_t s
     return ok;
 }
 
-// loops whose remainder bound is above this (in units of the fractional term) are summed exactly; the decision
-// margin covers the FP32 evaluation's own error (measured <= 4e-6 on 430k-segment boundaries)
-constexpr float kDecideBeta = 0.002f;
+// the decision margin covers the FP32 evaluation's own error (measured <= 4e-6 on 430k-segment boundaries)
 constexpr double kDecideMargin = 1e-4;
 
 int frac_decide(float const* pos, int64_t m, int32_t const* counts, frac_plan const& plan, double const x0[3],
-                float* val, int64_t* undecided, int64_t& nund, hipStream_t s)
+                float beta, float* val, int64_t* undecided, int64_t& nund, hipStream_t s)
 {
     nund = 0;
     if (m == 0)
         return ok;
+    AGWN_CHECK(beta > 0.0f, "beta must be positive");
     AGWN_CHECK(plan.chain && plan.loops_ok, "the far-field decision needs a boundary of closed unit-weight loops");
     AGWN_CHECK(plan.loops.size() < size_t(INT32_MAX) && plan.dec_npts < INT32_MAX, "boundary too large for the decision");
     frame const R = make_frame_impl(x0[0], x0[1], x0[2]);
@@ -1009,7 +1008,7 @@ int frac_decide(float const* pos, int64_t m, int32_t const* counts, frac_plan co
     int64_t const blocks = (m + kDecideBlock - 1) / kDecideBlock;
     AGWN_CHECK(blocks < (int64_t(1) << 31), "frac_decide: too many queries (%lld)", (long long)m);
     k_frac_decide<<<dim3(unsigned(blocks)), dim3(kDecideBlock), 0, s>>>(
-        pos, m, counts, dt.as<loop_rec>(), int(tab.size()), p32.as<float4>(), R, P, kTau, kDecideBeta * kFourPiF,
+        pos, m, counts, dt.as<loop_rec>(), int(tab.size()), p32.as<float4>(), R, P, kTau, beta * kFourPiF,
         kDecideMargin, val, undecided, dn.as<unsigned long long>());
     AGWN_HIP(hipGetLastError());
     unsigned long long h = 0;

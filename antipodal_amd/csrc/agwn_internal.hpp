@@ -203,8 +203,12 @@ int frac_plan_build(float const* seg7_host, int64_t nseg, frac_plan& plan, hipSt
 // arithmetic, FP64 for ill-conditioned pairs). Where count + estimate is farther from 0.5 than the bound plus a margin
 // that covers the FP32 evaluation's own error, val[i] = that estimate (on the same side of 0.5 as the full GWN);
 // otherwise i is appended to `undecided` (device, m entries) and val[i] is left unwritten. nund: how many (host).
+// beta: loops whose remainder bound (in units of the fractional term) exceeds it are summed exactly -- smaller is
+// tighter and slower; labelling runs a first pass at kDecideBeta and a second at kDecideBetaTight over what it leaves.
+constexpr float kDecideBeta = 0.002f;
+constexpr float kDecideBetaTight = 0.0002f;
 int frac_decide(float const* pos, int64_t m, int32_t const* counts, frac_plan const& plan, double const x0[3],
-                float* val, int64_t* undecided, int64_t& nund, hipStream_t s);
+                float beta, float* val, int64_t* undecided, int64_t& nund, hipStream_t s);
 
 // Fractional launchers (agwn_frac.hip). seg7/out/counts are device pointers. counts may be null (fractional
 // only); otherwise out[i] = fractional + counts[i] (the full GWN of gwn_mesh.hh:213-218).

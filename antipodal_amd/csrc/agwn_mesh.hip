@@ -919,6 +919,38 @@ __global__ void __launch_bounds__(kB) k_scatter_values(int64_t const* __restrict
         out[list[k]] = v[k];
 }
 
+// Values of m routed points (positions pts, integer terms cnt) on the correct side of 0.5: decision at the given
+// level (0: kDecideBeta, 1: kDecideBetaTight); the points it leaves open go to the next level, after the last to the
+// full boundary sum.
+int label_exact_levels(float const* pts, int32_t const* cnt, int64_t m, frac_plan const& pl, float const* seg7,
+                       int64_t nseg, double const x0d[3], int level, float* w, hipStream_t s)
+{
+    if (m == 0)
+        return ok;
+    if (level == 2)
+        return frac_f32_planned(src_array<float>{pts}, m, pl, seg7, nseg, x0d, cnt, w, s);
+    dbuf und;
+    AGWN_TRY(und.alloc(size_t(m) * sizeof(int64_t), s));
+    int64_t nu = 0;
+    AGWN_TRY(frac_decide(pts, m, cnt, pl, x0d, level == 0 ? kDecideBeta : kDecideBetaTight, w, und.as<int64_t>(), nu,
+                         s));
+    if (nu == 0)
+        return ok;
+    dbuf up, uc, uw;
+    AGWN_TRY(up.alloc(size_t(nu) * 3 * sizeof(float), s));
+    AGWN_TRY(uc.alloc(size_t(nu) * sizeof(int32_t), s));
+    AGWN_TRY(uw.alloc(size_t(nu) * sizeof(float), s));
+    unsigned const gb = unsigned((nu + kB - 1) / kB);
+    k_gather_undecided<<<dim3(gb), dim3(kB), 0, s>>>(pts, cnt, und.as<int64_t>(), nu, up.as<float>(),
+                                                     uc.as<int32_t>());
+    AGWN_HIP(hipGetLastError());
+    AGWN_TRY(label_exact_levels(up.as<float>(), uc.as<int32_t>(), nu, pl, seg7, nseg, x0d, level + 1, uw.as<float>(),
+                                s));
+    k_scatter_values<<<dim3(gb), dim3(kB), 0, s>>>(und.as<int64_t>(), nu, uw.as<float>(), w);
+    AGWN_HIP(hipGetLastError());
+    return ok;
+}
+
 // Surrogate labelling with the exact FP32 path as fallback (see mlp_route): network labels for every query, then the
 // mask, band and audit points are gathered and labelled exactly: their integer terms (the same kernel, x0 and query
 // coordinates as the plain evaluation), then the far-field decision (frac_decide) where the boundary is made of
@@ -976,28 +1008,11 @@ int label_surrogate(mesh_t<float>& mesh, agwn_mlp const* mlp, Src const& src, in
             }
             if (pl->chain && pl->loops_ok)
             {
-                // far-field decision first; only the points it cannot decide take the full boundary sum
-                dbuf und;
-                AGWN_TRY(und.alloc(size_t(m) * sizeof(int64_t), s));
-                int64_t nu = 0;
-                AGWN_TRY(frac_decide(pts.as<float>(), m, c.as<int32_t>(), *pl, x0d, w.as<float>(), und.as<int64_t>(),
-                                     nu, s));
-                if (nu > 0)
-                {
-                    dbuf up, uc, uw;
-                    AGWN_TRY(up.alloc(size_t(nu) * 3 * sizeof(float), s));
-                    AGWN_TRY(uc.alloc(size_t(nu) * sizeof(int32_t), s));
-                    AGWN_TRY(uw.alloc(size_t(nu) * sizeof(float), s));
-                    unsigned const gb = unsigned((nu + kB - 1) / kB);
-                    k_gather_undecided<<<dim3(gb), dim3(kB), 0, s>>>(pts.as<float>(), c.as<int32_t>(),
-                                                                     und.as<int64_t>(), nu, up.as<float>(),
-                                                                     uc.as<int32_t>());
-                    AGWN_HIP(hipGetLastError());
-                    AGWN_TRY(frac_f32_planned(src_array<float>{up.as<float>()}, nu, *pl, seg7, nseg, x0d,
-                                              uc.as<int32_t>(), uw.as<float>(), s));
-                    k_scatter_values<<<dim3(gb), dim3(kB), 0, s>>>(und.as<int64_t>(), nu, uw.as<float>(), w.as<float>());
-                    AGWN_HIP(hipGetLastError());
-                }
+                // far-field decision, then a tighter one over what it leaves; only the points neither decides take
+                // the full boundary sum. Each level gathers its open points (positions, counts) and scatters the
+                // values back.
+                AGWN_TRY(label_exact_levels(pts.as<float>(), c.as<int32_t>(), m, *pl, seg7, nseg, x0d, 0,
+                                            w.as<float>(), s));
             }
             else
                 AGWN_TRY(frac_f32_planned(src_array<float>{pts.as<float>()}, m, *pl, seg7, nseg, x0d, c.as<int32_t>(),

@@ -376,7 +376,19 @@ struct wcopies
 {
     __bf16* wr[kL];
     __bf16* wt[kL];
+    __bf16* wf[kL];
 };
+
+// Inference copy of a hidden layer's weights in MFMA fragment order: the A operand of unit block ub (32 units) and k
+// step st (16 inputs) is 64 lanes x 8 BF16 contiguous in lane order (lane L: unit 32 ub + L % 32, inputs
+// 16 st + 8 (L / 32) + [0, 8)), so each wave's weight load is one coalesced 1 KiB read. (From Wt rows, a lane's 16 B
+// came from a different 512 B row each: 32 cache lines per load, a quarter of each used -- the inference kernel's
+// bound, ahead of the MFMAs.)
+__host__ __device__ __forceinline__ int64_t frag_index(int64_t k, int64_t n, int64_t K)
+{
+    int64_t const lane = (n % 32) + 32 * ((k % 16) / 8);
+    return (((n / 32) * (K / 16) + k / 16) * 64 + lane) * 8 + k % 8;
+}
 
 __device__ __forceinline__ void store_bf16_copies(wcopies const& C, int64_t i, float v)
 {
@@ -391,6 +403,7 @@ __device__ __forceinline__ void store_bf16_copies(wcopies const& C, int64_t i, f
             __bf16 const b = __bf16(v);
             C.wr[l][k * kH + n] = b;
             C.wt[l][n * K + k] = b;
+            C.wf[l][frag_index(k, n, K)] = b;
         }
     }
 }
@@ -435,7 +448,8 @@ __global__ void __launch_bounds__(kB) k_perm(int64_t* __restrict__ perm, int64_t
 // ---------------------------------------------------------------------------------------------------------------------
 // Fused inference: a workgroup takes 128 points through the features, the 4 hidden layers and the output layer with
 // the activations in one LDS buffer (128 x 264 BF16, 66 KiB: two workgroups per CU). Each layer computes
-// out^T[n][m] = sum_k Wt[n][k] act[m][k]: wave w owns units [64w, 64w + 64) (A operand = Wt rows, from L2) for all 128
+// out^T[n][m] = sum_k Wt[n][k] act[m][k]: wave w owns units [64w, 64w + 64) (A operand = Wt rows in fragment order,
+// from L2) for all 128
 // points (B operand = LDS rows), so a lane's 4 consecutive accumulators are 4 consecutive units of one point and go
 // back to LDS as one 8-byte write after a barrier. The output layer is a 256-long dot per point.
 // Optionally labels the points (pred > 0.5) and appends to a routing list (entry = 2 * index + (audit ? 1 : 0)) the
@@ -447,7 +461,7 @@ constexpr int kLdsStride = kH + 8; // BF16 per LDS row: 528 B rows keep the 16-l
 
 struct infer_params
 {
-    __bf16 const* wt[kL];
+    __bf16 const* wf[kL]; // fragment order (frag_index)
     float const* b[kL];
     float const* wo;
     float const* bo;
@@ -601,16 +615,17 @@ __global__ void __launch_bounds__(256, 2) k_mlp_infer(Src const src, int64_t n, 
     for (int l = 0; l < kL; ++l)
     {
         int const K = l == 0 ? kIn : kH;
-        __bf16 const* wa0 = P.wt[l] + int64_t(n0 + li) * K + kh;
-        __bf16 const* wa1 = wa0 + int64_t(32) * K;
+        // fragments of unit blocks 2w and 2w + 1 (units n0 .. n0 + 63); k step st at + 512 st
+        __bf16 const* wa0 = P.wf[l] + (int64_t(n0 / 32) * (K / 16) * 64 + lane) * 8;
+        __bf16 const* wa1 = wa0 + int64_t(K / 16) * 512;
         f32x16 acc[2][4] = {};
         bf16x8 a0 = *reinterpret_cast<bf16x8 const*>(wa0);
         bf16x8 a1 = *reinterpret_cast<bf16x8 const*>(wa1);
         for (int k0 = 0; k0 < K; k0 += 16)
         {
             int const kn = k0 + 16 < K ? k0 + 16 : k0;
-            bf16x8 const na0 = *reinterpret_cast<bf16x8 const*>(wa0 + kn);
-            bf16x8 const na1 = *reinterpret_cast<bf16x8 const*>(wa1 + kn);
+            bf16x8 const na0 = *reinterpret_cast<bf16x8 const*>(wa0 + int64_t(kn) * 32);
+            bf16x8 const na1 = *reinterpret_cast<bf16x8 const*>(wa1 + int64_t(kn) * 32);
 #pragma unroll
             for (int j = 0; j < 4; ++j)
             {
@@ -754,6 +769,7 @@ struct mlp_impl
     float* g = nullptr; // gradient
     __bf16* wr[kL] = {};
     __bf16* wt[kL] = {};
+    __bf16* wf[kL] = {};
     uint32_t* mask_bits = nullptr;
     surface_mask mask{};
     float mask_eps = 0.0f;
@@ -775,6 +791,8 @@ struct mlp_impl
                 (void)hipFree(wr[l]);
             if (wt[l])
                 (void)hipFree(wt[l]);
+            if (wf[l])
+                (void)hipFree(wf[l]);
         }
     }
 
@@ -792,6 +810,7 @@ struct mlp_impl
         {
             AGWN_HIP(hipMalloc(&wr[l], size_t(kInDim[l] * kH) * sizeof(__bf16)));
             AGWN_HIP(hipMalloc(&wt[l], size_t(kInDim[l] * kH) * sizeof(__bf16)));
+            AGWN_HIP(hipMalloc(&wf[l], size_t(kInDim[l] * kH) * sizeof(__bf16)));
         }
         return ok;
     }
@@ -803,6 +822,7 @@ struct mlp_impl
         {
             C.wr[l] = wr[l];
             C.wt[l] = wt[l];
+            C.wf[l] = wf[l];
         }
         return C;
     }
@@ -819,7 +839,7 @@ struct mlp_impl
         infer_params P{};
         for (int l = 0; l < kL; ++l)
         {
-            P.wt[l] = wt[l];
+            P.wf[l] = wf[l];
             P.b[l] = w + kLay.b[l];
         }
         P.wo = w + kLay.wo;
