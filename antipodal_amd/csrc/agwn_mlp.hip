@@ -85,13 +85,17 @@ __device__ __forceinline__ float silu_grad(float z)
     return s * (1.0f + z * (1.0f - s));
 }
 
-// feature pair j (0..95) of a point: axis a = j / 32, frequency k = j % 32
+// feature pair j (0..95) of a point: axis a = j / 32, frequency k = j % 32: (sin, cos)(pi w_k u), u the box-normalised
+// coordinate. The hardware sine and cosine take revolutions (sin(2 pi t)), so t = w_k u / 2; their error (~1e-6 over
+// |t| <= 256, here |t| <= F / 2 for points in the box) is far below the BF16 rounding the features go through, and a
+// full-precision sincospi cost as much as a quarter of the fused inference kernel. Training and inference share it.
 __device__ __forceinline__ void feature_pair(feat_params const& F, float x, float y, float z, int j, float& s, float& c)
 {
     int const a = j / kFreq, k = j % kFreq;
     float const p = a == 0 ? x : (a == 1 ? y : z);
-    float const u = (p - F.c[a]) * F.inv_h[a];
-    sincospif(F.w[k] * u, &s, &c);
+    float const t = 0.5f * (F.w[k] * ((p - F.c[a]) * F.inv_h[a]));
+    s = __builtin_amdgcn_sinf(t);
+    c = __builtin_amdgcn_cosf(t);
 }
 
 // Training features: row m of X [rows x 192] and column m of XT [192 x rows] for sample perm[m] (m < nvalid; rows past
@@ -592,19 +596,30 @@ __global__ void __launch_bounds__(256, 2) k_mlp_infer(Src const src, int64_t n, 
     int const wave = threadIdx.x >> 6;
     int64_t const base = int64_t(blockIdx.x) * kTileP;
 
-    for (int t = threadIdx.x; t < kTileP * (kIn / 2); t += 256)
     {
-        int const p = t % kTileP, j = t / kTileP;
-        float s = 0.0f, c = 0.0f;
+        // features: two lanes per point (lanes of a wave = consecutive points), each 48 pairs written 4 pairs (16 B)
+        // at a time -- the rows' 528 B stride keeps each 16-lane group of those stores on distinct banks
+        int const p = threadIdx.x % kTileP, half = threadIdx.x / kTileP;
         int64_t const i = base + p;
-        if (i < n)
-        {
-            float x, y, z;
+        float x = 0.0f, y = 0.0f, z = 0.0f;
+        bool const live = i < n;
+        if (live)
             query_at(src, i, x, y, z);
-            feature_pair(F, x, y, z, j, s, c);
+        for (int g = 0; g < kIn / 16; ++g)
+        {
+            int const j0 = half * (kIn / 4) + 4 * g;
+            bf16x8 v;
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+            {
+                float s = 0.0f, c = 0.0f;
+                if (live)
+                    feature_pair(F, x, y, z, j0 + q, s, c);
+                v[2 * q] = __bf16(s);
+                v[2 * q + 1] = __bf16(c);
+            }
+            *reinterpret_cast<bf16x8*>(act + p * kLdsStride + 2 * j0) = v;
         }
-        act[p * kLdsStride + 2 * j] = __bf16(s);
-        act[p * kLdsStride + 2 * j + 1] = __bf16(c);
     }
     __syncthreads();
 
@@ -657,21 +672,29 @@ __global__ void __launch_bounds__(256, 2) k_mlp_infer(Src const src, int64_t n, 
             }
         __syncthreads();
     }
-    // output layer, summed in the order of the training kernel (4 products per lane, then a butterfly)
-    float w[4];
-    for (int j = 0; j < 4; ++j)
-        w[j] = P.wo[4 * lane + j];
-    float const bo = *P.bo;
-    for (int p = wave; p < kTileP; p += 4)
+    // output layer: two lanes per point (lanes = consecutive points: 16 B LDS reads conflict-free per 16-lane group),
+    // each a 128-unit dot product in FP32 (the weights are uniform: scalar loads), halves added through LDS
     {
-        bf16x4 const hv = *reinterpret_cast<bf16x4 const*>(act + p * kLdsStride + 4 * lane);
-        float s = 0.0f;
-        for (int j = 0; j < 4; ++j)
-            s += float(hv[j]) * w[j];
-        for (int off = 32; off > 0; off >>= 1)
-            s += __shfl_xor(s, off);
-        if (lane == 0)
-            predl[p] = s + bo;
+        int const p = threadIdx.x % kTileP, half = threadIdx.x / kTileP;
+        float const* wo = P.wo + half * (kH / 2);
+        __bf16 const* row = act + p * kLdsStride + half * (kH / 2);
+        float s0 = 0.0f, s1 = 0.0f;
+#pragma unroll
+        for (int g = 0; g < kH / 16; ++g)
+        {
+            bf16x8 const hv = *reinterpret_cast<bf16x8 const*>(row + 8 * g);
+#pragma unroll
+            for (int q = 0; q < 8; q += 2)
+            {
+                s0 = fmaf(float(hv[q]), wo[8 * g + q], s0);
+                s1 = fmaf(float(hv[q + 1]), wo[8 * g + q + 1], s1);
+            }
+        }
+        if (half == 1)
+            predl[p] = s0 + s1;
+        __syncthreads();
+        if (half == 0)
+            predl[p] += (s0 + s1) + *P.bo;
     }
     __syncthreads();
     if (threadIdx.x >= kTileP)

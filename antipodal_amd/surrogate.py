@@ -191,7 +191,7 @@ def training_samples(verts, idx, lo, hi, n: int, near_frac: float = 0.7, band: f
     return pts[rng.permutation(n)]
 
 
-def calibrate_delta(pred: np.ndarray, exact: np.ndarray, factor: float = 1.5, margin: float = 0.02) -> float:
+def calibrate_delta(pred: np.ndarray, exact: np.ndarray, factor: float = 2.0, margin: float = 0.02) -> float:
     """Band half-width from validation predictions: factor x the largest |pred - 0.5| of a wrongly labelled
     validation point, plus margin."""
     wrong = (pred > 0.5) != (exact > 0.5)
@@ -199,10 +199,10 @@ def calibrate_delta(pred: np.ndarray, exact: np.ndarray, factor: float = 1.5, ma
     return factor * worst + margin
 
 
-def fit_surrogate(verts, idx, boundary, x0, lo, hi, n_train: int = 1 << 24, n_val: int = 1 << 20,
+def fit_surrogate(verts, idx, boundary, x0, lo, hi, n_train: int = 1 << 24, n_val: int = 1 << 22,
                   epochs: int = 8, batch: int = 16384, lr: float = 2e-3, seed: int = 0, mesh: HipMesh | None = None,
                   max_freq: float = 64.0, eps: float | None = None, voxel: float = 0.0, jitter: float = 0.0,
-                  target_clip=(-0.25, 1.25), factor: float = 1.5, margin: float = 0.02, power: float = 1.0,
+                  target_clip=(-0.25, 1.25), factor: float = 2.0, margin: float = 0.02, power: float = 1.0,
                   log=None):
     """Trains a GwnSurrogate for the mesh on exact FP32 GWN values computed by the library, and picks the routing band.
 

@@ -13,8 +13,8 @@ lattice over the bounding box expanded by 10 % (134,217,728 points, generated in
              triangle), every point with |pred - 0.5| <= delta, and a 1/1024 audit sample, which take their exact
              labels: integer term, then a far-field decision per boundary loop (second-order expansion with a
              rigorous remainder bound; loops too close are summed exactly), and the full boundary sum only for the
-             points the decision leaves open. delta comes from a held-out validation set (1.5 x the worst wrong-side
-             margin + 0.02). Training happens before the timed region and is reported (train_s).
+             points the decision leaves open. delta comes from 2^23 held-out points (2 x the worst wrong-side margin +
+             0.02). Training happens before the timed region and is reported (train_s).
 The headline value is surrogate steps/s x queries; mislabels = labels that differ from the plain run's (must be 0).
 
 Reported beside it:
@@ -220,6 +220,8 @@ def main() -> int:
     ap.add_argument("--plain-steps", type=int, default=1)
     ap.add_argument("--train-samples", type=int, default=1 << 24)
     ap.add_argument("--epochs", type=int, default=16)
+    ap.add_argument("--val-samples", type=int, default=1 << 23,
+                    help="held-out points the routing band is calibrated on (the band must hold on every grid point)")
     ap.add_argument("--max-freq", type=float, default=64.0)
     ap.add_argument("--eps-frac", type=float, default=0.0010,
                     help="near-surface mask radius as a fraction of the box diagonal")
@@ -304,7 +306,7 @@ def main() -> int:
         log(f"surrogate loaded from {cache}")
     else:
         net, delta, fit = agwn.fit_surrogate(v32, idx, seg32, x0, lo.astype(np.float32), hi.astype(np.float32),
-                                             n_train=args.train_samples, n_val=1 << 20, epochs=args.epochs,
+                                             n_train=args.train_samples, n_val=args.val_samples, epochs=args.epochs,
                                              batch=args.batch, lr=args.lr, seed=0, mesh=mesh,
                                              max_freq=args.max_freq,
                                              eps=args.eps_frac * float(np.linalg.norm(hi - lo)),
