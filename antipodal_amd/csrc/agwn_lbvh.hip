@@ -205,7 +205,7 @@ __device__ __forceinline__ int delta(uint32_t const* __restrict__ k, int64_t n, 
 }
 
 __global__ void __launch_bounds__(kB) k_karras(uint32_t const* __restrict__ k, int64_t n, int32_t* __restrict__ child,
-                                               int32_t* __restrict__ parent)
+                                               int32_t* __restrict__ parent, int8_t* __restrict__ big)
 {
     int64_t const i = int64_t(blockIdx.x) * kB + threadIdx.x;
     if (i >= n - 1)
@@ -235,6 +235,7 @@ __global__ void __launch_bounds__(kB) k_karras(uint32_t const* __restrict__ k, i
     int32_t const right = (hi == g + 1) ? ~int32_t(g + 1) : int32_t(g + 1);
     child[2 * i] = left;
     child[2 * i + 1] = right;
+    big[i] = (hi - g) > (g - lo + 1) ? 1 : 0; // leaves right: [g+1, hi], left: [lo, g]
     parent[left >= 0 ? left : (n - 1) + ~left] = int32_t(i);
     parent[right >= 0 ? right : (n - 1) + ~right] = int32_t(i);
 }
@@ -266,8 +267,9 @@ __device__ __forceinline__ void ld_box(B const* lo, B const* hi, int64_t i, B (&
 
 template <class B>
 __global__ void __launch_bounds__(kB)
-    k_refit(int32_t const* __restrict__ child, int32_t const* __restrict__ parent, int64_t n, B const* leaf_lo,
-            B const* leaf_hi, B* node_lo, B* node_hi, bvh_node<B>* __restrict__ nodes, int32_t* flags)
+    k_refit(int32_t const* __restrict__ child, int32_t const* __restrict__ parent, int8_t const* __restrict__ big,
+            int64_t n, B const* leaf_lo, B const* leaf_hi, B* node_lo, B* node_hi, bvh_node<B>* __restrict__ nodes,
+            int32_t* flags)
 {
     int64_t const k = int64_t(blockIdx.x) * kB + threadIdx.x;
     if (k >= n)
@@ -302,6 +304,7 @@ __global__ void __launch_bounds__(kB)
         }
         nd.c0 = c[0];
         nd.c1 = c[1];
+        nd.big = big[p];
         nodes[p] = nd;
         p = parent[p];
     }
@@ -314,10 +317,12 @@ int lbvh_build_topology(float const* centroids, int64_t n, lbvh_topology& topo, 
     topo.n = n;
     if (n <= 0)
         return ok;
-    AGWN_CHECK(n < (int64_t(1) << 31) - 1, "too many triangles for the LBVH (%lld)", (long long)n);
+    AGWN_CHECK(n < (int64_t(1) << 27), "too many triangles for the LBVH (%lld; the traversal stack holds 2^27)",
+               (long long)n);
     AGWN_HIP(hipMalloc(&topo.order, size_t(n) * sizeof(int32_t)));
     AGWN_HIP(hipMalloc(&topo.parent, size_t(2 * n) * sizeof(int32_t)));
     AGWN_HIP(hipMalloc(&topo.child, size_t(std::max<int64_t>(2 * (n - 1), 1)) * sizeof(int32_t)));
+    AGWN_HIP(hipMalloc(&topo.big, size_t(std::max<int64_t>(n - 1, 1)) * sizeof(int8_t)));
     if (n == 1)
     {
         int32_t const zero = 0, none = -1;
@@ -359,7 +364,7 @@ int lbvh_build_topology(float const* centroids, int64_t n, lbvh_topology& topo, 
     }
     // after 4 passes the sorted data is back in (k0, order)
     AGWN_HIP(hipMemsetAsync(topo.parent, 0xff, size_t(2 * n) * sizeof(int32_t), s));
-    k_karras<<<dim3(unsigned((n - 1 + kB - 1) / kB)), dim3(kB), 0, s>>>(ka, n, topo.child, topo.parent);
+    k_karras<<<dim3(unsigned((n - 1 + kB - 1) / kB)), dim3(kB), 0, s>>>(ka, n, topo.child, topo.parent, topo.big);
     AGWN_HIP(hipGetLastError());
     AGWN_HIP(hipMemsetAsync(md.p, 0, sizeof(int), s));
     k_depth<<<dim3(g), dim3(kB), 0, s>>>(topo.parent, n, md.as<int>());
@@ -368,11 +373,6 @@ int lbvh_build_topology(float const* centroids, int64_t n, lbvh_topology& topo, 
     AGWN_HIP(hipMemcpyAsync(&depth, md.p, sizeof(int), hipMemcpyDeviceToHost, s));
     AGWN_HIP(hipStreamSynchronize(s));
     topo.max_depth = depth;
-    if (depth + 2 > kMaxStack)
-    {
-        set_error("LBVH depth %d exceeds the traversal stack (%d)", depth, kMaxStack);
-        return err_unsupported;
-    }
     return ok;
 }
 
@@ -384,6 +384,8 @@ void lbvh_release(lbvh_topology& topo)
         (void)hipFree(topo.child);
     if (topo.parent)
         (void)hipFree(topo.parent);
+    if (topo.big)
+        (void)hipFree(topo.big);
     topo = lbvh_topology{};
 }
 
@@ -395,8 +397,8 @@ int lbvh_refit(lbvh_topology const& topo, B const* leaf_lo, B const* leaf_hi, bv
     if (n <= 1)
         return ok;
     AGWN_HIP(hipMemsetAsync(flags, 0, size_t(n - 1) * sizeof(int32_t), s));
-    k_refit<B><<<dim3(unsigned((n + kB - 1) / kB)), dim3(kB), 0, s>>>(topo.child, topo.parent, n, leaf_lo, leaf_hi,
-                                                                     node_lo, node_hi, nodes, flags);
+    k_refit<B><<<dim3(unsigned((n + kB - 1) / kB)), dim3(kB), 0, s>>>(topo.child, topo.parent, topo.big, n, leaf_lo,
+                                                                     leaf_hi, node_lo, node_hi, nodes, flags);
     AGWN_HIP(hipGetLastError());
     return ok;
 }

@@ -19,6 +19,7 @@ struct lbvh_topology
     int32_t* order = nullptr;   // [n] sorted position -> original triangle index
     int32_t* child = nullptr;   // [2*(n-1)] children of internal nodes
     int32_t* parent = nullptr;  // [n-1 + n]: parent of internal i at [i], of leaf k at [n-1+k]; root's parent -1
+    int8_t* big = nullptr;      // [n-1]: 1 when internal i's right subtree has more leaves than its left
     int max_depth = 0;          // edges from the root to the deepest leaf
 };
 
@@ -26,12 +27,16 @@ struct lbvh_topology
 int lbvh_build_topology(float const* centroids, int64_t n, lbvh_topology& topo, hipStream_t s);
 void lbvh_release(lbvh_topology& topo);
 
-// Traversal node: the two child boxes and child indices. B = float or double box coordinates.
+// Traversal node: the two child boxes and child indices, and which child's subtree has more leaves (big = 1: c1).
+// Traversals that hit both internal children descend into the smaller subtree and push the larger one: every stack
+// entry then marks a step into a subtree of at most half its parent's leaves, so a traversal never holds more than
+// log2(n) entries (kMaxStack). B = float or double box coordinates.
 template <class B>
 struct alignas(sizeof(B) == 4 ? 64 : 128) bvh_node
 {
     B lo0[3], hi0[3], lo1[3], hi1[3];
     int32_t c0, c1;
+    int32_t big;
 };
 
 // Bottom-up refit. leaf_lo/leaf_hi: [n][3] per sorted leaf (an empty box has lo > hi). Writes nodes[n-1].
@@ -40,5 +45,7 @@ template <class B>
 int lbvh_refit(lbvh_topology const& topo, B const* leaf_lo, B const* leaf_hi, bvh_node<B>* nodes, B* node_lo,
                B* node_hi, int32_t* scratch_flags, hipStream_t s);
 
-constexpr int kMaxStack = 64;
+// traversal stack entries (LDS, per lane): > log2 of the largest supported leaf count (lbvh_build_topology checks
+// n < 2^27); 28 x 4 B x 128 lanes = 14 KiB per traversal block
+constexpr int kMaxStack = 28;
 } // namespace agwn

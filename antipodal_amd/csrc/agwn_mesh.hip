@@ -272,8 +272,12 @@ __device__ __forceinline__ int count_one(T px, T py, T pz, T dx, T dy, T dz, bvh
                 }
                 else if (have)
                 {
-                    if (sp < kMaxStack) // depth <= kMaxStack - 2 is checked at build
-                        stack[sp++][lane] = nd.c1;
+                    // both children internal: descend into the one with fewer leaves, push the other (the stack
+                    // then never exceeds log2 of the leaf count, < kMaxStack)
+                    int32_t const push = nd.big ? nd.c1 : nd.c0;
+                    next = nd.big ? nd.c0 : nd.c1;
+                    if (sp < kMaxStack)
+                        stack[sp++][lane] = push;
                 }
                 else
                 {

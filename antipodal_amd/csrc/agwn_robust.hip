@@ -324,8 +324,11 @@ __global__ void __launch_bounds__(kTB)
                     sum += leaf_count(qi, tri + 3 * int64_t(~nd.c1));
                 else if (have)
                 {
+                    // both children internal: the smaller subtree first, the larger one on the stack (<= log2 n)
+                    int32_t const push = nd.big ? nd.c1 : nd.c0;
+                    next = nd.big ? nd.c0 : nd.c1;
                     if (sp < kMaxStack)
-                        stack[sp++][threadIdx.x] = nd.c1;
+                        stack[sp++][threadIdx.x] = push;
                 }
                 else
                 {
