@@ -1408,7 +1408,8 @@ int batch_launches(agwn_batch& B, float const* pos, float const x0f[3], int vari
     AGWN_HIP(hipGetLastError());
     if (B.ncitems > 0)
     {
-        int const blocks = int(std::min<int64_t>(B.ncitems, int64_t(B.ncu) * 4));
+        // 10 traversal blocks per CU: the 14 KiB LDS stacks allow 11; 5 waves per SIMD hide the traversal's loads
+        int const blocks = int(std::min<int64_t>(B.ncitems, int64_t(B.ncu) * 10));
         k_count_batch<<<dim3(unsigned(blocks)), dim3(kTB), 0, s>>>(
             pos, B.cdesc.as<batch_mesh_desc>(), B.citem_m.as<int32_t>(), B.citem_q.as<int64_t>(), B.ncitems,
             B.counters.as<unsigned long long>(), x0f[0], x0f[1], x0f[2], B.counts.as<int32_t>());
