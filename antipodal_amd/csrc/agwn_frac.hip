@@ -352,7 +352,8 @@ __global__ void __launch_bounds__(kBlock) k_prep_stream(float4 const* __restrict
 template <int Q, int B, class Src>
 __device__ __forceinline__ void chain_block(Src const& src, int64_t base, int64_t n, float4 const* __restrict__ pts,
                                             int64_t npts, frame const& R, pole const& P, float tau,
-                                            int32_t const* __restrict__ counts, float* __restrict__ out, float4* tile)
+                                            int32_t const* __restrict__ counts, float* __restrict__ out,
+                                            double* __restrict__ part, float4* tile)
 {
     float ph[Q][3], pl[Q][3];                // the query, hi and lo
     float vx[Q], vy[Q], vz[Q], vp[Q], va[Q]; // the previous stream point: v, l + s X, |y| + |z|
@@ -465,6 +466,11 @@ __device__ __forceinline__ void chain_block(Src const& src, int64_t base, int64_
         int64_t const i = base + int64_t(q) * B;
         if (i < n)
         {
+            if (part) // a chunk of the stream: the raw sum, added up by k_chain_sum
+            {
+                part[i] = acc[q];
+                continue;
+            }
             double r = acc[q] / kTwoPi;
             if (counts)
                 r += double(counts[i]);
@@ -480,7 +486,36 @@ __global__ void __launch_bounds__(kBlock)
 {
     __shared__ float4 tile[2 * kBlock];
     int64_t const base = int64_t(blockIdx.x) * (kBlock * Q) + threadIdx.x;
-    chain_block<Q, kBlock>(src, base, n, pts, npts, R, P, tau, counts, out, tile);
+    chain_block<Q, kBlock>(src, base, n, pts, npts, R, P, tau, counts, out, nullptr, tile);
+}
+
+// Launches with too few query tiles to fill the GPU: the stream split into chunks at walk starts (cuts, blockIdx.y
+// picks the chunk; a chunk starting at a walk start carries no chain state), partial sums per chunk, added in chunk
+// order by k_chain_sum (deterministic).
+template <int Q, class Src>
+__global__ void __launch_bounds__(kBlock)
+    k_frac_f32_chain_part(Src const src, int64_t n, float4 const* __restrict__ pts, int64_t const* __restrict__ cuts,
+                          frame R, pole P, float tau, double* __restrict__ part)
+{
+    __shared__ float4 tile[2 * kBlock];
+    int64_t const base = int64_t(blockIdx.x) * (kBlock * Q) + threadIdx.x;
+    int64_t const c0 = cuts[blockIdx.y], c1 = cuts[blockIdx.y + 1];
+    chain_block<Q, kBlock>(src, base, n, pts + 2 * c0, c1 - c0, R, P, tau, nullptr, nullptr,
+                           part + int64_t(blockIdx.y) * n, tile);
+}
+__global__ void __launch_bounds__(kBlock) k_chain_sum(double const* __restrict__ part, int chunks, int64_t n,
+                                                      int32_t const* __restrict__ counts, float* __restrict__ out)
+{
+    int64_t const i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (i >= n)
+        return;
+    double a = 0.0;
+    for (int c = 0; c < chunks; ++c)
+        a += part[int64_t(c) * n + i];
+    double r = a / kTwoPi;
+    if (counts)
+        r += double(counts[i]);
+    out[i] = float(r);
 }
 
 // Persistent chain kernel over a batch of meshes: workgroups take work items (mesh, first query of a B x Q tile)
@@ -506,7 +541,7 @@ __global__ void __launch_bounds__(B)
             break;
         batch_frac_desc const D = meshes[item_mesh[it]];
         chain_block<Q, B>(src_array<float>{pos}, item_q0[it] + threadIdx.x, D.q_end, pts + 2 * D.pt0, D.npts, R, P,
-                          tau, counts, out, tile);
+                          tau, counts, out, nullptr, tile);
     }
 }
 
@@ -720,6 +755,9 @@ pole make_pole(double x0x, double x0y, double x0z)
 // ill-conditioning threshold of the FP32 kernel: pairs with |num| + |den| < tau * l0 * l1 go to FP64
 constexpr float kTau = 1.0f / 32.0f;
 constexpr int kQ = 1;
+// below kSplitBelow query tiles the chain kernel splits the stream to reach ~kSplitTiles workgroups
+constexpr int64_t kSplitBelow = 1024;
+constexpr int64_t kSplitTiles = 2048;
 } // namespace
 
 frame make_pole_frame(double x0x, double x0y, double x0z) { return make_frame_impl(x0x, x0y, x0z); }
@@ -916,12 +954,17 @@ int frac_plan_build(float const* h, int64_t nseg, frac_plan& plan, hipStream_t s
     plan.loops.clear();
     plan.loops_ok = false;
     plan.dec_npts = 0;
+    plan.walk_starts.clear();
     std::vector<float4> stream_pts;
     int64_t nunit = 0;
     if (!chain_stream(h, nseg, stream_pts, nunit))
         return ok;
     plan.npts = int64_t(stream_pts.size());
     plan.nunit = nunit;
+    plan.walk_starts.clear();
+    for (int64_t i = 0; i < plan.npts; ++i)
+        if (stream_pts[size_t(i)].w == 0.0f)
+            plan.walk_starts.push_back(i);
     std::vector<float4> dec;
     plan.loops_ok = loops_of_stream(stream_pts, plan.loops, dec);
     if (!plan.loops_ok)
@@ -1020,7 +1063,7 @@ int frac_decide(float const* pos, int64_t m, int32_t const* counts, frac_plan co
 
 template <class Src>
 int frac_f32_planned(Src const& pos, int64_t n, frac_plan const& plan, float const* seg7, int64_t nseg,
-                     double const x0[3], int32_t const* counts, float* out, hipStream_t stream)
+                     double const x0[3], int32_t const* counts, float* out, hipStream_t stream, bool split_small)
 {
     if (n == 0)
         return ok;
@@ -1038,6 +1081,33 @@ int frac_f32_planned(Src const& pos, int64_t n, frac_plan const& plan, float con
             k_prep_stream<<<dim3(unsigned((plan.npts + kBlock - 1) / kBlock)), dim3(kBlock), 0, stream>>>(
                 plan.pts(), plan.npts, R, p32.as<float4>());
             AGWN_HIP(hipGetLastError());
+        }
+        if (split_small && blocks < kSplitBelow && plan.walk_starts.size() > 1)
+        {
+            // split the stream at walk starts into about kSplitTiles / blocks chunks
+            int64_t const want = std::min<int64_t>({int64_t(32), (kSplitTiles + blocks - 1) / blocks,
+                                                    int64_t(plan.walk_starts.size())});
+            std::vector<int64_t> cuts{0};
+            for (int64_t c = 1; c < want; ++c)
+            {
+                auto it = std::lower_bound(plan.walk_starts.begin(), plan.walk_starts.end(), c * plan.npts / want);
+                if (it != plan.walk_starts.end() && *it > cuts.back())
+                    cuts.push_back(*it);
+            }
+            cuts.push_back(plan.npts);
+            int const chunks = int(cuts.size() - 1);
+            dbuf dc, part;
+            AGWN_TRY(dc.alloc(cuts.size() * sizeof(int64_t), stream));
+            AGWN_HIP(hipMemcpyAsync(dc.p, cuts.data(), cuts.size() * sizeof(int64_t), hipMemcpyHostToDevice, stream));
+            AGWN_TRY(part.alloc(size_t(chunks) * size_t(n) * sizeof(double), stream));
+            k_frac_f32_chain_part<kQ, Src><<<dim3(unsigned(blocks), unsigned(chunks)), dim3(kBlock), 0, stream>>>(
+                pos, n, p32.as<float4>(), dc.as<int64_t>(), R, P, kTau, part.as<double>());
+            AGWN_HIP(hipGetLastError());
+            k_chain_sum<<<dim3(unsigned((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, stream>>>(
+                part.as<double>(), chunks, n, counts, out);
+            AGWN_HIP(hipGetLastError());
+            AGWN_HIP(hipStreamSynchronize(stream)); // `cuts` is a host vector
+            return ok;
         }
         k_frac_f32_chain<kQ, Src><<<dim3(unsigned(blocks)), dim3(kBlock), 0, stream>>>(
             pos, n, p32.as<float4>(), plan.npts, R, P, kTau, counts, out);
@@ -1146,9 +1216,9 @@ int frac_batch_launch(int v, int blocks, float const* pos, batch_frac_desc const
 template int frac_f32_src<src_array<float>>(src_array<float> const&, int64_t, float const*, int64_t, double const*,
                                             int32_t const*, float*, hipStream_t);
 template int frac_f32_planned<src_array<float>>(src_array<float> const&, int64_t, frac_plan const&, float const*,
-                                                int64_t, double const*, int32_t const*, float*, hipStream_t);
+                                                int64_t, double const*, int32_t const*, float*, hipStream_t, bool);
 template int frac_f32_planned<src_grid>(src_grid const&, int64_t, frac_plan const&, float const*, int64_t,
-                                        double const*, int32_t const*, float*, hipStream_t);
+                                        double const*, int32_t const*, float*, hipStream_t, bool);
 template int frac_f32_src<src_grid>(src_grid const&, int64_t, float const*, int64_t, double const*, int32_t const*,
                                     float*, hipStream_t);
 template int frac_f64_src<src_array<double>>(src_array<double> const&, int64_t, double const*, int64_t,

@@ -188,6 +188,7 @@ struct frac_plan
     int64_t npts = 0;   // stream entries
     int64_t nunit = 0;  // unit segments
     bool chain = false;
+    std::vector<int64_t> walk_starts; // stream entries with flag 0 (host copy; chunking of small launches)
     std::vector<loop_host> loops; // closed loops (long walks split into arcs + a skeleton), in the decision stream
     bool loops_ok = false;
     dbuf dec;                     // decision stream: the loops' points, each loop contiguous (flag 0 at its start)
@@ -259,8 +260,12 @@ int frac_batch_launch(int variant, int blocks, float const* pos, batch_frac_desc
 
 // The FP32 launcher with a prepared plan (seg7 is still needed when the plan is not a chain plan).
 template <class Src>
+// split_small: a launch with few query tiles may split the stream into chunks summed separately (fills the GPU; the
+// FP64 sums then add in a different order, so values may differ from the unsplit kernel's in the last float bit --
+// only for callers that need the side of 0.5, not bit-stable values)
 int frac_f32_planned(Src const& src, int64_t n, frac_plan const& plan, float const* seg7, int64_t nseg,
-                     double const x0[3], int32_t const* counts, float* out, hipStream_t stream);
+                     double const x0[3], int32_t const* counts, float* out, hipStream_t stream,
+                     bool split_small = false);
 
 inline int frac_f32(float const* pos, int64_t n, float const* seg7, int64_t nseg, double const x0[3],
                     int32_t const* counts, float* out, hipStream_t stream)

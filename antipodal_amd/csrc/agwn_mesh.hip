@@ -931,8 +931,8 @@ int label_exact_levels(float const* pts, int32_t const* cnt, int64_t m, frac_pla
 {
     if (m == 0)
         return ok;
-    if (level == 2)
-        return frac_f32_planned(src_array<float>{pts}, m, pl, seg7, nseg, x0d, cnt, w, s);
+    if (level == 2) // labels only: small launches may split the stream
+        return frac_f32_planned(src_array<float>{pts}, m, pl, seg7, nseg, x0d, cnt, w, s, true);
     dbuf und;
     AGWN_TRY(und.alloc(size_t(m) * sizeof(int64_t), s));
     int64_t nu = 0;
