@@ -256,6 +256,64 @@ struct alignas(sizeof(C) == 4 ? 32 : 64) rseg
     double w;
 };
 
+// --- device construction (gwn_mesh_robust.hh:177-209, build_quantization :354-378) -----------------------------------
+// Exact bounding box: doubles mapped to order-preserving uint64 keys, reduced per lane over a strided range, then one
+// atomic min / max per lane. box[0..2] = min keys, box[3..5] = max keys.
+__device__ __forceinline__ unsigned long long dkey(double v)
+{
+    unsigned long long const b = static_cast<unsigned long long>(__double_as_longlong(v));
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+inline double dkey_value(unsigned long long k)
+{
+    unsigned long long const b = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+    double v;
+    std::memcpy(&v, &b, sizeof(v));
+    return v;
+}
+template <class T>
+__global__ void __launch_bounds__(kB) k_rbox(T const* __restrict__ v, int64_t nv, unsigned long long* __restrict__ box)
+{
+    unsigned long long mn[3] = {~0ull, ~0ull, ~0ull}, mx[3] = {0ull, 0ull, 0ull};
+    for (int64_t i = int64_t(blockIdx.x) * kB + threadIdx.x; i < nv; i += int64_t(gridDim.x) * kB)
+        for (int k = 0; k < 3; ++k)
+        {
+            unsigned long long const key = dkey(double(v[3 * i + k]));
+            mn[k] = key < mn[k] ? key : mn[k];
+            mx[k] = key > mx[k] ? key : mx[k];
+        }
+    for (int k = 0; k < 3; ++k)
+    {
+        atomicMin(box + k, mn[k]);
+        atomicMax(box + 3 + k, mx[k]);
+    }
+}
+// quantized triangle corners (input order) and swapped boundary segments
+template <class C, class T>
+__global__ void __launch_bounds__(kB) k_rquant_tri(T const* __restrict__ v, int32_t const* __restrict__ idx,
+                                                   int64_t n3, quant Q, v3<C>* __restrict__ out)
+{
+    int64_t const t = int64_t(blockIdx.x) * kB + threadIdx.x;
+    if (t >= n3)
+        return;
+    T const* p = v + 3 * int64_t(idx[t]);
+    out[t] = quantize<C>(Q, double(p[0]), double(p[1]), double(p[2]));
+}
+template <class C, class T>
+__global__ void __launch_bounds__(kB) k_rquant_seg(T const* __restrict__ seg7, int64_t ns, quant Q,
+                                                   rseg<C>* __restrict__ out)
+{
+    int64_t const k = int64_t(blockIdx.x) * kB + threadIdx.x;
+    if (k >= ns)
+        return;
+    T const* g = seg7 + 7 * k;
+    rseg<C> r;
+    r.p0 = quantize<C>(Q, double(g[3]), double(g[4]), double(g[5]));
+    r.p1 = quantize<C>(Q, double(g[0]), double(g[1]), double(g[2]));
+    r.w = double(g[6]);
+    out[k] = r;
+}
+
 // --- integer term ---------------------------------------------------------------------------------------------------
 template <class C>
 __device__ inline int leaf_count(v3<C> qi, v3<C> const* tri)
@@ -485,22 +543,46 @@ struct robust_t final : robust_impl
         lbvh_release(topo);
     }
 
-    // gwn_mesh_robust.hh:177-209 + build_quantization :354-378
+    // gwn_mesh_robust.hh:177-209 + build_quantization :354-378, on the device: the inputs are uploaded once, the bounding
+    // box is reduced by k_rbox, the quantization parameters follow from it on the host with the reference's
+    // arithmetic, and triangles and segments are quantized by kernels (the queries' quantize(), contraction off)
     int create(T const* hv, int64_t nv, int32_t const* hidx, int64_t nidx, T const* hseg, int64_t ns, int bits)
     {
 #pragma clang fp contract(off)
         ntri = nidx / 3;
         nseg = ns;
         Q.grid_bits = bits;
+        hipStream_t s = nullptr;
+        AGWN_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        struct sguard
+        {
+            hipStream_t s;
+            ~sguard() { (void)hipStreamDestroy(s); }
+        } sg{s};
+        dbuf dv, di, ds;
+        AGWN_TRY(dv.alloc(size_t(std::max<int64_t>(nv, 1)) * 3 * sizeof(T), s));
+        AGWN_TRY(di.alloc(size_t(std::max<int64_t>(nidx, 1)) * sizeof(int32_t), s));
+        AGWN_TRY(ds.alloc(size_t(std::max<int64_t>(ns, 1)) * 7 * sizeof(T), s));
+        if (nv > 0)
+            AGWN_HIP(hipMemcpyAsync(dv.p, hv, size_t(nv) * 3 * sizeof(T), hipMemcpyHostToDevice, s));
+        if (nidx > 0)
+            AGWN_HIP(hipMemcpyAsync(di.p, hidx, size_t(nidx) * sizeof(int32_t), hipMemcpyHostToDevice, s));
+        if (ns > 0)
+            AGWN_HIP(hipMemcpyAsync(ds.p, hseg, size_t(ns) * 7 * sizeof(T), hipMemcpyHostToDevice, s));
         if (bits > 0 && nv > 0)
         {
-            double mn[3] = {double(hv[0]), double(hv[1]), double(hv[2])}, mx[3] = {mn[0], mn[1], mn[2]};
-            for (int64_t i = 0; i < nv; ++i)
-                for (int k = 0; k < 3; ++k)
-                {
-                    mn[k] = std::min(mn[k], double(hv[3 * i + k]));
-                    mx[k] = std::max(mx[k], double(hv[3 * i + k]));
-                }
+            dbuf db;
+            AGWN_TRY(db.alloc(6 * sizeof(unsigned long long), s));
+            unsigned long long const init[6] = {~0ull, ~0ull, ~0ull, 0ull, 0ull, 0ull};
+            AGWN_HIP(hipMemcpyAsync(db.p, init, sizeof(init), hipMemcpyHostToDevice, s));
+            unsigned const gb = unsigned(std::min<int64_t>((nv + kB - 1) / kB, 1024));
+            k_rbox<T><<<dim3(gb), dim3(kB), 0, s>>>(dv.as<T>(), nv, db.as<unsigned long long>());
+            AGWN_HIP(hipGetLastError());
+            unsigned long long hb[6];
+            AGWN_HIP(hipMemcpyAsync(hb, db.p, sizeof(hb), hipMemcpyDeviceToHost, s));
+            AGWN_HIP(hipStreamSynchronize(s));
+            double const mn[3] = {dkey_value(hb[0]), dkey_value(hb[1]), dkey_value(hb[2])};
+            double const mx[3] = {dkey_value(hb[3]), dkey_value(hb[4]), dkey_value(hb[5])};
             double const ex = mx[0] - mn[0], ey = mx[1] - mn[1], ez = mx[2] - mn[2];
             Q.cx = mn[0] + ex * 0.5;
             Q.cy = mn[1] + ey * 0.5;
@@ -509,38 +591,25 @@ struct robust_t final : robust_impl
             double const max_coord = double((int64_t(1) << bits) - 1);
             Q.scale = half > 0.0 ? max_coord / half : 1.0;
         }
-        auto const q = [&](T const* p) { return quantize<C>(Q, double(p[0]), double(p[1]), double(p[2])); };
-        std::vector<v3<C>> ht(size_t(3 * ntri));
-        for (int64_t t = 0; t < ntri; ++t)
-            for (int k = 0; k < 3; ++k)
-                ht[size_t(3 * t + k)] = q(hv + 3 * int64_t(hidx[3 * t + k]));
-        std::vector<rseg<C>> hs(size_t(std::max<int64_t>(ns, 1)));
-        for (int64_t s = 0; s < ns; ++s)
+        AGWN_HIP(hipMalloc(&seg, size_t(std::max<int64_t>(ns, 1)) * sizeof(rseg<C>)));
+        if (ns > 0)
         {
-            hs[size_t(s)].p0 = q(hseg + 7 * s + 3);
-            hs[size_t(s)].p1 = q(hseg + 7 * s + 0);
-            hs[size_t(s)].w = double(hseg[7 * s + 6]);
+            k_rquant_seg<C, T><<<dim3(unsigned((ns + kB - 1) / kB)), dim3(kB), 0, s>>>(ds.as<T>(), ns, Q, seg);
+            AGWN_HIP(hipGetLastError());
         }
-        hipStream_t s = nullptr;
-        AGWN_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-        struct sguard
-        {
-            hipStream_t s;
-            ~sguard() { (void)hipStreamDestroy(s); }
-        } sg{s};
-        AGWN_HIP(hipMalloc(&seg, hs.size() * sizeof(rseg<C>)));
-        AGWN_HIP(hipMemcpyAsync(seg, hs.data(), hs.size() * sizeof(rseg<C>), hipMemcpyHostToDevice, s));
         if (ntri > 0)
         {
             dbuf dt, dc;
-            AGWN_TRY(dt.alloc(ht.size() * sizeof(v3<C>), s));
+            AGWN_TRY(dt.alloc(size_t(3 * ntri) * sizeof(v3<C>), s));
             AGWN_TRY(dc.alloc(size_t(ntri) * 3 * sizeof(float), s));
-            AGWN_HIP(hipMemcpyAsync(dt.p, ht.data(), ht.size() * sizeof(v3<C>), hipMemcpyHostToDevice, s));
+            k_rquant_tri<C, T><<<dim3(unsigned((3 * ntri + kB - 1) / kB)), dim3(kB), 0, s>>>(dv.as<T>(), di.as<int32_t>(),
+                                                                                         3 * ntri, Q, dt.as<v3<C>>());
+            AGWN_HIP(hipGetLastError());
             unsigned const g = unsigned((ntri + kB - 1) / kB);
             k_rleaf_boxes<C><<<dim3(g), dim3(kB), 0, s>>>(dt.as<v3<C>>(), ntri, 0.0, nullptr, nullptr, dc.as<float>());
             AGWN_HIP(hipGetLastError());
             AGWN_TRY(lbvh_build_topology(dc.as<float>(), ntri, topo, s));
-            AGWN_HIP(hipMalloc(&tri, ht.size() * sizeof(v3<C>)));
+            AGWN_HIP(hipMalloc(&tri, size_t(3 * ntri) * sizeof(v3<C>)));
             k_rgather<C><<<dim3(g), dim3(kB), 0, s>>>(dt.as<v3<C>>(), topo.order, ntri, tri);
             AGWN_HIP(hipGetLastError());
             if (ntri > 1)
