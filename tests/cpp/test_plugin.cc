@@ -87,9 +87,12 @@ void plain_cases(T tol)
             CHECK(got2[i] == got[i]);
             CHECK(std::abs(got[i] - base) < tol);
             CHECK(gi[i] == ri[i]);
-            // single-query Intersector concept
+            // single-query Intersector concept, with x0 and with other directions (unnormalised ones included)
             CHECK(hip.signed_intersection_count(pts[i], x0) == naive.signed_intersection_count(pts[i], x0));
             CHECK(eval_gwnr_mesh_single_integer(pts[i], x0, hip) == ri[i]);
+            for (vec3<T> const d : {vec3<T>{T(1), T(0), T(0)}, vec3<T>{T(0), T(-2), T(0)}, vec3<T>{T(0), T(0), T(0.5)},
+                                    vec3<T>{T(-0.3), T(0.9), T(-0.2)}, vec3<T>{T(0.7), T(-0.1), T(0.45)}})
+                CHECK(hip.signed_intersection_count(pts[i], d) == naive.signed_intersection_count(pts[i], d));
         }
         std::vector<T> fr(pts.size()), frr(pts.size());
         eval_gwnr_mesh_batch_fractional(hd, P, std::span<T>{fr}, x0, B);
