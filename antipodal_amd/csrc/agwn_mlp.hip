@@ -6,7 +6,7 @@
 //   u_a = (p_a - c_a) / h_a                     c, h: centre and half-extent of the training box, per axis
 //   features f[2j], f[2j+1] = sin(pi w_k u_a), cos(pi w_k u_a),  j = 32 a + k,  w_k = F^(k/31)   (192 features;
 //                                                                               F = max_freq, 128 by default)
-//   z_l = H_{l-1} W_l + b_l,  H_l = silu(z_l)   l = 0..3, 256 units, H_{-1} = f
+//   z_l = H_{l-1} W_l + b_l,  H_l = hsilu(z_l)  l = 0..3, 256 units, H_{-1} = f,  hsilu(z) = z clamp(z/4 + 1/2, 0, 1)
 //   pred = H_3 . wo + bo
 // Rounding points (what the kernels store, and what the numerics tests emulate): features, weights W_l, activations
 // H_l and backward activations dz_l are BF16; products accumulate in FP32; z_l, biases, wo, bo, the loss and every
@@ -75,14 +75,15 @@ struct feat_params
     float w[kFreq];
 };
 
-// SiLU and its derivative with the hardware reciprocal (1 ulp): an IEEE division costs ~10 VALU instructions, which
-// would dominate the fused inference kernel's epilogues. Training and inference share these definitions.
-__device__ __forceinline__ float sigmoid(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
-__device__ __forceinline__ float silu(float z) { return z * sigmoid(z); }
-__device__ __forceinline__ float silu_grad(float z)
+// The activation: hard SiLU, z clamp(z/4 + 1/2, 0, 1) -- SiLU with its sigmoid replaced by the sigmoid's tangent at 0,
+// clamped -- and its derivative, z/2 + 1/2 for |z| < 2, 1 above, 0 below (the one-sided values at z = +-2 are 1 and
+// 0). Three VALU instructions per unit where SiLU's exponential and reciprocal cost a fifth of the fused inference
+// kernel (1024 activations per point); networks trained with it route fewer points than SiLU's on the benchmark mesh.
+// Training and inference share these definitions.
+__device__ __forceinline__ float hsilu(float z) { return z * __builtin_amdgcn_fmed3f(fmaf(0.25f, z, 0.5f), 0.0f, 1.0f); }
+__device__ __forceinline__ float hsilu_grad(float z)
 {
-    float const s = sigmoid(z);
-    return s * (1.0f + z * (1.0f - s));
+    return fabsf(z) < 2.0f ? fmaf(0.5f, z, 0.5f) : (z >= 2.0f ? 1.0f : 0.0f);
 }
 
 // feature pair j (0..95) of a point: axis a = j / 32, frequency k = j % 32: (sin, cos)(pi w_k u), u the box-normalised
@@ -131,8 +132,8 @@ __global__ void __launch_bounds__(kB) k_features(float const* __restrict__ pts, 
 // ---------------------------------------------------------------------------------------------------------------------
 enum epi_kind
 {
-    EPI_FWD,  // r = unit n, c = sample m: z = acc + bias[n] -> Z[m][n] (FP32), H[m][n], HT[n][m] = bf16(silu(z))
-    EPI_DX,   // r = input k, c = sample m: dz = acc * silu'(Z[m][k]) -> D[m][k], DT[k][m] (BF16)
+    EPI_FWD,  // r = unit n, c = sample m: z = acc + bias[n] -> Z[m][n] (FP32), H[m][n], HT[n][m] = bf16(hsilu(z))
+    EPI_DX,   // r = input k, c = sample m: dz = acc * hsilu'(Z[m][k]) -> D[m][k], DT[k][m] (BF16)
     EPI_PART, // r = input k, c = unit n: P[blockIdx.z][k][n] = acc
 };
 
@@ -216,7 +217,7 @@ __global__ void __launch_bounds__(256)
                     for (int e = 0; e < 4; ++e)
                     {
                         z[e] = v[e] + E.bias[r + e];
-                        hv[e] = __bf16(silu(z[e]));
+                        hv[e] = __bf16(hsilu(z[e]));
                     }
                     *reinterpret_cast<float4*>(E.zout + c * R + r) = make_float4(z[0], z[1], z[2], z[3]);
                     *reinterpret_cast<bf16x4*>(E.h + c * R + r) = hv;
@@ -231,7 +232,7 @@ __global__ void __launch_bounds__(256)
                     bf16x4 dv;
 #pragma unroll
                     for (int e = 0; e < 4; ++e)
-                        dv[e] = __bf16(v[e] * silu_grad(zv[e]));
+                        dv[e] = __bf16(v[e] * hsilu_grad(zv[e]));
                     *reinterpret_cast<bf16x4*>(E.h + c * R + r) = dv;
 #pragma unroll
                     for (int e = 0; e < 4; ++e)
@@ -279,7 +280,7 @@ __global__ void __launch_bounds__(kB) k_rowsum(__bf16 const* __restrict__ DT, in
 constexpr int64_t kObl = 64; // blocks of the output-layer kernel
 
 // Output layer and loss. Wave-per-sample; lane l covers units 4l .. 4l+3.
-//   pred = H3 . wo + bo;  g = 2 (pred - y) / nvalid;  dz3 = bf16(g wo silu'(Z3));  per-block partials of g H3 (dwo),
+//   pred = H3 . wo + bo;  g = 2 (pred - y) / nvalid;  dz3 = bf16(g wo hsilu'(Z3));  per-block partials of g H3 (dwo),
 //   g (dbo) and (pred - y)^2.
 // y is read through perm (sample order of the batch). Without D only pred is written.
 __global__ void __launch_bounds__(kB)
@@ -325,7 +326,7 @@ __global__ void __launch_bounds__(kB)
         bf16x4 dv;
         for (int j = 0; j < 4; ++j)
         {
-            dv[j] = __bf16(g * w[j] * silu_grad(zv[j]));
+            dv[j] = __bf16(g * w[j] * hsilu_grad(zv[j]));
             gw[j] += g * h[j];
         }
         *reinterpret_cast<bf16x4*>(D + m * kH + 4 * lane) = dv;
@@ -663,10 +664,10 @@ __global__ void __launch_bounds__(256, 2) k_mlp_infer(Src const src, int64_t n, 
                 for (int j = 0; j < 4; ++j)
                 {
                     bf16x4 hv;
-                    hv[0] = __bf16(silu(acc[i][j][4 * q] + b0));
-                    hv[1] = __bf16(silu(acc[i][j][4 * q + 1] + b1));
-                    hv[2] = __bf16(silu(acc[i][j][4 * q + 2] + b2));
-                    hv[3] = __bf16(silu(acc[i][j][4 * q + 3] + b3));
+                    hv[0] = __bf16(hsilu(acc[i][j][4 * q] + b0));
+                    hv[1] = __bf16(hsilu(acc[i][j][4 * q + 1] + b1));
+                    hv[2] = __bf16(hsilu(acc[i][j][4 * q + 2] + b2));
+                    hv[3] = __bf16(hsilu(acc[i][j][4 * q + 3] + b3));
                     *reinterpret_cast<bf16x4*>(act + (j * 32 + li) * kLdsStride + u) = hv;
                 }
             }
@@ -984,7 +985,7 @@ int pass(mlp_impl& M, train_ws& ws, int64_t nvalid, float const* y, int64_t cons
         k_rowsum<<<dim3(kH), blk, 0, s>>>(dt, rows, M.g + kLay.b[l]);
         if (l > 0)
         {
-            // D_{l-1}[m][k] = (sum_n Wr_l[k][n] D_l[m][n]) silu'(Z_{l-1}[m][k])
+            // D_{l-1}[m][k] = (sum_n Wr_l[k][n] D_l[m][n]) hsilu'(Z_{l-1}[m][k])
             epi_args E2{};
             E2.zin = ws.Z[l - 1].as<float>();
             E2.h = ws.D[(l - 1) & 1].as<__bf16>();

@@ -6,7 +6,7 @@ Workload: seeded noisy torus R=1, r=0.35, 1353 x 473 quads (2 triangles each), s
 lattice over the bounding box expanded by 10 % (134,217,728 points, generated in-kernel); FP32.
 
   plain      one eval_grid (HipGridGwn): full GWN (fractional + integer term) of every lattice point, labels = w > 0.5
-  surrogate  a per-mesh MLP (192 Fourier features up to 64 periods, 4 x 256 SiLU, BF16 MFMA) trained once on 2^24
+  surrogate  a per-mesh MLP (192 Fourier features up to 64 periods, 4 x 256 hard-SiLU, BF16 MFMA) trained once on 2^24
              exact GWN samples (70 % drawn within 0.05 of the surface, 30 % uniform in the box, minus those in the
              near-surface mask); each step labels the lattice with the fused inference kernel and routes to the exact
              kernels every point in the mask (voxels of edge eps/4 within eps = 0.10 % of the box diagonal of a

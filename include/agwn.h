@@ -185,9 +185,9 @@ int agwn_batch_info(const agwn_batch* batch, int64_t* k, int64_t* ntri, int64_t*
                     double* tune_ms);
 
 /* ---------------------------------------------------------------------------------------------------------------
- * Learned inside/outside surrogate (no counterpart in the reference: an accelerator in front of the exact path, which
- * stays the judge). A per-mesh MLP -- 192 Fourier features of the position normalised to a box (32 frequencies
- * max_freq^(k/31) per axis, sin and cos), 4 hidden layers x 256 SiLU, linear output -- with BF16 weights/activations and
+ * Learned inside/outside surrogate (no counterpart in the reference: an accelerator in front of the exact path, whose
+ * labels stay authoritative). A per-mesh MLP -- 192 Fourier features of the position normalised to a box (32 frequencies
+ * max_freq^(k/31) per axis, sin and cos), 4 hidden layers x 256 hard-SiLU (z clamp(z/4 + 1/2, 0, 1)), linear output -- with BF16 weights/activations and
  * FP32 accumulation, trained on GPU-computed exact GWN values by the library's own MFMA forward/backward kernels and
  * Adam (FP32 master weights). Labels are GWN > 0.5 ("inside").
  * --------------------------------------------------------------------------------------------------------------- */

@@ -2,7 +2,7 @@
 
 The learned surrogate has no counterpart in the reference; its definition is the one the library documents
 (include/agwn.h, antipodal_amd/csrc/agwn_mlp.hip): 192 Fourier features of the box-normalised position, 4 hidden
-layers of 256 SiLU units, a linear output, mean squared error, Adam. This module evaluates that definition in float64:
+layers of 256 hard-SiLU units (z clamp(z/4 + 1/2, 0, 1)), a linear output, mean squared error, Adam. This module evaluates that definition in float64:
 
   forward / loss_grad(..., emulate=False)   exact float64 math (pinned by central finite differences and by closed
                                             forms in tests/test_oracle_mlp.py)
@@ -94,13 +94,16 @@ def features(pts, lo, hi, max_freq: float = 128.0) -> np.ndarray:
     return out.reshape(len(p), IN)
 
 
-def silu(z):
-    return z / (1.0 + np.exp(-z))
+def hsilu(z):
+    """Hard SiLU: z clamp(z/4 + 1/2, 0, 1)."""
+    z = np.asarray(z, dtype=np.float64)
+    return z * np.clip(0.25 * z + 0.5, 0.0, 1.0)
 
 
-def silu_grad(z):
-    s = 1.0 / (1.0 + np.exp(-z))
-    return s * (1.0 + z * (1.0 - s))
+def hsilu_grad(z):
+    """z/2 + 1/2 for |z| < 2, 1 for z >= 2, 0 for z <= -2 (the library's one-sided choices at the kinks)."""
+    z = np.asarray(z, dtype=np.float64)
+    return np.where(np.abs(z) < 2.0, 0.5 * z + 0.5, np.where(z >= 2.0, 1.0, 0.0))
 
 
 def forward(p, pts, lo, hi, emulate: bool = False, max_freq: float = 128.0):
@@ -111,7 +114,7 @@ def forward(p, pts, lo, hi, emulate: bool = False, max_freq: float = 128.0):
     xs, zs = [x], []
     for l in range(LAYERS):
         z = x @ r(Ws[l]) + bs[l]
-        x = r(silu(z))
+        x = r(hsilu(z))
         zs.append(z)
         xs.append(x)
     pred = x @ wo + bo
@@ -131,12 +134,12 @@ def loss_grad(p, pts, y, lo, hi, emulate: bool = False, max_freq: float = 128.0)
     gWs, gbs = [None] * LAYERS, [None] * LAYERS
     gwo = xs[LAYERS].T @ g
     gbo = float(g.sum())
-    d = r(g[:, None] * wo[None, :] * silu_grad(zs[LAYERS - 1]))
+    d = r(g[:, None] * wo[None, :] * hsilu_grad(zs[LAYERS - 1]))
     for l in range(LAYERS - 1, -1, -1):
         gWs[l] = xs[l].T @ d
         gbs[l] = d.sum(axis=0)
         if l > 0:
-            d = r((d @ r(Ws[l]).T) * silu_grad(zs[l - 1]))
+            d = r((d @ r(Ws[l]).T) * hsilu_grad(zs[l - 1]))
     return loss, pack(gWs, gbs, gwo, gbo), pred
 
 
