@@ -37,6 +37,7 @@
 #include "agwn_internal.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <vector>
 
@@ -641,13 +642,41 @@ __device__ double loop_exact(float4 const* __restrict__ p, int count, float cons
     return acc;
 }
 
-// One lane per query (FP32 positions in the caller's frame, counts = integer terms). Every lane of the block runs the
-// loop over loops to the end (wave-wide votes decide which loops are summed exactly).
+// Far-field test of one record (a loop, or a cluster of loops) at the query: true when it must be summed exactly (or,
+// for a cluster, split into its loops); otherwise its estimate and bound are added to est and bnd (steradians).
+__device__ __forceinline__ bool farfield_near(loop_rec const& r, float const ph[3], float beta4pi, float& est,
+                                              float& bnd)
+{
+    float const dx = r.cx - ph[0], dy = r.cy - ph[1], dz = r.cz - ph[2];
+    float const yz = dy * dy + dz * dz;
+    float const d = __builtin_amdgcn_sqrtf(dx * dx + yz);
+    float const gap = d - r.rho;
+    // near when the line along x0 (the pole frame's first axis) meets the ball, or the query is in or close to it
+    if (!(yz > r.rho * r.rho) || !(gap > 1e-3f * (d + r.cnorm)))
+        return true;
+    float const g2 = gap * gap;
+    float const b = r.k3 * __builtin_amdgcn_rcpf(g2 * g2);
+    if (b > beta4pi)
+        return true;
+    float const inv = __builtin_amdgcn_rcpf(d);
+    float const ux = dx * inv, uy = dy * inv, uz = dz * inv;
+    float const ad = r.ax * ux + r.ay * uy + r.az * uz;
+    float const mq = ux * (r.m[0] * ux + r.m[1] * uy + r.m[2] * uz) + uy * (r.m[3] * ux + r.m[4] * uy + r.m[5] * uz) +
+                     uz * (r.m[6] * ux + r.m[7] * uy + r.m[8] * uz);
+    est += (ad + (r.trm - 3.0f * mq) * inv) * (inv * inv);
+    bnd += b;
+    return false;
+}
+
+// One lane per query (FP32 positions in the caller's frame, counts = integer terms). Clusters first: a cluster whose
+// ball passes the far-field test for a lane contributes its combined estimate and bound; otherwise that lane takes its
+// loops one by one (the wave walks a cluster's loops when any lane needs them, and sums a loop exactly when any lane
+// needs that, by wave-wide votes).
 __global__ void __launch_bounds__(kDecideBlock)
     k_frac_decide(float const* __restrict__ pos, int64_t n, int32_t const* __restrict__ counts,
-                  loop_rec const* __restrict__ loops, int nloops, float4 const* __restrict__ pts, frame R, pole P,
-                  float tau, float beta4pi, double margin, float* __restrict__ val, int64_t* __restrict__ undecided,
-                  unsigned long long* __restrict__ nund)
+                  loop_rec const* __restrict__ clusters, int nclusters, loop_rec const* __restrict__ loops,
+                  float4 const* __restrict__ pts, frame R, pole P, float tau, float beta4pi, double margin,
+                  float* __restrict__ val, int64_t* __restrict__ undecided, unsigned long long* __restrict__ nund)
 {
     __shared__ float4 stage[kDecideBlock / 64][128];
     int const lane = int(threadIdx.x & 63);
@@ -656,41 +685,26 @@ __global__ void __launch_bounds__(kDecideBlock)
     bool const valid = i < n;
     float ph[3], pl[3];
     query_split(src_array<float>{pos}, i, n, R, ph, pl);
-    double fe = 0.0;                // exact loops (radians)
-    float est = 0.0f, bnd = 0.0f;   // far loops: -Omega estimate and remainder bound (steradians)
-    for (int L = 0; L < nloops; ++L)
+    double fe = 0.0;              // exact loops (radians)
+    float est = 0.0f, bnd = 0.0f; // far clusters and loops: -Omega estimate and remainder bound (steradians)
+    for (int C = 0; C < nclusters; ++C)
     {
-        loop_rec const r = loops[L];
-        float const dx = r.cx - ph[0], dy = r.cy - ph[1], dz = r.cz - ph[2];
-        float const yz = dy * dy + dz * dz;
-        float const d = __builtin_amdgcn_sqrtf(dx * dx + yz);
-        float const gap = d - r.rho;
-        // exact when the line along x0 (the pole frame's first axis) meets the ball, or the query is in or close to it
-        bool near = !(yz > r.rho * r.rho) || !(gap > 1e-3f * (d + r.cnorm));
-        if (!near)
+        loop_rec const cr = clusters[C];
+        bool const split = farfield_near(cr, ph, beta4pi, est, bnd) && valid;
+        if (!__any(split))
+            continue;
+        for (int L = cr.start; L < cr.start + cr.count; ++L)
         {
-            float const g2 = gap * gap;
-            float const b = r.k3 * __builtin_amdgcn_rcpf(g2 * g2);
-            if (b > beta4pi)
-                near = true;
-            else
+            loop_rec const r = loops[L];
+            bool near = false;
+            if (split)
+                near = farfield_near(r, ph, beta4pi, est, bnd);
+            if (__any(near))
             {
-                float const inv = __builtin_amdgcn_rcpf(d);
-                float const ux = dx * inv, uy = dy * inv, uz = dz * inv;
-                float const ad = r.ax * ux + r.ay * uy + r.az * uz;
-                float const mq = ux * (r.m[0] * ux + r.m[1] * uy + r.m[2] * uz) +
-                                 uy * (r.m[3] * ux + r.m[4] * uy + r.m[5] * uz) +
-                                 uz * (r.m[6] * ux + r.m[7] * uy + r.m[8] * uz);
-                est += (ad + (r.trm - 3.0f * mq) * inv) * (inv * inv);
-                bnd += b;
+                double const sl = loop_exact(pts + 2 * int64_t(r.start), r.count, ph, pl, P, tau, st, lane);
+                if (near)
+                    fe += sl;
             }
-        }
-        near = near && valid;
-        if (__any(near))
-        {
-            double const sl = loop_exact(pts + 2 * int64_t(r.start), r.count, ph, pl, P, tau, st, lane);
-            if (near)
-                fe += sl;
         }
     }
     if (valid)
@@ -947,6 +961,76 @@ bool loops_of_stream(std::vector<float4> const& p, std::vector<loop_host>& loops
     return true;
 }
 
+// Clusters of the decision's loops: the loops in the Morton order of their centres (10 bits per axis over the box of
+// the centres), taken kClusterLoops at a time. A cluster's ball holds its members' balls (centre: the mean of theirs,
+// radius: max |c_L - c| + rho_L), so it holds their fans; its vector area and fan area are the members' sums and its
+// first moment, about its centre, sum_L M_L + (c_L - c) (x) a_L. The far-field expansion and bound hold for any surface
+// inside the ball, so a cluster is tested and estimated like one loop. Returns the clusters (start, count: their
+// members' range in `order`) and fills `order` with the loop indices.
+constexpr int kClusterLoops = 16;
+
+std::vector<loop_host> loop_clusters(std::vector<loop_host> const& loops, std::vector<int32_t>& order)
+{
+    size_t const n = loops.size();
+    double lo[3] = {HUGE_VAL, HUGE_VAL, HUGE_VAL}, hi[3] = {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
+    for (loop_host const& L : loops)
+        for (int a = 0; a < 3; ++a)
+        {
+            lo[a] = std::min(lo[a], L.c[a]);
+            hi[a] = std::max(hi[a], L.c[a]);
+        }
+    auto spread = [](uint64_t v) {
+        uint64_t x = v & 0x3FFu;
+        x = (x | (x << 16)) & 0x030000FFull;
+        x = (x | (x << 8)) & 0x0300F00Full;
+        x = (x | (x << 4)) & 0x030C30C3ull;
+        x = (x | (x << 2)) & 0x09249249ull;
+        return x;
+    };
+    std::vector<std::pair<uint64_t, int32_t>> key(n);
+    for (size_t k = 0; k < n; ++k)
+    {
+        uint64_t code = 0;
+        for (int a = 0; a < 3; ++a)
+        {
+            double const ext = hi[a] - lo[a];
+            double const f = ext > 0.0 ? (loops[k].c[a] - lo[a]) / ext : 0.0;
+            code |= spread(uint64_t(std::min(1023.0, std::max(0.0, f * 1024.0)))) << a;
+        }
+        key[k] = {code, int32_t(k)};
+    }
+    std::sort(key.begin(), key.end());
+    order.resize(n);
+    for (size_t k = 0; k < n; ++k)
+        order[k] = key[k].second;
+    std::vector<loop_host> out;
+    for (size_t k0 = 0; k0 < n; k0 += kClusterLoops)
+    {
+        size_t const k1 = std::min(n, k0 + kClusterLoops);
+        loop_host C{};
+        for (size_t k = k0; k < k1; ++k)
+            for (int a = 0; a < 3; ++a)
+                C.c[a] += loops[size_t(order[k])].c[a] / double(k1 - k0);
+        for (size_t k = k0; k < k1; ++k)
+        {
+            loop_host const& L = loops[size_t(order[k])];
+            double const dc[3] = {L.c[0] - C.c[0], L.c[1] - C.c[1], L.c[2] - C.c[2]};
+            C.rho = std::max(C.rho, std::sqrt(dc[0] * dc[0] + dc[1] * dc[1] + dc[2] * dc[2]) + L.rho);
+            C.area += L.area;
+            for (int r = 0; r < 3; ++r)
+            {
+                C.a[r] += L.a[r];
+                for (int j = 0; j < 3; ++j)
+                    C.m[3 * r + j] += L.m[3 * r + j] + dc[r] * L.a[j];
+            }
+        }
+        C.start = int64_t(k0);
+        C.count = int64_t(k1 - k0);
+        out.push_back(C);
+    }
+    return out;
+}
+
 int frac_plan_build(float const* h, int64_t nseg, frac_plan& plan, hipStream_t stream)
 {
     plan.chain = false;
@@ -1000,11 +1084,10 @@ int frac_decide(float const* pos, int64_t m, int32_t const* counts, frac_plan co
     AGWN_CHECK(plan.loops.size() < size_t(INT32_MAX) && plan.dec_npts < INT32_MAX, "boundary too large for the decision");
     frame const R = make_frame_impl(x0[0], x0[1], x0[2]);
     pole const P = make_pole(x0[0], x0[1], x0[2]);
-    std::vector<loop_rec> tab(plan.loops.size());
-    for (size_t k = 0; k < tab.size(); ++k)
-    {
-        loop_host const& L = plan.loops[k];
-        loop_rec& r = tab[k];
+    // a record in the pole frame; its ball grown by 0.1 % (+ 1e-6 |c|) and its bound numerator by 0.1 % so that the
+    // FP32 tests stay conservative
+    auto to_rec = [&](loop_host const& L, int64_t start, int64_t count) {
+        loop_rec r{};
         double c[3], a[3], t[9], mr[9];
         for (int i = 0; i < 3; ++i)
         {
@@ -1031,9 +1114,19 @@ int frac_decide(float const* pos, int64_t m, int32_t const* counts, frac_plan co
             r.m[i] = float(mr[i]);
         r.trm = float(mr[0] + mr[4] + mr[8]);
         r.cnorm = float(cn);
-        r.start = int32_t(L.start);
-        r.count = int32_t(L.count);
-        r.pad[0] = r.pad[1] = r.pad[2] = 0.0f;
+        r.start = int32_t(start);
+        r.count = int32_t(count);
+        return r;
+    };
+    std::vector<int32_t> order;
+    std::vector<loop_host> cl = loop_clusters(plan.loops, order);
+    std::vector<loop_rec> tab(plan.loops.size() + cl.size());
+    for (size_t k = 0; k < cl.size(); ++k)
+        tab[k] = to_rec(cl[k], cl[k].start, cl[k].count);
+    for (size_t k = 0; k < order.size(); ++k)
+    {
+        loop_host const& L = plan.loops[size_t(order[k])];
+        tab[cl.size() + k] = to_rec(L, L.start, L.count);
     }
     dbuf dt, p32, dn;
     AGWN_TRY(dt.alloc(std::max<size_t>(tab.size(), 1) * sizeof(loop_rec), s));
@@ -1051,8 +1144,8 @@ int frac_decide(float const* pos, int64_t m, int32_t const* counts, frac_plan co
     int64_t const blocks = (m + kDecideBlock - 1) / kDecideBlock;
     AGWN_CHECK(blocks < (int64_t(1) << 31), "frac_decide: too many queries (%lld)", (long long)m);
     k_frac_decide<<<dim3(unsigned(blocks)), dim3(kDecideBlock), 0, s>>>(
-        pos, m, counts, dt.as<loop_rec>(), int(tab.size()), p32.as<float4>(), R, P, kTau, beta * kFourPiF,
-        kDecideMargin, val, undecided, dn.as<unsigned long long>());
+        pos, m, counts, dt.as<loop_rec>(), int(cl.size()), dt.as<loop_rec>() + cl.size(), p32.as<float4>(), R, P, kTau,
+        beta * kFourPiF, kDecideMargin, val, undecided, dn.as<unsigned long long>());
     AGWN_HIP(hipGetLastError());
     unsigned long long h = 0;
     AGWN_HIP(hipMemcpyAsync(&h, dn.p, sizeof(h), hipMemcpyDeviceToHost, s));

@@ -10,7 +10,9 @@ query q outside the ball such that the line through q along x0 misses the ball, 
     frac_L(q) = -Omega_D(q) / 4 pi,   -Omega_D(q) = a.u / d^2 + (tr M - 3 u.M.u) / d^3 + R,
     |R| <= 3 rho^2 A / (d - rho)^4,
 with x = c - q, d = |x|, u = x / d, a = the loop's vector area (1/2) sum u_i x u_i+1 (u_i = P_i - c), A the fan's area
-and M = sum over fan triangles of (centroid - c) (x) (n dA).
+and M = sum over fan triangles of (centroid - c) (x) (n dA). The expansion and the bound hold for any surface inside the
+ball, so a group of loops (cluster_table: the union of their fans, moments about a common centre) is estimated the same
+way.
 """
 from __future__ import annotations
 
@@ -78,6 +80,24 @@ def loop_table(points: np.ndarray) -> dict:
         "a": t.sum(axis=0),
         "area": float(np.sqrt((t * t).sum(axis=1)).sum()),
         "m": (g[:, :, None] * t[:, None, :]).sum(axis=0),
+    }
+
+
+def cluster_table(tabs: list, shift: bool = True) -> dict:
+    """One record for a group of loops (the library's decision clusters): centre = mean of the loops' centres, radius =
+    max |c_L - c| + rho_L (a ball holding every member's ball, hence every fan), vector area and fan area summed, first
+    moment about the new centre sum_L M_L + (c_L - c) (x) a_L. `shift=False` drops that last term, only so that tests
+    can plant the mistake."""
+    c = np.mean([t["c"] for t in tabs], axis=0)
+    m = np.zeros((3, 3))
+    for t in tabs:
+        m += t["m"] + (np.outer(t["c"] - c, t["a"]) if shift else 0.0)
+    return {
+        "c": c,
+        "rho": float(max(np.linalg.norm(t["c"] - c) + t["rho"] for t in tabs)),
+        "a": np.sum([t["a"] for t in tabs], axis=0),
+        "area": float(sum(t["area"] for t in tabs)),
+        "m": m,
     }
 
 
