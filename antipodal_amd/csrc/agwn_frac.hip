@@ -491,8 +491,10 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // Launches with too few query tiles to fill the GPU: the stream split into chunks at walk starts (cuts, blockIdx.y
-// picks the chunk; a chunk starting at a walk start carries no chain state), partial sums per chunk, added in chunk
-// order by k_chain_sum (deterministic).
+// picks the chunk), partial sums per chunk, added in chunk order by k_chain_sum (deterministic). The cuts are walk
+// starts preceded by an even number of segments: there the whole-stream chain has no half-formed pair either (the
+// pairing runs across walks), so every chunk forms the same pairs, with the same FP32 arithmetic, as
+// k_frac_f32_chain -- only the FP64 partial sums are added in another order.
 template <int Q, class Src>
 __global__ void __launch_bounds__(kBlock)
     k_frac_f32_chain_part(Src const src, int64_t n, float4 const* __restrict__ pts, int64_t const* __restrict__ cuts,
@@ -564,9 +566,12 @@ struct alignas(16) loop_rec
     float m[9];            // first moment, pole frame
     float trm;             // its trace
     float cnorm;           // |c|
-    int32_t start, count;  // the loop's stream entries
-    float pad[3];
+    int32_t start, count;  // a loop: its stream entries
+    int32_t leaf;          // 1 for a loop
+    int32_t skip;          // in the loop tree (depth-first order): the first node after this one's subtree
+    float pad;
 };
+static_assert(sizeof(loop_rec) == 96, "loop_rec layout");
 
 constexpr int kDecideBlock = 256;
 constexpr float kFourPiF = 12.566370614359172f;
@@ -668,15 +673,22 @@ __device__ __forceinline__ bool farfield_near(loop_rec const& r, float const ph[
     return false;
 }
 
-// One lane per query (FP32 positions in the caller's frame, counts = integer terms). Clusters first: a cluster whose
-// ball passes the far-field test for a lane contributes its combined estimate and bound; otherwise that lane takes its
-// loops one by one (the wave walks a cluster's loops when any lane needs them, and sums a loop exactly when any lane
-// needs that, by wave-wide votes).
+// An inner node of the loop tree is accepted only when its bound is below beta / 16 (a loop: beta). Accepted high in
+// the tree at the full beta, the summed bound grew until more points stayed undecided: the benchmark grid's step took
+// 118.8 ms that way, 112.4 ms at beta / 16 and the same within 0.2 ms from beta / 4 to beta / 32.
+constexpr float kInnerBeta = 1.0f / 16.0f;
+
+// One lane per query (FP32 positions in the caller's frame, counts = integer terms). The wave walks the loop tree in
+// depth-first order without a stack: a lane whose far-field test passes at node k takes that node's estimate and
+// bound for every loop below it and resumes at skip[k]; a lane that fails goes on to k + 1 (the first child; at a
+// loop, it sums the loop exactly -- one exact sum per wave, staged through the wave's LDS slots). The wave visits
+// k + 1 when any lane failed at k, else skip[k]: every lane that stopped at an ancestor resumes at that ancestor's
+// skip, which is no earlier, so the lanes active at a visited node are exactly those whose resume index is that node.
 __global__ void __launch_bounds__(kDecideBlock)
     k_frac_decide(float const* __restrict__ pos, int64_t n, int32_t const* __restrict__ counts,
-                  loop_rec const* __restrict__ clusters, int nclusters, loop_rec const* __restrict__ loops,
-                  float4 const* __restrict__ pts, frame R, pole P, float tau, float beta4pi, double margin,
-                  float* __restrict__ val, int64_t* __restrict__ undecided, unsigned long long* __restrict__ nund)
+                  loop_rec const* __restrict__ tree, int nnodes, float4 const* __restrict__ pts, frame R, pole P,
+                  float tau, float beta4pi, double margin, float* __restrict__ val, int64_t* __restrict__ undecided,
+                  unsigned long long* __restrict__ nund)
 {
     __shared__ float4 stage[kDecideBlock / 64][128];
     int const lane = int(threadIdx.x & 63);
@@ -686,33 +698,33 @@ __global__ void __launch_bounds__(kDecideBlock)
     float ph[3], pl[3];
     query_split(src_array<float>{pos}, i, n, R, ph, pl);
     double fe = 0.0;              // exact loops (radians)
-    float est = 0.0f, bnd = 0.0f; // far clusters and loops: -Omega estimate and remainder bound (steradians)
-    for (int C = 0; C < nclusters; ++C)
+    float est = 0.0f, bnd = 0.0f; // accepted nodes: -Omega estimate and remainder bound (steradians)
+    int resume = valid ? 0 : INT32_MAX;
+    int k = 0;
+    while (k < nnodes)
     {
-        loop_rec const cr = clusters[C];
-        bool const split = farfield_near(cr, ph, beta4pi, est, bnd) && valid;
-        if (!__any(split))
-            continue;
-        for (int L = cr.start; L < cr.start + cr.count; ++L)
+        loop_rec const r = tree[k];
+        bool split = false;
+        if (resume == k)
         {
-            loop_rec const r = loops[L];
-            bool near = false;
-            if (split)
-                near = farfield_near(r, ph, beta4pi, est, bnd);
-            if (__any(near))
-            {
-                double const sl = loop_exact(pts + 2 * int64_t(r.start), r.count, ph, pl, P, tau, st, lane);
-                if (near)
-                    fe += sl;
-            }
+            split = farfield_near(r, ph, r.leaf ? beta4pi : beta4pi * kInnerBeta, est, bnd);
+            resume = split ? k + 1 : r.skip;
         }
+        bool const down = __any(split);
+        if (r.leaf && down)
+        {
+            double const sl = loop_exact(pts + 2 * int64_t(r.start), r.count, ph, pl, P, tau, st, lane);
+            if (split)
+                fe += sl;
+        }
+        k = down ? k + 1 : r.skip;
     }
     if (valid)
     {
-        double const w = double(counts[i]) + fe / kTwoPi + double(est) / (4.0 * 3.141592653589793);
+        double const wv = double(counts[i]) + fe / kTwoPi + double(est) / (4.0 * 3.141592653589793);
         double const B = double(bnd) / (4.0 * 3.141592653589793) * 1.01 + margin;
-        if (fabs(w - 0.5) > B)
-            val[i] = float(w);
+        if (fabs(wv - 0.5) > B)
+            val[i] = float(wv);
         else
             undecided[atomicAdd(nund, 1ull)] = i;
     }
@@ -961,17 +973,25 @@ bool loops_of_stream(std::vector<float4> const& p, std::vector<loop_host>& loops
     return true;
 }
 
-// Clusters of the decision's loops: the loops in the Morton order of their centres (10 bits per axis over the box of
-// the centres), taken kClusterLoops at a time. A cluster's ball holds its members' balls (centre: the mean of theirs,
-// radius: max |c_L - c| + rho_L), so it holds their fans; its vector area and fan area are the members' sums and its
-// first moment, about its centre, sum_L M_L + (c_L - c) (x) a_L. The far-field expansion and bound hold for any surface
-// inside the ball, so a cluster is tested and estimated like one loop. Returns the clusters (start, count: their
-// members' range in `order`) and fills `order` with the loop indices.
-constexpr int kClusterLoops = 16;
+// The loop tree of the decision: the loops in the Morton order of their centres (10 bits per axis over the box of the
+// centres), halved recursively. A node's ball is the smallest one holding its children's balls (so it holds every fan
+// below it); its vector area and fan area are the children's sums and its first moment, about its centre,
+// sum_child M_child + (c_child - c) (x) a_child. The far-field expansion and bound hold for any surface inside the
+// ball, so a node is tested and estimated like one loop. Nodes in depth-first order, the root first, each with the
+// index just past its subtree (skip). Returns the depth (root = 1).
+struct tree_node
+{
+    loop_host L;
+    int32_t left, right, loop; // loop >= 0: a leaf
+    int32_t skip;              // the first node after this one's subtree
+};
 
-std::vector<loop_host> loop_clusters(std::vector<loop_host> const& loops, std::vector<int32_t>& order)
+int loop_tree(std::vector<loop_host> const& loops, std::vector<tree_node>& nodes)
 {
     size_t const n = loops.size();
+    nodes.clear();
+    if (n == 0)
+        return 0;
     double lo[3] = {HUGE_VAL, HUGE_VAL, HUGE_VAL}, hi[3] = {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
     for (loop_host const& L : loops)
         for (int a = 0; a < 3; ++a)
@@ -1000,35 +1020,67 @@ std::vector<loop_host> loop_clusters(std::vector<loop_host> const& loops, std::v
         key[k] = {code, int32_t(k)};
     }
     std::sort(key.begin(), key.end());
-    order.resize(n);
-    for (size_t k = 0; k < n; ++k)
-        order[k] = key[k].second;
-    std::vector<loop_host> out;
-    for (size_t k0 = 0; k0 < n; k0 += kClusterLoops)
-    {
-        size_t const k1 = std::min(n, k0 + kClusterLoops);
-        loop_host C{};
-        for (size_t k = k0; k < k1; ++k)
-            for (int a = 0; a < 3; ++a)
-                C.c[a] += loops[size_t(order[k])].c[a] / double(k1 - k0);
-        for (size_t k = k0; k < k1; ++k)
+    nodes.reserve(2 * n - 1);
+    int depth = 0;
+    // builds the node of sorted range [b, e) at the back of `nodes`; returns its index
+    auto build = [&](auto&& self, size_t b, size_t e, int d) -> int32_t {
+        depth = std::max(depth, d);
+        int32_t const me = int32_t(nodes.size());
+        nodes.push_back(tree_node{});
+        if (e - b == 1)
         {
-            loop_host const& L = loops[size_t(order[k])];
-            double const dc[3] = {L.c[0] - C.c[0], L.c[1] - C.c[1], L.c[2] - C.c[2]};
-            C.rho = std::max(C.rho, std::sqrt(dc[0] * dc[0] + dc[1] * dc[1] + dc[2] * dc[2]) + L.rho);
-            C.area += L.area;
-            for (int r = 0; r < 3; ++r)
+            nodes[size_t(me)].L = loops[size_t(key[b].second)];
+            nodes[size_t(me)].left = nodes[size_t(me)].right = -1;
+            nodes[size_t(me)].loop = key[b].second;
+            nodes[size_t(me)].skip = me + 1;
+            return me;
+        }
+        size_t const mid = b + (e - b) / 2;
+        int32_t const l = self(self, b, mid, d + 1);
+        int32_t const r = self(self, mid, e, d + 1);
+        loop_host const& A = nodes[size_t(l)].L;
+        loop_host const& B = nodes[size_t(r)].L;
+        loop_host C{};
+        double const dv[3] = {B.c[0] - A.c[0], B.c[1] - A.c[1], B.c[2] - A.c[2]};
+        double const D = std::sqrt(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2]);
+        if (D + B.rho <= A.rho) // A's ball holds B's
+        {
+            std::copy(A.c, A.c + 3, C.c);
+            C.rho = A.rho;
+        }
+        else if (D + A.rho <= B.rho)
+        {
+            std::copy(B.c, B.c + 3, C.c);
+            C.rho = B.rho;
+        }
+        else
+        {
+            double const t = (D + B.rho - A.rho) / (2.0 * D);
+            for (int k = 0; k < 3; ++k)
+                C.c[k] = A.c[k] + t * dv[k];
+            C.rho = 0.5 * (D + A.rho + B.rho);
+        }
+        for (loop_host const* K : {&A, &B}) // guard the rounding of the centre
+        {
+            double const dc[3] = {K->c[0] - C.c[0], K->c[1] - C.c[1], K->c[2] - C.c[2]};
+            C.rho = std::max(C.rho, std::sqrt(dc[0] * dc[0] + dc[1] * dc[1] + dc[2] * dc[2]) + K->rho);
+            C.area += K->area;
+            for (int a = 0; a < 3; ++a)
             {
-                C.a[r] += L.a[r];
+                C.a[a] += K->a[a];
                 for (int j = 0; j < 3; ++j)
-                    C.m[3 * r + j] += L.m[3 * r + j] + dc[r] * L.a[j];
+                    C.m[3 * a + j] += K->m[3 * a + j] + dc[a] * K->a[j];
             }
         }
-        C.start = int64_t(k0);
-        C.count = int64_t(k1 - k0);
-        out.push_back(C);
-    }
-    return out;
+        nodes[size_t(me)].L = C;
+        nodes[size_t(me)].left = l;
+        nodes[size_t(me)].right = r;
+        nodes[size_t(me)].loop = -1;
+        nodes[size_t(me)].skip = int32_t(nodes.size());
+        return me;
+    };
+    build(build, 0, n, 1);
+    return depth;
 }
 
 int frac_plan_build(float const* h, int64_t nseg, frac_plan& plan, hipStream_t stream)
@@ -1046,9 +1098,17 @@ int frac_plan_build(float const* h, int64_t nseg, frac_plan& plan, hipStream_t s
     plan.npts = int64_t(stream_pts.size());
     plan.nunit = nunit;
     plan.walk_starts.clear();
+    int64_t segs = 0; // segments before entry i
     for (int64_t i = 0; i < plan.npts; ++i)
+    {
         if (stream_pts[size_t(i)].w == 0.0f)
-            plan.walk_starts.push_back(i);
+        {
+            if (segs % 2 == 0)
+                plan.walk_starts.push_back(i);
+        }
+        else
+            ++segs;
+    }
     std::vector<float4> dec;
     plan.loops_ok = loops_of_stream(stream_pts, plan.loops, dec);
     if (!plan.loops_ok)
@@ -1118,15 +1178,24 @@ int frac_decide(float const* pos, int64_t m, int32_t const* counts, frac_plan co
         r.count = int32_t(count);
         return r;
     };
-    std::vector<int32_t> order;
-    std::vector<loop_host> cl = loop_clusters(plan.loops, order);
-    std::vector<loop_rec> tab(plan.loops.size() + cl.size());
-    for (size_t k = 0; k < cl.size(); ++k)
-        tab[k] = to_rec(cl[k], cl[k].start, cl[k].count);
-    for (size_t k = 0; k < order.size(); ++k)
+    std::vector<tree_node> nodes;
+    loop_tree(plan.loops, nodes);
+    std::vector<loop_rec> tab(nodes.size());
+    for (size_t k = 0; k < nodes.size(); ++k)
     {
-        loop_host const& L = plan.loops[size_t(order[k])];
-        tab[cl.size() + k] = to_rec(L, L.start, L.count);
+        tree_node const& T = nodes[k];
+        if (T.loop >= 0)
+        {
+            loop_host const& L = plan.loops[size_t(T.loop)];
+            tab[k] = to_rec(L, L.start, L.count);
+            tab[k].leaf = 1;
+        }
+        else
+        {
+            tab[k] = to_rec(T.L, 0, 0);
+            tab[k].leaf = 0;
+        }
+        tab[k].skip = T.skip;
     }
     dbuf dt, p32, dn;
     AGWN_TRY(dt.alloc(std::max<size_t>(tab.size(), 1) * sizeof(loop_rec), s));
@@ -1144,8 +1213,8 @@ int frac_decide(float const* pos, int64_t m, int32_t const* counts, frac_plan co
     int64_t const blocks = (m + kDecideBlock - 1) / kDecideBlock;
     AGWN_CHECK(blocks < (int64_t(1) << 31), "frac_decide: too many queries (%lld)", (long long)m);
     k_frac_decide<<<dim3(unsigned(blocks)), dim3(kDecideBlock), 0, s>>>(
-        pos, m, counts, dt.as<loop_rec>(), int(cl.size()), dt.as<loop_rec>() + cl.size(), p32.as<float4>(), R, P, kTau,
-        beta * kFourPiF, kDecideMargin, val, undecided, dn.as<unsigned long long>());
+        pos, m, counts, dt.as<loop_rec>(), int(tab.size()), p32.as<float4>(), R, P, kTau, beta * kFourPiF,
+        kDecideMargin, val, undecided, dn.as<unsigned long long>());
     AGWN_HIP(hipGetLastError());
     unsigned long long h = 0;
     AGWN_HIP(hipMemcpyAsync(&h, dn.p, sizeof(h), hipMemcpyDeviceToHost, s));

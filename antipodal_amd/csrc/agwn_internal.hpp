@@ -188,7 +188,8 @@ struct frac_plan
     int64_t npts = 0;   // stream entries
     int64_t nunit = 0;  // unit segments
     bool chain = false;
-    std::vector<int64_t> walk_starts; // stream entries with flag 0 (host copy; chunking of small launches)
+    std::vector<int64_t> walk_starts; // stream entries with flag 0 after an even number of segments (host copy;
+                                      // the cuts of small launches)
     std::vector<loop_host> loops; // closed loops (long walks split into arcs + a skeleton), in the decision stream
     bool loops_ok = false;
     dbuf dec;                     // decision stream: the loops' points, each loop contiguous (flag 0 at its start)

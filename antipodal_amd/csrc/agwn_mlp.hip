@@ -451,18 +451,22 @@ __global__ void __launch_bounds__(kB) k_perm(int64_t* __restrict__ perm, int64_t
 }
 
 // ---------------------------------------------------------------------------------------------------------------------
-// Fused inference: a workgroup takes 128 points through the features, the 4 hidden layers and the output layer with
-// the activations in one LDS buffer (128 x 264 BF16, 66 KiB: two workgroups per CU). Each layer computes
+// Fused inference: a workgroup takes kTileP = 96 points through the features, the 4 hidden layers and the output
+// layer with the activations in one LDS buffer (96 x 264 BF16, 50 KiB: three workgroups per CU -- 128-point tiles,
+// two per CU, measured 1.5 % slower, 64-point tiles 2 % slower). Each layer computes
 // out^T[n][m] = sum_k Wt[n][k] act[m][k]: wave w owns units [64w, 64w + 64) (A operand = Wt rows in fragment order,
-// from L2) for all 128
-// points (B operand = LDS rows), so a lane's 4 consecutive accumulators are 4 consecutive units of one point and go
-// back to LDS as one 8-byte write after a barrier. The output layer is a 256-long dot per point.
+// from L2) for all the tile's points (B operand = LDS rows); the accumulators start at the biases, so a lane's 4
+// consecutive accumulators are 4 consecutive units of one point, activated and written back to LDS as one 8-byte
+// write after a barrier. The output layer is a 256-long dot per point.
 // Optionally labels the points (pred > 0.5) and appends to a routing list (entry = 2 * index + (audit ? 1 : 0)) the
 // points in the near-surface mask or outside the box, those with |pred - 0.5| <= delta, and a hashed audit sample of
 // the rest.
 // ---------------------------------------------------------------------------------------------------------------------
-constexpr int kTileP = 128;
 constexpr int kLdsStride = kH + 8; // BF16 per LDS row: 528 B rows keep the 16-lane groups of ds_read_b128 conflict-free
+constexpr int kTileP = 96;          // points per workgroup (a multiple of 32)
+constexpr int kTileJ = kTileP / 32; // 32-point MFMA column blocks per tile
+constexpr int kInferBlocks = (160 * 1024) / (kTileP * kLdsStride * 2 + 1024); // workgroups per CU that the LDS holds
+static_assert(kTileP % 32 == 0 && kInferBlocks >= 1, "tile shape");
 
 struct infer_params
 {
@@ -588,7 +592,7 @@ __device__ __forceinline__ uint32_t hash32(uint64_t x)
 }
 
 template <class Src, bool ROUTE>
-__global__ void __launch_bounds__(256, 2) k_mlp_infer(Src const src, int64_t n, feat_params F, infer_params P,
+__global__ void __launch_bounds__(256, kInferBlocks) k_mlp_infer(Src const src, int64_t n, feat_params F, infer_params P,
                                                       float* __restrict__ pred, route_params RP)
 {
     __shared__ __attribute__((aligned(16))) __bf16 act[kTileP * kLdsStride];
@@ -597,18 +601,20 @@ __global__ void __launch_bounds__(256, 2) k_mlp_infer(Src const src, int64_t n, 
     int const wave = threadIdx.x >> 6;
     int64_t const base = int64_t(blockIdx.x) * kTileP;
 
+    // features: a point's 96 pairs in 8 parts of 12; work item (point p, part) -> thread item % 256 (threads of a wave
+    // = consecutive points), each part written 4 pairs (16 B) at a time -- the rows' 528 B stride keeps each 16-lane
+    // group of those stores on distinct banks
+    for (int item = threadIdx.x; item < kTileP * 8; item += 256)
     {
-        // features: two lanes per point (lanes of a wave = consecutive points), each 48 pairs written 4 pairs (16 B)
-        // at a time -- the rows' 528 B stride keeps each 16-lane group of those stores on distinct banks
-        int const p = threadIdx.x % kTileP, half = threadIdx.x / kTileP;
+        int const p = item % kTileP, part = item / kTileP;
         int64_t const i = base + p;
         float x = 0.0f, y = 0.0f, z = 0.0f;
         bool const live = i < n;
         if (live)
             query_at(src, i, x, y, z);
-        for (int g = 0; g < kIn / 16; ++g)
+        for (int g = 0; g < 3; ++g)
         {
-            int const j0 = half * (kIn / 4) + 4 * g;
+            int const j0 = 12 * part + 4 * g;
             bf16x8 v;
 #pragma unroll
             for (int q = 0; q < 4; ++q)
@@ -634,7 +640,22 @@ __global__ void __launch_bounds__(256, 2) k_mlp_infer(Src const src, int64_t n, 
         // fragments of unit blocks 2w and 2w + 1 (units n0 .. n0 + 63); k step st at + 512 st
         __bf16 const* wa0 = P.wf[l] + (int64_t(n0 / 32) * (K / 16) * 64 + lane) * 8;
         __bf16 const* wa1 = wa0 + int64_t(K / 16) * 512;
-        f32x16 acc[2][4] = {};
+        f32x16 acc[2][kTileJ]; // starts at the biases of the lane's units (C[r][c]: unit r = 8(e/4) + 4(l/32) + e%4)
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+            {
+                float4 const bq = *reinterpret_cast<float4 const*>(P.b[l] + n0 + i * 32 + 8 * q + 4 * (lane >> 5));
+#pragma unroll
+                for (int j = 0; j < kTileJ; ++j)
+                {
+                    acc[i][j][4 * q] = bq.x;
+                    acc[i][j][4 * q + 1] = bq.y;
+                    acc[i][j][4 * q + 2] = bq.z;
+                    acc[i][j][4 * q + 3] = bq.w;
+                }
+            }
         bf16x8 a0 = *reinterpret_cast<bf16x8 const*>(wa0);
         bf16x8 a1 = *reinterpret_cast<bf16x8 const*>(wa1);
         for (int k0 = 0; k0 < K; k0 += 16)
@@ -643,7 +664,7 @@ __global__ void __launch_bounds__(256, 2) k_mlp_infer(Src const src, int64_t n, 
             bf16x8 const na0 = *reinterpret_cast<bf16x8 const*>(wa0 + int64_t(kn) * 32);
             bf16x8 const na1 = *reinterpret_cast<bf16x8 const*>(wa1 + int64_t(kn) * 32);
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
+            for (int j = 0; j < kTileJ; ++j)
             {
                 bf16x8 const b = *reinterpret_cast<bf16x8 const*>(act + (j * 32 + li) * kLdsStride + k0 + kh);
                 acc[0][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b, acc[0][j], 0, 0, 0);
@@ -659,36 +680,38 @@ __global__ void __launch_bounds__(256, 2) k_mlp_infer(Src const src, int64_t n, 
             for (int q = 0; q < 4; ++q)
             {
                 int const u = n0 + i * 32 + 8 * q + 4 * (lane >> 5); // units u .. u+3
-                float const b0 = P.b[l][u], b1 = P.b[l][u + 1], b2 = P.b[l][u + 2], b3 = P.b[l][u + 3];
 #pragma unroll
-                for (int j = 0; j < 4; ++j)
+                for (int j = 0; j < kTileJ; ++j)
                 {
                     bf16x4 hv;
-                    hv[0] = __bf16(hsilu(acc[i][j][4 * q] + b0));
-                    hv[1] = __bf16(hsilu(acc[i][j][4 * q + 1] + b1));
-                    hv[2] = __bf16(hsilu(acc[i][j][4 * q + 2] + b2));
-                    hv[3] = __bf16(hsilu(acc[i][j][4 * q + 3] + b3));
+                    hv[0] = __bf16(hsilu(acc[i][j][4 * q]));
+                    hv[1] = __bf16(hsilu(acc[i][j][4 * q + 1]));
+                    hv[2] = __bf16(hsilu(acc[i][j][4 * q + 2]));
+                    hv[3] = __bf16(hsilu(acc[i][j][4 * q + 3]));
                     *reinterpret_cast<bf16x4*>(act + (j * 32 + li) * kLdsStride + u) = hv;
                 }
             }
         __syncthreads();
     }
     // output layer: two lanes per point (lanes = consecutive points: 16 B LDS reads conflict-free per 16-lane group),
-    // each a 128-unit dot product in FP32 (the weights are uniform: scalar loads), halves added through LDS
+    // each a 128-unit dot product in FP32, halves added through LDS
     {
         int const p = threadIdx.x % kTileP, half = threadIdx.x / kTileP;
-        float const* wo = P.wo + half * (kH / 2);
-        __bf16 const* row = act + p * kLdsStride + half * (kH / 2);
         float s0 = 0.0f, s1 = 0.0f;
-#pragma unroll
-        for (int g = 0; g < kH / 16; ++g)
+        if (half < 2)
         {
-            bf16x8 const hv = *reinterpret_cast<bf16x8 const*>(row + 8 * g);
+            float const* wo = P.wo + half * (kH / 2);
+            __bf16 const* row = act + p * kLdsStride + half * (kH / 2);
 #pragma unroll
-            for (int q = 0; q < 8; q += 2)
+            for (int g = 0; g < kH / 16; ++g)
             {
-                s0 = fmaf(float(hv[q]), wo[8 * g + q], s0);
-                s1 = fmaf(float(hv[q + 1]), wo[8 * g + q + 1], s1);
+                bf16x8 const hv = *reinterpret_cast<bf16x8 const*>(row + 8 * g);
+#pragma unroll
+                for (int q = 0; q < 8; q += 2)
+                {
+                    s0 = fmaf(float(hv[q]), wo[8 * g + q], s0);
+                    s1 = fmaf(float(hv[q + 1]), wo[8 * g + q + 1], s1);
+                }
             }
         }
         if (half == 1)

@@ -84,10 +84,11 @@ def loop_table(points: np.ndarray) -> dict:
 
 
 def cluster_table(tabs: list, shift: bool = True) -> dict:
-    """One record for a group of loops (the library's decision clusters): centre = mean of the loops' centres, radius =
-    max |c_L - c| + rho_L (a ball holding every member's ball, hence every fan), vector area and fan area summed, first
-    moment about the new centre sum_L M_L + (c_L - c) (x) a_L. `shift=False` drops that last term, only so that tests
-    can plant the mistake."""
+    """One record for a group of loops about their mean centre: radius = max |c_L - c| + rho_L (a ball holding every
+    member's ball, hence every fan), vector area and fan area summed, first moment about the new centre
+    sum_L M_L + (c_L - c) (x) a_L -- the combination rule of the library's loop-tree nodes (merge_tables, which picks
+    the smallest enclosing ball of two children instead of the mean centre). `shift=False` drops the shift term, only
+    so that tests can plant the mistake."""
     c = np.mean([t["c"] for t in tabs], axis=0)
     m = np.zeros((3, 3))
     for t in tabs:
@@ -99,6 +100,25 @@ def cluster_table(tabs: list, shift: bool = True) -> dict:
         "area": float(sum(t["area"] for t in tabs)),
         "m": m,
     }
+
+
+def merge_tables(ta: dict, tb: dict) -> dict:
+    """The library's loop-tree node over two children: the smallest ball holding both children's balls, vector areas
+    and fan areas summed, first moments shifted to the new centre (M_child + (c_child - c) (x) a_child)."""
+    dv = tb["c"] - ta["c"]
+    dist = float(np.linalg.norm(dv))
+    if dist + tb["rho"] <= ta["rho"]:
+        c, rho = ta["c"].copy(), ta["rho"]
+    elif dist + ta["rho"] <= tb["rho"]:
+        c, rho = tb["c"].copy(), tb["rho"]
+    else:
+        c = ta["c"] + (dist + tb["rho"] - ta["rho"]) / (2.0 * dist) * dv
+        rho = 0.5 * (dist + ta["rho"] + tb["rho"])
+    m = np.zeros((3, 3))
+    for t in (ta, tb):
+        rho = max(rho, float(np.linalg.norm(t["c"] - c)) + t["rho"])
+        m += t["m"] + np.outer(t["c"] - c, t["a"])
+    return {"c": c, "rho": rho, "a": ta["a"] + tb["a"], "area": ta["area"] + tb["area"], "m": m}
 
 
 def loop_segments(points: np.ndarray) -> np.ndarray:

@@ -5,9 +5,10 @@ Bars: for every closed loop and every query where the expansion applies, |frac_L
 worst ratio ~1e-1); the split of a long loop into chord-closed arcs plus a skeleton sums to the loop (1e-9) and each
 piece keeps the bound; a planted sign error in the dipole term breaks it; on the axis of a planar 4096-gon the oracle's
 loop sum equals the disk's closed-form solid angle / 4 pi (to 1e-5) and the estimate approaches it as 1/z^4. Clusters
-of 16 nearby loops (one combined record, as the library's decision uses them) keep the bound on the torus; a cluster of
-one loop is that loop, and a cluster's estimate and its members' estimates agree to O(d^-4) -- O(d^-3) when the shift
-of the first moments to the common centre is planted out.
+of 16 nearby loops (one combined record) keep the bound on the torus; a cluster of one loop is that loop, and a
+cluster's estimate and its members' estimates agree to O(d^-4) -- O(d^-3) when the shift of the first moments to the
+common centre is planted out. Every inner node of a binary tree over the torus's loops, merged two children at a time
+as the library's decision tree is (merge_tables), keeps the bound.
 """
 from __future__ import annotations
 
@@ -110,7 +111,7 @@ def test_disk_axis_closed_form(oracle):
 
 def clusters_of(walks, size=16):
     """Pieces of the split walks, grouped `size` at a time in the order of their centres along a space-filling sort
-    (coarse voxel, then x): nearby loops share a cluster, as in the library."""
+    (coarse voxel, then x): nearby loops share a cluster."""
     pieces = [p for w in walks for p in ff.split_loop(w, 32)]
     tabs = [ff.loop_table(p) for p in pieces]
     cs = np.array([t["c"] for t in tabs])
@@ -175,3 +176,44 @@ def test_cluster_of_one_is_the_loop_and_far_clusters_add_up():
     assert g[0] / g[1] > 14.0 and g[1] / g[2] > 14.0 and g[0] < 10.0 / 40.0 ** 4
     b = gaps(False)
     assert 7.0 < b[0] / b[1] < 9.0 and 7.0 < b[1] / b[2] < 9.0
+
+
+def test_tree_nodes_keep_the_bound_on_the_torus(oracle, torus_walks):
+    """Every node of a binary tree over the torus's loop pieces (sorted along a coarse space-filling order, halved
+    recursively, merged as the library merges them) keeps the bound for every applicable query, and its ball holds
+    its children's."""
+    walks, q = torus_walks
+    x0 = mg.example_x0(np.float64)
+    rng = np.random.default_rng(6)
+    q = np.concatenate([q, rng.uniform(-2.0, 2.0, (24, 3))])
+    pieces = [p for w in walks for p in ff.split_loop(w, 32)]
+    tabs = [ff.loop_table(p) for p in pieces]
+    cs = np.array([t["c"] for t in tabs])
+    lo, hi = cs.min(axis=0), cs.max(axis=0)
+    vox = np.floor((cs - lo) / np.maximum(hi - lo, 1e-12) * 8).clip(0, 7)
+    order = list(np.lexsort((cs[:, 0], vox[:, 0], vox[:, 1], vox[:, 2])))
+    # frac of every piece at every query, once
+    fr = np.stack([oracle.frac(q, ff.loop_segments(p), x0) for p in pieces])
+    cases, worst, nodes = 0, 0.0, 0
+
+    def build(idx):
+        nonlocal cases, worst, nodes
+        if len(idx) == 1:
+            return tabs[idx[0]]
+        mid = len(idx) // 2
+        ta, tb = build(idx[:mid]), build(idx[mid:])
+        t = ff.merge_tables(ta, tb)
+        for k in (ta, tb):
+            assert np.linalg.norm(k["c"] - t["c"]) + k["rho"] <= t["rho"] * (1 + 1e-12)
+        ok = ff.applicable(t, q, x0)
+        nodes += 1
+        if ok.any():
+            est, b = ff.estimate(t, q[ok])
+            err = np.abs(fr[idx][:, ok].sum(axis=0) - est)
+            worst = max(worst, float((err / b).max()))
+            cases += int(ok.sum())
+        return t
+
+    build(order)
+    print(f"tree: {nodes} inner nodes, {cases} (query, node) cases, worst |frac - estimate| / bound = {worst:.3e}")
+    assert nodes == len(pieces) - 1 and cases > 2000 and worst <= 1.0
