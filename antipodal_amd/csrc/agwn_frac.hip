@@ -978,20 +978,20 @@ bool loops_of_stream(std::vector<float4> const& p, std::vector<loop_host>& loops
 // below it); its vector area and fan area are the children's sums and its first moment, about its centre,
 // sum_child M_child + (c_child - c) (x) a_child. The far-field expansion and bound hold for any surface inside the
 // ball, so a node is tested and estimated like one loop. Nodes in depth-first order, the root first, each with the
-// index just past its subtree (skip). Returns the depth (root = 1).
+// index just past its subtree (skip).
 struct tree_node
 {
     loop_host L;
-    int32_t left, right, loop; // loop >= 0: a leaf
-    int32_t skip;              // the first node after this one's subtree
+    int32_t loop; // >= 0: a leaf, the loop's index
+    int32_t skip; // the first node after this one's subtree
 };
 
-int loop_tree(std::vector<loop_host> const& loops, std::vector<tree_node>& nodes)
+void loop_tree(std::vector<loop_host> const& loops, std::vector<tree_node>& nodes)
 {
     size_t const n = loops.size();
     nodes.clear();
     if (n == 0)
-        return 0;
+        return;
     double lo[3] = {HUGE_VAL, HUGE_VAL, HUGE_VAL}, hi[3] = {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
     for (loop_host const& L : loops)
         for (int a = 0; a < 3; ++a)
@@ -1021,23 +1021,20 @@ int loop_tree(std::vector<loop_host> const& loops, std::vector<tree_node>& nodes
     }
     std::sort(key.begin(), key.end());
     nodes.reserve(2 * n - 1);
-    int depth = 0;
     // builds the node of sorted range [b, e) at the back of `nodes`; returns its index
-    auto build = [&](auto&& self, size_t b, size_t e, int d) -> int32_t {
-        depth = std::max(depth, d);
+    auto build = [&](auto&& self, size_t b, size_t e) -> int32_t {
         int32_t const me = int32_t(nodes.size());
         nodes.push_back(tree_node{});
         if (e - b == 1)
         {
             nodes[size_t(me)].L = loops[size_t(key[b].second)];
-            nodes[size_t(me)].left = nodes[size_t(me)].right = -1;
             nodes[size_t(me)].loop = key[b].second;
             nodes[size_t(me)].skip = me + 1;
             return me;
         }
         size_t const mid = b + (e - b) / 2;
-        int32_t const l = self(self, b, mid, d + 1);
-        int32_t const r = self(self, mid, e, d + 1);
+        int32_t const l = self(self, b, mid);
+        int32_t const r = self(self, mid, e);
         loop_host const& A = nodes[size_t(l)].L;
         loop_host const& B = nodes[size_t(r)].L;
         loop_host C{};
@@ -1073,14 +1070,11 @@ int loop_tree(std::vector<loop_host> const& loops, std::vector<tree_node>& nodes
             }
         }
         nodes[size_t(me)].L = C;
-        nodes[size_t(me)].left = l;
-        nodes[size_t(me)].right = r;
         nodes[size_t(me)].loop = -1;
         nodes[size_t(me)].skip = int32_t(nodes.size());
         return me;
     };
-    build(build, 0, n, 1);
-    return depth;
+    build(build, 0, n);
 }
 
 int frac_plan_build(float const* h, int64_t nseg, frac_plan& plan, hipStream_t stream)
