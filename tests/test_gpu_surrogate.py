@@ -154,6 +154,46 @@ def test_grid_labels_zero_mislabels(gpu, c1, trained_c1):
         assert torch.equal(lab_all, exact)
 
 
+def test_grid_labels_follow_the_routing_rule(gpu, c1, trained_c1):
+    """On a 256^3 grid (2^24 queries) every label is what the routing rule gives -- the exact label where the point is
+    in the near-surface mask or in the band |pred - 0.5| <= delta, the network's elsewhere (pred from the prediction
+    entry on the same float32 lattice points) -- and the stats count those sets; with delta = +inf every point is
+    routed and the labels are the exact path's bit for bit. (This network was calibrated on 2^16 points, so on a grid
+    64 times the 64^3 one a few confident network errors are expected: they are counted, not failed.)"""
+    import torch
+
+    v, idx, lo, hi = c1
+    net, delta, info, bnd, x0 = trained_c1
+    nres = 256
+    start, dx, dy, dz = mg.grid_axes(lo, hi, nres)
+    f = np.float32
+    r = np.arange(nres, dtype=f)
+    s0, a, b, c = (np.asarray(t, dtype=f) for t in (start, dx, dy, dz))
+    pts = (((s0[None, None, None, :] + a * r[None, None, :, None]) + b * r[None, :, None, None])
+           + c * r[:, None, None, None]).astype(f).reshape(-1, 3)  # the kernels' float32 lattice, x fastest
+    P = torch.from_numpy(pts).cuda()
+    with gpu.HipGridGwn() as grid:
+        grid.set_scene(v.astype(np.float32), idx, bnd)
+        w = torch.empty(nres ** 3, dtype=torch.float32, device="cuda")
+        grid.eval_grid(start, dx, dy, dz, nres, nres, nres, out=w)
+        exact = (w > 0.5).to(torch.int8)
+        near = net.near_surface(P).bool()
+        pred = net.predict(P)
+        band = ~near & ((pred - 0.5).abs() <= delta)
+        want = torch.where(near | band, exact, (pred > 0.5).to(torch.int8))
+        lab, st = net.label_grid(grid, start, dx, dy, dz, nres, nres, nres, delta)
+        torch.cuda.synchronize()
+        wrong = int((lab != exact).sum())
+        print(f"grid {nres}^3: stats {st}, network mislabels {wrong}")
+        assert torch.equal(lab, want)
+        assert st["near"] == int(near.sum()) and st["band"] == int(band.sum()) and st["audit"] == 0
+        assert st["routed"] == st["near"] + st["band"]
+        assert wrong < 1e-5 * nres ** 3
+        lab_all, st_all = net.label_grid(grid, start, dx, dy, dz, nres, nres, nres, float("inf"))
+        assert st_all["routed"] == nres ** 3
+        assert torch.equal(lab_all, exact)
+
+
 def test_point_labels_and_audit(gpu, c1, trained_c1):
     import torch
 
