@@ -647,8 +647,8 @@ __device__ double loop_exact(float4 const* __restrict__ p, int count, float cons
     return acc;
 }
 
-// Far-field test of one record (a loop, or a cluster of loops) at the query: true when it must be summed exactly (or,
-// for a cluster, split into its loops); otherwise its estimate and bound are added to est and bnd (steradians).
+// Far-field test of one record (a loop, or a node of the loop tree) at the query: true when it must be summed exactly
+// (a node: descended into); otherwise its estimate and bound are added to est and bnd (steradians).
 __device__ __forceinline__ bool farfield_near(loop_rec const& r, float const ph[3], float beta4pi, float& est,
                                               float& bnd)
 {
