@@ -233,12 +233,13 @@ int mlp_route(agwn_mlp const* mlp, Src const& src, int64_t n, float delta, uint3
               int8_t* labels, int64_t* list, unsigned long long* counters, hipStream_t s);
 template <class Src>
 int gather_routed(Src const& src, int64_t const* list, int64_t m, float* pts, hipStream_t s);
-//   route_order    orders the m list entries of n queries spatially (Morton code of the query in the network's box
-//                  above the entry; index order when the key would not fit 64 bits); tmp: m uint64 of scratch;
-//                  *ordered = list or tmp
+//   route_order    orders the m list entries of n queries: *by_index = the entries in index order (list or tmp);
+//                  with perm, *perm[e] = the position in *by_index of the e-th entry in the Morton order of the
+//                  queries in the network's box (null when position and code would not fit 64 bits: index order);
+//                  tmp, keys, keys2: m uint64 of scratch each
 template <class Src>
 int route_order(agwn_mlp const* mlp, Src const& src, int64_t n, int64_t* list, int64_t m, uint64_t* tmp,
-                hipStream_t s, int64_t const** ordered);
+                uint64_t* keys, uint64_t* keys2, hipStream_t s, int64_t const** by_index, uint64_t const** perm);
 // LSD radix sort of n uint64 keys over their low `bits` bits (agwn_boundary.hip); *sorted = a or b
 int radix_sort_u64(uint64_t* a, uint64_t* b, int64_t n, int bits, hipStream_t s, uint64_t** sorted);
 int route_apply(int64_t const* list, int64_t m, float const* w, int8_t* labels, unsigned long long* counters,

@@ -302,19 +302,123 @@ __device__ __forceinline__ int count_one(T px, T py, T pz, T dx, T dy, T dz, bvh
     return sum;
 }
 
+// The same count for a wave of rays at once (packet traversal): the wave walks the LBVH together, carrying the mask
+// of lanes whose rays reach the current node; a node's two boxes are tested by those lanes, a leaf's triangle by the
+// lanes whose rays hit its box. Every lane thus tests exactly the triangles the per-lane traversal would, with the
+// same arithmetic, so the counts are the same; nodes are read once per wave (scalar loads) and the control flow is
+// uniform, which pays when a wave's rays are close and parallel (lattice rows, spatially ordered queries). The stack
+// (node, mask) lives in the wave's LDS slots; the larger subtree is pushed, as in count_one, so it stays within
+// kMaxStack.
+template <class T>
+__device__ __forceinline__ int count_packet(T px, T py, T pz, bool valid, T dx, T dy, T dz,
+                                            bvh_node<float> const* __restrict__ nodes,
+                                            leaf_rec<T> const* __restrict__ rec, int32_t const* __restrict__ brute,
+                                            int32_t nb, int use_bvh, int32_t* stk_node, uint64_t* stk_mask, int lane)
+{
+    int sum = 0;
+    if (use_bvh)
+    {
+        float const ox = float(px), oy = float(py), oz = float(pz);
+        float const ix = inv_dir(double(dx)), iy = inv_dir(double(dy)), iz = inv_dir(double(dz));
+        uint64_t m = __ballot(valid); // lanes whose rays reach `node` (uniform)
+        int32_t node = 0;
+        int sp = 0;
+        while (m != 0)
+        {
+            bvh_node<float> const nd = nodes[node];
+            bool const act = (m >> lane) & 1ull;
+            bool const h0 = act && slab(nd.lo0, nd.hi0, ox, oy, oz, ix, iy, iz);
+            bool const h1 = act && slab(nd.lo1, nd.hi1, ox, oy, oz, ix, iy, iz);
+            uint64_t const m0 = __ballot(h0), m1 = __ballot(h1);
+            int32_t next = -1;
+            uint64_t nm = 0;
+            if (m0)
+            {
+                if (nd.c0 < 0)
+                {
+                    if (h0)
+                    {
+                        leaf_rec<T> const r = rec[~nd.c0];
+                        if (r.live)
+                            sum += mt_test(r, px, py, pz, dx, dy, dz);
+                    }
+                }
+                else
+                {
+                    next = nd.c0;
+                    nm = m0;
+                }
+            }
+            if (m1)
+            {
+                if (nd.c1 < 0)
+                {
+                    if (h1)
+                    {
+                        leaf_rec<T> const r = rec[~nd.c1];
+                        if (r.live)
+                            sum += mt_test(r, px, py, pz, dx, dy, dz);
+                    }
+                }
+                else if (next >= 0)
+                {
+                    // both children internal: go down the one with fewer leaves, push the other
+                    if (lane == 0)
+                    {
+                        stk_node[sp] = nd.big ? nd.c1 : nd.c0;
+                        stk_mask[sp] = nd.big ? m1 : m0;
+                    }
+                    ++sp;
+                    next = nd.big ? nd.c0 : nd.c1;
+                    nm = nd.big ? m0 : m1;
+                }
+                else
+                {
+                    next = nd.c1;
+                    nm = m1;
+                }
+            }
+            if (next >= 0)
+            {
+                node = next;
+                m = nm;
+            }
+            else if (sp > 0)
+            {
+                --sp;
+                __builtin_amdgcn_wave_barrier();
+                node = __builtin_amdgcn_readfirstlane(stk_node[sp]);
+                m = stk_mask[sp];
+                __builtin_amdgcn_wave_barrier();
+            }
+            else
+                m = 0;
+        }
+    }
+    if (valid)
+        for (int32_t b = 0; b < nb; ++b)
+            sum += mt_test(rec[brute[b]], px, py, pz, dx, dy, dz);
+    return sum;
+}
+
 template <class T, class Src>
 __global__ void __launch_bounds__(kTB)
     k_count(Src const src, int64_t n, T dx, T dy, T dz, bvh_node<float> const* __restrict__ nodes,
             leaf_rec<T> const* __restrict__ rec, int32_t const* __restrict__ brute, int32_t const* __restrict__ nbrute_p,
             int use_bvh, int32_t* __restrict__ out)
 {
-    __shared__ int32_t stack[kMaxStack][kTB];
+    __shared__ int32_t stk_node[kTB / 64][kMaxStack];
+    __shared__ uint64_t stk_mask[kTB / 64][kMaxStack];
     int64_t const i = int64_t(blockIdx.x) * kTB + threadIdx.x;
-    if (i >= n)
-        return;
-    T px, py, pz;
-    query_at(src, i, px, py, pz);
-    out[i] = count_one(px, py, pz, dx, dy, dz, nodes, rec, brute, *nbrute_p, use_bvh, stack, int(threadIdx.x));
+    bool const valid = i < n;
+    T px = T(0), py = T(0), pz = T(0);
+    if (valid)
+        query_at(src, i, px, py, pz);
+    int const w = int(threadIdx.x >> 6);
+    int const c = count_packet(px, py, pz, valid, dx, dy, dz, nodes, rec, brute, *nbrute_p, use_bvh, stk_node[w],
+                               stk_mask[w], int(threadIdx.x & 63));
+    if (valid)
+        out[i] = c;
 }
 
 // One prepared mesh of a batch, for the persistent counting kernel.
@@ -923,6 +1027,23 @@ __global__ void __launch_bounds__(kB) k_scatter_values(int64_t const* __restrict
         out[list[k]] = v[k];
 }
 
+// Routed entries, positions and integer terms from index order to the decision's order: out[e] = in[perm[e]].
+__global__ void __launch_bounds__(kB) k_permute_routed(uint64_t const* __restrict__ perm, int64_t m,
+                                                       int64_t const* __restrict__ li, float const* __restrict__ pi,
+                                                       int32_t const* __restrict__ ci, int64_t* __restrict__ lo,
+                                                       float* __restrict__ po, int32_t* __restrict__ co)
+{
+    int64_t const e = int64_t(blockIdx.x) * kB + threadIdx.x;
+    if (e >= m)
+        return;
+    int64_t const k = int64_t(perm[e]);
+    lo[e] = li[k];
+    po[3 * e] = pi[3 * k];
+    po[3 * e + 1] = pi[3 * k + 1];
+    po[3 * e + 2] = pi[3 * k + 2];
+    co[e] = ci[k];
+}
+
 // Values of m routed points (positions pts, integer terms cnt) on the correct side of 0.5: decision at the given
 // level (0: kDecideBeta, 1: kDecideBetaTight); the points it leaves open go to the next level, after the last to the
 // full boundary sum.
@@ -984,19 +1105,41 @@ int label_surrogate(mesh_t<float>& mesh, agwn_mlp const* mlp, Src const& src, in
         int64_t const m = int64_t(cnt[3]);
         if (m > 0)
         {
-            // the list was appended tile by tile in completion order: order it spatially, so that the exact
-            // kernels' waves see coherent runs of queries
-            dbuf tmp;
+            // the list was appended tile by tile in completion order. The integer terms take it in index order
+            // (on a lattice, a wave's rays then run along a few neighbouring rows: the packet traversal shares most
+            // of their nodes); the decision takes it in Morton order (a wave's queries then need the same loops).
+            dbuf tmp, keys, keys2;
             AGWN_TRY(tmp.alloc(size_t(m) * sizeof(uint64_t), s));
-            int64_t const* L = nullptr;
-            AGWN_TRY(route_order(mlp, src, n, list.as<int64_t>(), m, tmp.as<uint64_t>(), s, &L));
+            AGWN_TRY(keys.alloc(size_t(m) * sizeof(uint64_t), s));
+            AGWN_TRY(keys2.alloc(size_t(m) * sizeof(uint64_t), s));
+            int64_t const* Li = nullptr;
+            uint64_t const* perm = nullptr;
+            AGWN_TRY(route_order(mlp, src, n, list.as<int64_t>(), m, tmp.as<uint64_t>(), keys.as<uint64_t>(),
+                                 keys2.as<uint64_t>(), s, &Li, &perm));
             double const x0d[3] = {double(x0f[0]), double(x0f[1]), double(x0f[2])};
-            dbuf pts, c, w;
+            dbuf pts, c, w, pts_i, c_i, lm;
             AGWN_TRY(pts.alloc(size_t(m) * 3 * sizeof(float), s));
             AGWN_TRY(c.alloc(size_t(m) * sizeof(int32_t), s));
             AGWN_TRY(w.alloc(size_t(m) * sizeof(float), s));
-            AGWN_TRY(gather_routed(src, L, m, pts.as<float>(), s));
-            AGWN_TRY(mesh.count_src(src_array<float>{pts.as<float>()}, m, qmax, x0f, c.as<int32_t>(), s));
+            int64_t const* L = Li;
+            if (perm)
+            {
+                AGWN_TRY(pts_i.alloc(size_t(m) * 3 * sizeof(float), s));
+                AGWN_TRY(c_i.alloc(size_t(m) * sizeof(int32_t), s));
+                AGWN_TRY(lm.alloc(size_t(m) * sizeof(int64_t), s));
+                AGWN_TRY(gather_routed(src, Li, m, pts_i.as<float>(), s));
+                AGWN_TRY(mesh.count_src(src_array<float>{pts_i.as<float>()}, m, qmax, x0f, c_i.as<int32_t>(), s));
+                k_permute_routed<<<dim3(unsigned((m + kB - 1) / kB)), dim3(kB), 0, s>>>(
+                    perm, m, Li, pts_i.as<float>(), c_i.as<int32_t>(), lm.as<int64_t>(), pts.as<float>(),
+                    c.as<int32_t>());
+                AGWN_HIP(hipGetLastError());
+                L = lm.as<int64_t>();
+            }
+            else
+            {
+                AGWN_TRY(gather_routed(src, Li, m, pts.as<float>(), s));
+                AGWN_TRY(mesh.count_src(src_array<float>{pts.as<float>()}, m, qmax, x0f, c.as<int32_t>(), s));
+            }
             frac_plan local;
             frac_plan const* pl = plan;
             if (!pl)

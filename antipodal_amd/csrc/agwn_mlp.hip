@@ -1344,16 +1344,16 @@ struct key_box
 {
     float lo[3], scale[3]; // cell = (p - lo) * scale, clamped to [0, 1023]
 };
+// keys[e] = (Morton code of the query of entry e of the index-ordered list) << pbits | e
 template <class Src>
-__global__ void __launch_bounds__(kB) k_route_keys(Src const src, uint64_t* __restrict__ list, int64_t m, key_box K,
-                                                   int ebits)
+__global__ void __launch_bounds__(kB) k_route_keys(Src const src, uint64_t const* __restrict__ list, int64_t m,
+                                                   key_box K, int pbits, uint64_t* __restrict__ keys)
 {
     int64_t const k = int64_t(blockIdx.x) * kB + threadIdx.x;
     if (k >= m)
         return;
-    uint64_t const e = list[k];
     float p[3];
-    query_at(src, int64_t(e >> 1), p[0], p[1], p[2]);
+    query_at(src, int64_t(list[k] >> 1), p[0], p[1], p[2]);
     uint64_t code = 0;
 #pragma unroll
     for (int a = 0; a < 3; ++a)
@@ -1361,33 +1361,36 @@ __global__ void __launch_bounds__(kB) k_route_keys(Src const src, uint64_t* __re
         float const c = fminf(fmaxf((p[a] - K.lo[a]) * K.scale[a], 0.0f), 1023.0f); // NaN -> 0
         code |= spread3_10(uint32_t(c)) << a;
     }
-    list[k] = (code << ebits) | e;
+    keys[k] = (code << pbits) | uint64_t(k);
 }
-__global__ void __launch_bounds__(kB) k_strip_keys(uint64_t* __restrict__ list, int64_t m, uint64_t mask)
+__global__ void __launch_bounds__(kB) k_strip_keys(uint64_t* __restrict__ keys, int64_t m, uint64_t mask)
 {
     int64_t const k = int64_t(blockIdx.x) * kB + threadIdx.x;
     if (k < m)
-        list[k] &= mask;
+        keys[k] &= mask;
 }
 
 template <class Src>
-int route_order(agwn_mlp const* h, Src const& src, int64_t n, int64_t* list, int64_t m, uint64_t* tmp, hipStream_t s,
-                int64_t const** ordered)
+int route_order(agwn_mlp const* h, Src const& src, int64_t n, int64_t* list, int64_t m, uint64_t* tmp,
+                uint64_t* keys, uint64_t* keys2, hipStream_t s, int64_t const** by_index, uint64_t const** perm)
 {
     AGWN_CHECK(h != nullptr, "mlp handle is null");
-    *ordered = list;
+    *by_index = list;
+    if (perm)
+        *perm = nullptr;
     if (m == 0)
         return ok;
     int ebits = 1;
     while (ebits < 63 && (uint64_t(2 * n) >> ebits) != 0)
         ++ebits;
     uint64_t* sorted = nullptr;
-    if (ebits + 30 > 64) // index order only
-    {
-        AGWN_TRY(radix_sort_u64(reinterpret_cast<uint64_t*>(list), tmp, m, ebits, s, &sorted));
-        *ordered = reinterpret_cast<int64_t const*>(sorted);
+    AGWN_TRY(radix_sort_u64(reinterpret_cast<uint64_t*>(list), tmp, m, ebits, s, &sorted));
+    *by_index = reinterpret_cast<int64_t const*>(sorted);
+    int pbits = 1;
+    while (pbits < 63 && (uint64_t(m - 1) >> pbits) != 0)
+        ++pbits;
+    if (!perm || pbits + 30 > 64) // index order only
         return ok;
-    }
     mlp_impl const& M = h->impl;
     key_box K{};
     for (int a = 0; a < 3; ++a)
@@ -1397,18 +1400,20 @@ int route_order(agwn_mlp const* h, Src const& src, int64_t n, int64_t* list, int
         K.scale[a] = ext > 0.0 ? float(1024.0 / ext) : 0.0f;
     }
     unsigned const g = unsigned((m + kB - 1) / kB);
-    k_route_keys<Src><<<dim3(g), dim3(kB), 0, s>>>(src, reinterpret_cast<uint64_t*>(list), m, K, ebits);
+    k_route_keys<Src><<<dim3(g), dim3(kB), 0, s>>>(src, sorted, m, K, pbits, keys);
     AGWN_HIP(hipGetLastError());
-    AGWN_TRY(radix_sort_u64(reinterpret_cast<uint64_t*>(list), tmp, m, ebits + 30, s, &sorted));
-    k_strip_keys<<<dim3(g), dim3(kB), 0, s>>>(sorted, m, (uint64_t(1) << ebits) - 1);
+    uint64_t* ks = nullptr;
+    AGWN_TRY(radix_sort_u64(keys, keys2, m, pbits + 30, s, &ks));
+    k_strip_keys<<<dim3(g), dim3(kB), 0, s>>>(ks, m, (uint64_t(1) << pbits) - 1);
     AGWN_HIP(hipGetLastError());
-    *ordered = reinterpret_cast<int64_t const*>(sorted);
+    *perm = ks;
     return ok;
 }
 template int route_order<src_array<float>>(agwn_mlp const*, src_array<float> const&, int64_t, int64_t*, int64_t,
-                                           uint64_t*, hipStream_t, int64_t const**);
-template int route_order<src_grid>(agwn_mlp const*, src_grid const&, int64_t, int64_t*, int64_t, uint64_t*,
-                                   hipStream_t, int64_t const**);
+                                           uint64_t*, uint64_t*, uint64_t*, hipStream_t, int64_t const**,
+                                           uint64_t const**);
+template int route_order<src_grid>(agwn_mlp const*, src_grid const&, int64_t, int64_t*, int64_t, uint64_t*, uint64_t*,
+                                   uint64_t*, hipStream_t, int64_t const**, uint64_t const**);
 
 template <class Src>
 int gather_routed(Src const& src, int64_t const* list, int64_t m, float* pts, hipStream_t s)
