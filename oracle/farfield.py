@@ -53,7 +53,7 @@ def unit_loops(seg7: np.ndarray):
     return walks, closed
 
 
-def split_loop(points: np.ndarray, arc: int = 32):
+def split_loop(points: np.ndarray, arc: int = 16):
     """The library's split of a long closed loop (more than `arc` segments): arcs of `arc` segments, each closed by a
     chord back to its first point, plus the skeleton loop through the split points (its chords cancel the arcs')."""
     p = np.asarray(points, dtype=np.float64)

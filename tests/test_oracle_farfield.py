@@ -19,6 +19,14 @@ from antipodal_amd import meshgen as mg
 from oracle import farfield as ff
 
 
+def ratio_max(err, b):
+    """max of err / bound; a zero bound (a degenerate loop with no fan area, e.g. a skeleton that goes out and back)
+    requires a zero error (to rounding)."""
+    zero = b <= 0.0
+    assert np.all(err[zero] <= 1e-12), err[zero].max()
+    return float((err[~zero] / b[~zero]).max()) if (~zero).any() else 0.0
+
+
 def worst_ratio(oracle, walks, x0, q, sign=1.0):
     worst, n = 0.0, 0
     for w in walks:
@@ -29,7 +37,7 @@ def worst_ratio(oracle, walks, x0, q, sign=1.0):
         qq = q[ok]
         exact = oracle.frac(qq, ff.loop_segments(w), x0)
         est, b = ff.estimate(tab, qq, sign)
-        worst = max(worst, float((np.abs(exact - est) / b).max()))
+        worst = max(worst, ratio_max(np.abs(exact - est), b))
         n += len(qq)
     return worst, n
 
@@ -62,12 +70,12 @@ def test_split_pieces_sum_to_the_loop_and_keep_the_bound(oracle, torus_walks):
     longw = [w for w in walks if len(w) > 200][:6]
     assert longw
     for w in longw:
-        pieces = ff.split_loop(w, 32)
+        pieces = ff.split_loop(w, 16)
         assert len(pieces) > 2 and all(np.array_equal(p[0], p[-1]) for p in pieces)
         whole = oracle.frac(q, ff.loop_segments(w), x0)
         parts = sum(oracle.frac(q, ff.loop_segments(p), x0) for p in pieces)
         np.testing.assert_allclose(parts, whole, rtol=0, atol=1e-9)
-    worst, n = worst_ratio(oracle, [p for w in longw for p in ff.split_loop(w, 32)], x0, q)
+    worst, n = worst_ratio(oracle, [p for w in longw for p in ff.split_loop(w, 16)], x0, q)
     assert n > 100 and worst <= 1.0
 
 
@@ -112,7 +120,7 @@ def test_disk_axis_closed_form(oracle):
 def clusters_of(walks, size=16):
     """Pieces of the split walks, grouped `size` at a time in the order of their centres along a space-filling sort
     (coarse voxel, then x): nearby loops share a cluster."""
-    pieces = [p for w in walks for p in ff.split_loop(w, 32)]
+    pieces = [p for w in walks for p in ff.split_loop(w, 16)]
     tabs = [ff.loop_table(p) for p in pieces]
     cs = np.array([t["c"] for t in tabs])
     lo, hi = cs.min(axis=0), cs.max(axis=0)
@@ -132,7 +140,7 @@ def cluster_worst(oracle, clusters, x0, q, shift=True):
         qq = q[ok]
         exact = sum(oracle.frac(qq, ff.loop_segments(p), x0) for p in pieces)
         est, b = ff.estimate(tab, qq)
-        worst = max(worst, float((np.abs(exact - est) / b).max()))
+        worst = max(worst, ratio_max(np.abs(exact - est), b))
         n += len(qq)
     return worst, n
 
@@ -186,7 +194,7 @@ def test_tree_nodes_keep_the_bound_on_the_torus(oracle, torus_walks):
     x0 = mg.example_x0(np.float64)
     rng = np.random.default_rng(6)
     q = np.concatenate([q, rng.uniform(-2.0, 2.0, (24, 3))])
-    pieces = [p for w in walks for p in ff.split_loop(w, 32)]
+    pieces = [p for w in walks for p in ff.split_loop(w, 16)]
     tabs = [ff.loop_table(p) for p in pieces]
     cs = np.array([t["c"] for t in tabs])
     lo, hi = cs.min(axis=0), cs.max(axis=0)
@@ -210,7 +218,7 @@ def test_tree_nodes_keep_the_bound_on_the_torus(oracle, torus_walks):
         if ok.any():
             est, b = ff.estimate(t, q[ok])
             err = np.abs(fr[idx][:, ok].sum(axis=0) - est)
-            worst = max(worst, float((err / b).max()))
+            worst = max(worst, ratio_max(err, b))
             cases += int(ok.sum())
         return t
 
