@@ -11,9 +11,9 @@ lattice over the bounding box expanded by 10 % (134,217,728 points, generated in
              near-surface mask); each step labels the lattice with the fused inference kernel and routes to the exact
              kernels every point in the mask (voxels of edge eps/4 within eps = 0.10 % of the box diagonal of a
              triangle), every point with |pred - 0.5| <= delta, and a 1/1024 audit sample, which take their exact
-             labels: integer term, then a far-field decision per boundary loop (second-order expansion with a
-             rigorous remainder bound; loops too close are summed exactly), and the full boundary sum only for the
-             points the decision leaves open. delta comes from 2^23 held-out points (2 x the worst wrong-side margin +
+             labels: integer term, then a far-field decision over a tree of the boundary's loops (second-order
+             expansion with a rigorous remainder bound per accepted node; loops too close are summed exactly), and
+             the full boundary sum only for the points the decision leaves open. delta comes from 2^23 held-out points (2 x the worst wrong-side margin +
              0.02). Training happens before the timed region and is reported (train_s).
 The headline value is surrogate steps/s x queries; mislabels = labels that differ from the plain run's (must be 0).
 
