@@ -457,7 +457,7 @@ __global__ void __launch_bounds__(kB) k_perm(int64_t* __restrict__ perm, int64_t
 // out^T[n][m] = sum_k Wt[n][k] act[m][k]: wave w owns units [64w, 64w + 64) (A operand = Wt rows in fragment order,
 // from L2) for all the tile's points (B operand = LDS rows); the accumulators start at the biases, so a lane's 4
 // consecutive accumulators are 4 consecutive units of one point, activated and written back to LDS as one 8-byte
-// write after a barrier. The output layer is a 256-long dot per point.
+// write after a barrier. The output layer (a 256-long dot per point) is fused into the last layer's epilogue.
 // Optionally labels the points (pred > 0.5) and appends to a routing list (entry = 2 * index + (audit ? 1 : 0)) the
 // points in the near-surface mask or outside the box, those with |pred - 0.5| <= delta, and a hashed audit sample of
 // the rest.
@@ -465,7 +465,7 @@ __global__ void __launch_bounds__(kB) k_perm(int64_t* __restrict__ perm, int64_t
 constexpr int kLdsStride = kH + 8; // BF16 per LDS row: 528 B rows keep the 16-lane groups of ds_read_b128 conflict-free
 constexpr int kTileP = 96;          // points per workgroup (a multiple of 32)
 constexpr int kTileJ = kTileP / 32; // 32-point MFMA column blocks per tile
-constexpr int kInferBlocks = (160 * 1024) / (kTileP * kLdsStride * 2 + 1024); // workgroups per CU that the LDS holds
+constexpr int kInferBlocks = (160 * 1024) / (kTileP * kLdsStride * 2 + 16 * kTileP); // workgroups per CU that the LDS holds
 static_assert(kTileP % 32 == 0 && kInferBlocks >= 1, "tile shape");
 
 struct infer_params
@@ -596,7 +596,7 @@ __global__ void __launch_bounds__(256, kInferBlocks) k_mlp_infer(Src const src, 
                                                       float* __restrict__ pred, route_params RP)
 {
     __shared__ __attribute__((aligned(16))) __bf16 act[kTileP * kLdsStride];
-    __shared__ float predl[kTileP];
+    __shared__ float predw[4][kTileP]; // per wave: the dot over its 64 units
     int const lane = threadIdx.x & 63;
     int const wave = threadIdx.x >> 6;
     int64_t const base = int64_t(blockIdx.x) * kTileP;
@@ -633,14 +633,13 @@ __global__ void __launch_bounds__(256, kInferBlocks) k_mlp_infer(Src const src, 
     int const li = lane & 31;
     int const kh = (lane >> 5) * 8;
     int const n0 = wave * 64;
-#pragma unroll 1
-    for (int l = 0; l < kL; ++l)
-    {
+    f32x16 acc[2][kTileJ]; // starts at the biases of the lane's units (C[r][c]: unit r = 8(e/4) + 4(l/32) + e%4)
+    // layer l's MFMAs: acc = b_l + Wt_l[units of the wave] act^T
+    auto layer = [&](int l) __attribute__((always_inline)) {
         int const K = l == 0 ? kIn : kH;
         // fragments of unit blocks 2w and 2w + 1 (units n0 .. n0 + 63); k step st at + 512 st
         __bf16 const* wa0 = P.wf[l] + (int64_t(n0 / 32) * (K / 16) * 64 + lane) * 8;
         __bf16 const* wa1 = wa0 + int64_t(K / 16) * 512;
-        f32x16 acc[2][kTileJ]; // starts at the biases of the lane's units (C[r][c]: unit r = 8(e/4) + 4(l/32) + e%4)
 #pragma unroll
         for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -658,6 +657,7 @@ __global__ void __launch_bounds__(256, kInferBlocks) k_mlp_infer(Src const src, 
             }
         bf16x8 a0 = *reinterpret_cast<bf16x8 const*>(wa0);
         bf16x8 a1 = *reinterpret_cast<bf16x8 const*>(wa1);
+#pragma unroll 1 // unrolled (K is a constant in the last layer's copy), the loads are hoisted and spill
         for (int k0 = 0; k0 < K; k0 += 16)
         {
             int const kn = k0 + 16 < K ? k0 + 16 : k0;
@@ -673,6 +673,11 @@ __global__ void __launch_bounds__(256, kInferBlocks) k_mlp_infer(Src const src, 
             a0 = na0;
             a1 = na1;
         }
+    };
+#pragma unroll 1
+    for (int l = 0; l + 1 < kL; ++l)
+    {
+        layer(l);
         __syncthreads(); // every wave has read the layer input
 #pragma unroll
         for (int i = 0; i < 2; ++i)
@@ -693,39 +698,42 @@ __global__ void __launch_bounds__(256, kInferBlocks) k_mlp_infer(Src const src, 
             }
         __syncthreads();
     }
-    // output layer: two lanes per point (lanes = consecutive points: 16 B LDS reads conflict-free per 16-lane group),
-    // each a 128-unit dot product in FP32, halves added through LDS
+    layer(kL - 1);
     {
-        int const p = threadIdx.x % kTileP, half = threadIdx.x / kTileP;
-        float s0 = 0.0f, s1 = 0.0f;
-        if (half < 2)
-        {
-            float const* wo = P.wo + half * (kH / 2);
-            __bf16 const* row = act + p * kLdsStride + half * (kH / 2);
+        // the output layer fused into the last epilogue: each lane's 32 units of its kTileJ points, activated and
+        // rounded to BF16 (the stored activations' values), dotted with wo in FP32; the two lane halves (other units,
+        // same points) through a shuffle, the four waves (other units) through LDS
+        float part[kTileJ] = {};
 #pragma unroll
-            for (int g = 0; g < kH / 16; ++g)
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
             {
-                bf16x8 const hv = *reinterpret_cast<bf16x8 const*>(row + 8 * g);
+                float4 const wq = *reinterpret_cast<float4 const*>(P.wo + n0 + i * 32 + 8 * q + 4 * (lane >> 5));
 #pragma unroll
-                for (int q = 0; q < 8; q += 2)
+                for (int j = 0; j < kTileJ; ++j)
                 {
-                    s0 = fmaf(float(hv[q]), wo[8 * g + q], s0);
-                    s1 = fmaf(float(hv[q + 1]), wo[8 * g + q + 1], s1);
+                    part[j] = fmaf(float(__bf16(hsilu(acc[i][j][4 * q]))), wq.x, part[j]);
+                    part[j] = fmaf(float(__bf16(hsilu(acc[i][j][4 * q + 1]))), wq.y, part[j]);
+                    part[j] = fmaf(float(__bf16(hsilu(acc[i][j][4 * q + 2]))), wq.z, part[j]);
+                    part[j] = fmaf(float(__bf16(hsilu(acc[i][j][4 * q + 3]))), wq.w, part[j]);
                 }
             }
+#pragma unroll
+        for (int j = 0; j < kTileJ; ++j)
+        {
+            part[j] += __shfl_xor(part[j], 32);
+            if (lane < 32)
+                predw[wave][j * 32 + lane] = part[j];
         }
-        if (half == 1)
-            predl[p] = s0 + s1;
         __syncthreads();
-        if (half == 0)
-            predl[p] += (s0 + s1) + *P.bo;
     }
-    __syncthreads();
     if (threadIdx.x >= kTileP)
         return;
     int64_t const i = base + threadIdx.x;
     bool const live = i < n;
-    float const pv = predl[threadIdx.x];
+    float const pv = ((predw[0][threadIdx.x] + predw[1][threadIdx.x]) + (predw[2][threadIdx.x] + predw[3][threadIdx.x])) +
+                     *P.bo;
     if (pred && live)
         pred[i] = pv;
     if constexpr (ROUTE)
