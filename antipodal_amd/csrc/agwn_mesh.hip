@@ -28,7 +28,7 @@ namespace agwn
 namespace
 {
 constexpr int kB = 256;
-constexpr int kTB = 128; // traversal block: 128 lanes x 64-entry LDS stack x 4 B = 32 KiB
+constexpr int kTB = 128; // traversal block: per-lane stacks (count_one) take kMaxStack x 4 B x 128 lanes = 14 KiB of LDS
 
 template <class T>
 struct alignas(sizeof(T) == 4 ? 64 : 128) leaf_rec
@@ -228,7 +228,7 @@ __device__ __forceinline__ float inv_dir(double d)
 
 // Signed crossing count of one ray (p, d) against a prepared mesh: LBVH short-stack traversal (stack column `lane` of
 // an LDS array [kMaxStack][kTB]) with the Moeller-Trumbore replica at the leaves, plus the exhaustively tested
-// triangles.
+// triangles. The batched kernel's traversal: its queries are scattered, where a wave's rays share few nodes.
 template <class T>
 __device__ __forceinline__ int count_one(T px, T py, T pz, T dx, T dy, T dz, bvh_node<float> const* __restrict__ nodes,
                                          leaf_rec<T> const* __restrict__ rec, int32_t const* __restrict__ brute,

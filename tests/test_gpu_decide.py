@@ -74,3 +74,21 @@ def test_open_walks_take_the_full_sum(gpu, torus):
     x0 = mg.example_x0(np.float32)
     exact, lab, st, _ = exact_and_routed(gpu, v32, idx, part, q, x0)
     assert int((lab != exact).sum()) == 0
+
+
+@pytest.mark.parametrize("kind,ntri,holes,seed", [("cap", 0, 0, 0), ("sphere", 3000, 5, 11), ("torus", 20000, 40, 12)])
+def test_decision_on_other_boundaries(gpu, kind, ntri, holes, seed):
+    """Other loop trees: the cap's single 36-segment walk (arcs of 16, 16 and 4 segments and a skeleton), and seeded
+    open meshes whose holes give walks of every length -- among them walks of 17 to 31 segments, whose skeleton goes
+    out to the split point and back (no fan area, bound 0). Labels equal the full evaluation's on every query."""
+    if kind == "cap":
+        v, idx = mg.icosphere_cap_mesh()
+    else:
+        v, idx = mg.open_mesh(kind, ntri, holes, seed=seed)
+    v32 = np.asarray(v, dtype=np.float32)
+    seg = gpu.build_boundary_segments(v32, idx)
+    q = queries(v32, idx, seg, 60_000, seed=seed + 1)
+    x0 = mg.example_x0(np.float32)
+    exact, lab, st, _ = exact_and_routed(gpu, v32, idx, seg, q, x0)
+    assert st["routed"] == len(q)
+    assert int((lab != exact).sum()) == 0
