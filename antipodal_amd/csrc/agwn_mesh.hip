@@ -401,24 +401,38 @@ __device__ __forceinline__ int count_packet(T px, T py, T pz, bool valid, T dx, 
     return sum;
 }
 
-template <class T, class Src>
+// PACKET: the wave walks the tree together (queries in lattice order or close to it); otherwise each lane alone
+// (queries in any order: scattered rays share few nodes, and a packet would visit the union of their paths).
+template <class T, class Src, bool PACKET>
 __global__ void __launch_bounds__(kTB)
     k_count(Src const src, int64_t n, T dx, T dy, T dz, bvh_node<float> const* __restrict__ nodes,
             leaf_rec<T> const* __restrict__ rec, int32_t const* __restrict__ brute, int32_t const* __restrict__ nbrute_p,
             int use_bvh, int32_t* __restrict__ out)
 {
-    __shared__ int32_t stk_node[kTB / 64][kMaxStack];
-    __shared__ uint64_t stk_mask[kTB / 64][kMaxStack];
     int64_t const i = int64_t(blockIdx.x) * kTB + threadIdx.x;
-    bool const valid = i < n;
-    T px = T(0), py = T(0), pz = T(0);
-    if (valid)
+    if constexpr (PACKET)
+    {
+        __shared__ int32_t stk_node[kTB / 64][kMaxStack];
+        __shared__ uint64_t stk_mask[kTB / 64][kMaxStack];
+        bool const valid = i < n;
+        T px = T(0), py = T(0), pz = T(0);
+        if (valid)
+            query_at(src, i, px, py, pz);
+        int const w = int(threadIdx.x >> 6);
+        int const c = count_packet(px, py, pz, valid, dx, dy, dz, nodes, rec, brute, *nbrute_p, use_bvh, stk_node[w],
+                                   stk_mask[w], int(threadIdx.x & 63));
+        if (valid)
+            out[i] = c;
+    }
+    else
+    {
+        __shared__ int32_t stack[kMaxStack][kTB];
+        if (i >= n)
+            return;
+        T px, py, pz;
         query_at(src, i, px, py, pz);
-    int const w = int(threadIdx.x >> 6);
-    int const c = count_packet(px, py, pz, valid, dx, dy, dz, nodes, rec, brute, *nbrute_p, use_bvh, stk_node[w],
-                               stk_mask[w], int(threadIdx.x & 63));
-    if (valid)
-        out[i] = c;
+        out[i] = count_one(px, py, pz, dx, dy, dz, nodes, rec, brute, *nbrute_p, use_bvh, stack, int(threadIdx.x));
+    }
 }
 
 // One prepared mesh of a batch, for the persistent counting kernel.
@@ -606,7 +620,7 @@ struct mesh_t final : mesh_impl
         }
         double qmax = 0.0;
         AGWN_TRY(max_query_norm(pos, n, s, qmax));
-        return count_src(src_array<T>{pos}, n, qmax, d, out, s);
+        return count_src(src_array<T>{pos}, n, qmax, d, out, s, false);
     }
 
     static int max_query_norm(T const* pos, int64_t n, hipStream_t s, double& qmax)
@@ -626,7 +640,7 @@ struct mesh_t final : mesh_impl
 
     // counts of the queries of `src`, all of norm <= qmax
     template <class Src>
-    int count_src(Src const& src, int64_t n, double qmax, T const d[3], int32_t* out, hipStream_t s)
+    int count_src(Src const& src, int64_t n, double qmax, T const d[3], int32_t* out, hipStream_t s, bool packet)
     {
         if (n == 0)
             return ok;
@@ -639,8 +653,12 @@ struct mesh_t final : mesh_impl
         AGWN_TRY(prepare(d, qmax, s, st));
         int64_t const blocks = (n + kTB - 1) / kTB;
         AGWN_CHECK(blocks < (int64_t(1) << 31), "too many queries (%lld)", (long long)n);
-        k_count<T, Src><<<dim3(unsigned(blocks)), dim3(kTB), 0, s>>>(src, n, d[0], d[1], d[2], st->nodes, st->rec,
-                                                                     st->brute, st->nbrute, ntri > 1 ? 1 : 0, out);
+        if (packet)
+            k_count<T, Src, true><<<dim3(unsigned(blocks)), dim3(kTB), 0, s>>>(
+                src, n, d[0], d[1], d[2], st->nodes, st->rec, st->brute, st->nbrute, ntri > 1 ? 1 : 0, out);
+        else
+            k_count<T, Src, false><<<dim3(unsigned(blocks)), dim3(kTB), 0, s>>>(
+                src, n, d[0], d[1], d[2], st->nodes, st->rec, st->brute, st->nbrute, ntri > 1 ? 1 : 0, out);
         AGWN_HIP(hipGetLastError());
         // keep the state alive until the kernel has consumed it (it may be replaced by another thread's call)
         AGWN_HIP(hipStreamSynchronize(s));
@@ -1001,7 +1019,7 @@ int grid_eval_dev(agwn_grid* g, float const* start, float const* dx, float const
     AGWN_CHECK(g->mesh != nullptr, "set_scene was not called");
     dbuf cnt;
     AGWN_TRY(cnt.alloc(size_t(c.n) * sizeof(int32_t), s));
-    AGWN_TRY(g->mesh->count_src(src_grid{c.gd}, c.n, c.qmax, c.x0f, cnt.as<int32_t>(), s));
+    AGWN_TRY(g->mesh->count_src(src_grid{c.gd}, c.n, c.qmax, c.x0f, cnt.as<int32_t>(), s, true));
     return frac_f32_planned(src_grid{c.gd}, c.n, *g->plan, g->seg7, g->nseg, c.x0d, cnt.as<int32_t>(), out, s);
 }
 
@@ -1128,7 +1146,8 @@ int label_surrogate(mesh_t<float>& mesh, agwn_mlp const* mlp, Src const& src, in
                 AGWN_TRY(c_i.alloc(size_t(m) * sizeof(int32_t), s));
                 AGWN_TRY(lm.alloc(size_t(m) * sizeof(int64_t), s));
                 AGWN_TRY(gather_routed(src, Li, m, pts_i.as<float>(), s));
-                AGWN_TRY(mesh.count_src(src_array<float>{pts_i.as<float>()}, m, qmax, x0f, c_i.as<int32_t>(), s));
+                AGWN_TRY(
+                    mesh.count_src(src_array<float>{pts_i.as<float>()}, m, qmax, x0f, c_i.as<int32_t>(), s, true));
                 k_permute_routed<<<dim3(unsigned((m + kB - 1) / kB)), dim3(kB), 0, s>>>(
                     perm, m, Li, pts_i.as<float>(), c_i.as<int32_t>(), lm.as<int64_t>(), pts.as<float>(),
                     c.as<int32_t>());
@@ -1138,7 +1157,7 @@ int label_surrogate(mesh_t<float>& mesh, agwn_mlp const* mlp, Src const& src, in
             else
             {
                 AGWN_TRY(gather_routed(src, Li, m, pts.as<float>(), s));
-                AGWN_TRY(mesh.count_src(src_array<float>{pts.as<float>()}, m, qmax, x0f, c.as<int32_t>(), s));
+                AGWN_TRY(mesh.count_src(src_array<float>{pts.as<float>()}, m, qmax, x0f, c.as<int32_t>(), s, true));
             }
             frac_plan local;
             frac_plan const* pl = plan;
