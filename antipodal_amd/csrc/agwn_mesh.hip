@@ -401,6 +401,56 @@ __device__ __forceinline__ int count_packet(T px, T py, T pz, bool valid, T dx, 
     return sum;
 }
 
+// Ray order for the packet traversal: key = (query index << 32) | 2D Morton code (16 bits per axis) of the origin
+// projected on the plane across the direction (basis u, w), quantized over [-r, r] (r = the largest query norm).
+struct ray_key_frame
+{
+    float u[3], w[3];
+    float r, scale; // scale = 65535 / (2 r)
+};
+constexpr int64_t kSortRaysMin = int64_t(1) << 16; // queries from which count() sorts the rays
+
+__device__ __forceinline__ uint32_t spread2_16(uint32_t v)
+{
+    v &= 0xFFFFu;
+    v = (v | (v << 8)) & 0x00FF00FFu;
+    v = (v | (v << 4)) & 0x0F0F0F0Fu;
+    v = (v | (v << 2)) & 0x33333333u;
+    v = (v | (v << 1)) & 0x55555555u;
+    return v;
+}
+template <class T>
+__global__ void __launch_bounds__(kB) k_ray_keys(T const* __restrict__ pos, int64_t n, ray_key_frame F,
+                                                 uint64_t* __restrict__ keys)
+{
+    int64_t const i = int64_t(blockIdx.x) * kB + threadIdx.x;
+    if (i >= n)
+        return;
+    float const x = float(pos[3 * i]), y = float(pos[3 * i + 1]), z = float(pos[3 * i + 2]);
+    float const a = fminf(fmaxf((F.u[0] * x + F.u[1] * y + F.u[2] * z + F.r) * F.scale, 0.0f), 65535.0f); // NaN -> 0
+    float const b = fminf(fmaxf((F.w[0] * x + F.w[1] * y + F.w[2] * z + F.r) * F.scale, 0.0f), 65535.0f);
+    keys[i] = (uint64_t(i) << 32) | uint64_t(spread2_16(uint32_t(a)) | (spread2_16(uint32_t(b)) << 1));
+}
+template <class T>
+__global__ void __launch_bounds__(kB) k_ray_gather(uint64_t const* __restrict__ sorted, int64_t n,
+                                                   T const* __restrict__ pos, T* __restrict__ out)
+{
+    int64_t const e = int64_t(blockIdx.x) * kB + threadIdx.x;
+    if (e >= n)
+        return;
+    int64_t const i = int64_t(sorted[e] >> 32);
+    out[3 * e] = pos[3 * i];
+    out[3 * e + 1] = pos[3 * i + 1];
+    out[3 * e + 2] = pos[3 * i + 2];
+}
+__global__ void __launch_bounds__(kB) k_ray_scatter(uint64_t const* __restrict__ sorted, int64_t n,
+                                                    int32_t const* __restrict__ cs, int32_t* __restrict__ out)
+{
+    int64_t const e = int64_t(blockIdx.x) * kB + threadIdx.x;
+    if (e < n)
+        out[int64_t(sorted[e] >> 32)] = cs[e];
+}
+
 // PACKET: the wave walks the tree together (queries in lattice order or close to it); otherwise each lane alone
 // (queries in any order: scattered rays share few nodes, and a packet would visit the union of their paths).
 template <class T, class Src, bool PACKET>
@@ -609,6 +659,11 @@ struct mesh_t final : mesh_impl
         return ok;
     }
 
+    // Integer terms of queries in the caller's order. Large calls sort the rays first -- by the 2D Morton code of
+    // their origins projected along d, so that a wave's rays run close together -- count them with the packet
+    // traversal and scatter the counts back (2M random rays in a 200k-triangle mesh: 2.84 ms per lane in the
+    // caller's order, 0.89 ms sorted and in packets). Counts are integers and each ray's triangle tests are the
+    // same either way, so the order changes nothing but the time.
     int count(T const* pos, int64_t n, T const d[3], int32_t* out, hipStream_t s)
     {
         if (n == 0)
@@ -620,7 +675,37 @@ struct mesh_t final : mesh_impl
         }
         double qmax = 0.0;
         AGWN_TRY(max_query_norm(pos, n, s, qmax));
-        return count_src(src_array<T>{pos}, n, qmax, d, out, s, false);
+        if (n < kSortRaysMin || n >= (int64_t(1) << 32))
+            return count_src(src_array<T>{pos}, n, qmax, d, out, s, false);
+        double e[3] = {double(d[0]), double(d[1]), double(d[2])};
+        double const el = std::sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+        AGWN_CHECK(el > 0.0, "zero ray direction");
+        for (double& c : e)
+            c /= el;
+        double const h[3] = {std::fabs(e[0]) < 0.9 ? 1.0 : 0.0, std::fabs(e[0]) < 0.9 ? 0.0 : 1.0, 0.0};
+        double u[3] = {e[1] * h[2] - e[2] * h[1], e[2] * h[0] - e[0] * h[2], e[0] * h[1] - e[1] * h[0]};
+        double const ul = std::sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+        for (double& c : u)
+            c /= ul;
+        double const w[3] = {e[1] * u[2] - e[2] * u[1], e[2] * u[0] - e[0] * u[2], e[0] * u[1] - e[1] * u[0]};
+        ray_key_frame F{{float(u[0]), float(u[1]), float(u[2])}, {float(w[0]), float(w[1]), float(w[2])},
+                        float(qmax), qmax > 0.0 ? float(65535.0 / (2.0 * qmax)) : 0.0f};
+        dbuf keys, keys2, ps, cs;
+        AGWN_TRY(keys.alloc(size_t(n) * sizeof(uint64_t), s));
+        AGWN_TRY(keys2.alloc(size_t(n) * sizeof(uint64_t), s));
+        AGWN_TRY(ps.alloc(size_t(n) * 3 * sizeof(T), s));
+        AGWN_TRY(cs.alloc(size_t(n) * sizeof(int32_t), s));
+        unsigned const g = unsigned((n + kB - 1) / kB);
+        k_ray_keys<T><<<dim3(g), dim3(kB), 0, s>>>(pos, n, F, keys.as<uint64_t>());
+        AGWN_HIP(hipGetLastError());
+        uint64_t* sorted = nullptr;
+        AGWN_TRY(radix_sort_u64(keys.as<uint64_t>(), keys2.as<uint64_t>(), n, 32, s, &sorted));
+        k_ray_gather<T><<<dim3(g), dim3(kB), 0, s>>>(sorted, n, pos, ps.as<T>());
+        AGWN_HIP(hipGetLastError());
+        AGWN_TRY(count_src(src_array<T>{ps.as<T>()}, n, qmax, d, cs.as<int32_t>(), s, true));
+        k_ray_scatter<<<dim3(g), dim3(kB), 0, s>>>(sorted, n, cs.as<int32_t>(), out);
+        AGWN_HIP(hipGetLastError());
+        return ok;
     }
 
     static int max_query_norm(T const* pos, int64_t n, hipStream_t s, double& qmax)

@@ -399,7 +399,8 @@ def main() -> int:
         raise SystemExit("exact-stage labels of the routed points differ from the plain run's")
     log(f"kernels per step: inference {infer_ms:.2f} ms; on the {len(rp)} routed points: exact stage {stage_ms:.2f} ms "
         f"(integer terms by packet traversal in index order, decision, full sums); on the same points the public "
-        f"integer entry (per-lane traversal) takes {int_ms:.2f} ms and the full boundary sum {frac_ms:.2f} ms")
+        f"integer entry (rays sorted by their projected origin, then packets) takes {int_ms:.2f} ms and the full "
+        f"boundary sum {frac_ms:.2f} ms")
     if stage_ms > infer_ms:
         achieved = len(rp) * nseg * FLOP_PER_PAIR / (frac_ms * 1e-3) / 1e12
         roof = {"bound": "valu", "kernel": "k_frac_f32_chain (full boundary sum on the routed points)",
