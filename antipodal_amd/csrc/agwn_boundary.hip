@@ -290,29 +290,42 @@ int boundary_dev(T const* verts, int64_t nv, int32_t const* idx, int64_t nidx, T
 
 } // namespace
 
-// LSD radix sort of n uint64 keys over their low `bits` bits (8-bit digits); *sorted points at a or b.
-int radix_sort_u64(uint64_t* a, uint64_t* b, int64_t n, int bits, hipStream_t s, uint64_t** sorted)
+// Scratch of radix_sort_u64_ws for n keys, in uint32 words: the per-tile digit histograms and the 256 digit starts.
+int64_t radix_sort_words(int64_t n) { return 256 * ((n + kTileN - 1) / kTileN) + 256; }
+
+// LSD radix sort of n uint64 keys over their low `bits` bits (8-bit digits) with the caller's scratch
+// (radix_sort_words(n) uint32); only kernel launches, so it can be captured in a graph; *sorted points at a or b.
+int radix_sort_u64_ws(uint64_t* a, uint64_t* b, int64_t n, int bits, uint32_t* ws, hipStream_t s, uint64_t** sorted)
 {
     *sorted = a;
     if (n <= 1)
         return ok;
     AGWN_CHECK(n < (int64_t(1) << 32), "radix sort of %lld keys", (long long)n);
     int const nblocks = int((n + kTileN - 1) / kTileN);
-    dbuf hist, rows;
-    AGWN_TRY(hist.alloc(size_t(256) * nblocks * sizeof(uint32_t), s));
-    AGWN_TRY(rows.alloc(256 * sizeof(uint32_t), s));
+    uint32_t* const hist = ws;
+    uint32_t* const rows = ws + size_t(256) * nblocks;
     for (int shift = 0; shift < bits; shift += 8)
     {
-        k_hist64<<<dim3(unsigned(nblocks)), dim3(kB), 0, s>>>(a, n, shift, hist.as<uint32_t>(), nblocks);
-        k_scan_rows<<<dim3(256), dim3(kB), 0, s>>>(hist.as<uint32_t>(), nblocks, rows.as<uint32_t>());
-        k_scan_digits<<<dim3(1), dim3(kB), 0, s>>>(rows.as<uint32_t>());
-        k_scatter64<<<dim3(unsigned(nblocks)), dim3(kB), 0, s>>>(a, n, shift, hist.as<uint32_t>(),
-                                                                 rows.as<uint32_t const>(), nblocks, b);
+        k_hist64<<<dim3(unsigned(nblocks)), dim3(kB), 0, s>>>(a, n, shift, hist, nblocks);
+        k_scan_rows<<<dim3(256), dim3(kB), 0, s>>>(hist, nblocks, rows);
+        k_scan_digits<<<dim3(1), dim3(kB), 0, s>>>(rows);
+        k_scatter64<<<dim3(unsigned(nblocks)), dim3(kB), 0, s>>>(a, n, shift, hist, rows, nblocks, b);
         AGWN_HIP(hipGetLastError());
         std::swap(a, b);
     }
     *sorted = a;
     return ok;
+}
+
+// The same with scratch of its own; *sorted points at a or b.
+int radix_sort_u64(uint64_t* a, uint64_t* b, int64_t n, int bits, hipStream_t s, uint64_t** sorted)
+{
+    *sorted = a;
+    if (n <= 1)
+        return ok;
+    dbuf ws;
+    AGWN_TRY(ws.alloc(size_t(radix_sort_words(n)) * sizeof(uint32_t), s));
+    return radix_sort_u64_ws(a, b, n, bits, ws.as<uint32_t>(), s, sorted);
 }
 
 namespace

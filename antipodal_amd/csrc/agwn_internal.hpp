@@ -240,8 +240,11 @@ int gather_routed(Src const& src, int64_t const* list, int64_t m, float* pts, hi
 template <class Src>
 int route_order(agwn_mlp const* mlp, Src const& src, int64_t n, int64_t* list, int64_t m, uint64_t* tmp,
                 uint64_t* keys, uint64_t* keys2, hipStream_t s, int64_t const** by_index, uint64_t const** perm);
-// LSD radix sort of n uint64 keys over their low `bits` bits (agwn_boundary.hip); *sorted = a or b
+// LSD radix sort of n uint64 keys over their low `bits` bits (agwn_boundary.hip); *sorted = a or b. The _ws form takes
+// its scratch (radix_sort_words(n) uint32) from the caller and only launches kernels (graph capture).
 int radix_sort_u64(uint64_t* a, uint64_t* b, int64_t n, int bits, hipStream_t s, uint64_t** sorted);
+int64_t radix_sort_words(int64_t n);
+int radix_sort_u64_ws(uint64_t* a, uint64_t* b, int64_t n, int bits, uint32_t* ws, hipStream_t s, uint64_t** sorted);
 int route_apply(int64_t const* list, int64_t m, float const* w, int8_t* labels, unsigned long long* counters,
                 hipStream_t s);
 

@@ -495,6 +495,7 @@ struct batch_mesh_desc
     int32_t use_bvh;
     int32_t pad;
     int64_t q_end; // one past the mesh's last query in the concatenated query array
+    float kr, kscale; // ray sort keys: projected origins quantized over [-kr, kr], scale = (2^bits - 1) / (2 kr)
 };
 
 // Persistent counting kernel over a batch: workgroups take work items (mesh, first query; kTB queries at most, not
@@ -521,6 +522,75 @@ __global__ void __launch_bounds__(kTB)
         if (q < D.q_end)
             out[q] = count_one(pos[3 * q], pos[3 * q + 1], pos[3 * q + 2], dx, dy, dz, D.nodes, D.rec, D.brute,
                                D.nbrute ? *D.nbrute : 0, D.use_bvh, stack, int(threadIdx.x));
+    }
+}
+
+// Batched rays in sorted order (full batches of 2^16 queries or more): every query's key is (index << 32) | (mesh <<
+// mbits_low) | 2D Morton code of its origin projected across the direction (frame u, w; per-mesh quantization), so a
+// sort of the low 32 bits keeps each mesh's queries in its own index range, in ray order; the packet traversal then
+// counts them and the counts go back to the callers' order.
+struct batch_key_frame
+{
+    float u[3], w[3];
+    int axis_bits; // Morton bits per axis
+    int mesh_shift; // 2 * axis_bits
+};
+__global__ void __launch_bounds__(kTB) k_batch_keys(float const* __restrict__ pos,
+                                                   batch_mesh_desc const* __restrict__ meshes,
+                                                   int32_t const* __restrict__ item_mesh,
+                                                   int64_t const* __restrict__ item_q0, batch_key_frame F,
+                                                   uint64_t* __restrict__ keys)
+{
+    int64_t const it = blockIdx.x;
+    int32_t const m = item_mesh[it];
+    batch_mesh_desc const D = meshes[m];
+    int64_t const q = item_q0[it] + threadIdx.x;
+    if (q >= D.q_end)
+        return;
+    float const x = pos[3 * q], y = pos[3 * q + 1], z = pos[3 * q + 2];
+    float const top = float((1u << F.axis_bits) - 1u);
+    float const a = fminf(fmaxf((F.u[0] * x + F.u[1] * y + F.u[2] * z + D.kr) * D.kscale, 0.0f), top); // NaN -> 0
+    float const b = fminf(fmaxf((F.w[0] * x + F.w[1] * y + F.w[2] * z + D.kr) * D.kscale, 0.0f), top);
+    uint32_t const code = spread2_16(uint32_t(a)) | (spread2_16(uint32_t(b)) << 1);
+    keys[q] = (uint64_t(q) << 32) | uint64_t((uint32_t(m) << F.mesh_shift) | code);
+}
+
+// The persistent counting kernel on sorted queries (the same work items: each mesh keeps its index range), by packet
+// traversal: a work item's kTB queries are the block's two waves' packets.
+__global__ void __launch_bounds__(kTB)
+    k_count_batch_packet(float const* __restrict__ pos, batch_mesh_desc const* __restrict__ meshes,
+                         int32_t const* __restrict__ item_mesh, int64_t const* __restrict__ item_q0, int64_t nitems,
+                         unsigned long long* __restrict__ counter, float dx, float dy, float dz,
+                         int32_t* __restrict__ out)
+{
+    __shared__ int32_t stk_node[kTB / 64][kMaxStack];
+    __shared__ uint64_t stk_mask[kTB / 64][kMaxStack];
+    __shared__ unsigned long long cur;
+    int const w = int(threadIdx.x >> 6);
+    while (true)
+    {
+        if (threadIdx.x == 0)
+            cur = atomicAdd(counter, 1ull);
+        __syncthreads();
+        int64_t const it = int64_t(cur);
+        __syncthreads();
+        if (it >= nitems)
+            break;
+        int32_t const m = item_mesh[it];
+        batch_mesh_desc const D = meshes[m];
+        int64_t const q = item_q0[it] + threadIdx.x;
+        bool const valid = q < D.q_end;
+        float px = 0.0f, py = 0.0f, pz = 0.0f;
+        if (valid)
+        {
+            px = pos[3 * q];
+            py = pos[3 * q + 1];
+            pz = pos[3 * q + 2];
+        }
+        int const c = count_packet(px, py, pz, valid, dx, dy, dz, D.nodes, D.rec, D.brute, D.nbrute ? *D.nbrute : 0,
+                                   D.use_bvh, stk_node[w], stk_mask[w], int(threadIdx.x & 63));
+        if (valid)
+            out[q] = c;
     }
 }
 
@@ -1400,6 +1470,11 @@ struct agwn_batch
     std::vector<std::shared_ptr<agwn::dir_state<float>>> states;
     agwn::dbuf cdesc, fdesc, citem_m, citem_q, fitem_m, fitem_q, counters, counts, p32;
     int64_t ncitems = 0, nfitems = 0;
+    // rays in sorted order for the integer term (full batches of kSortRaysMin queries or more)
+    bool sorted = false;
+    int64_t g_sorted_n = 0; // queries of the sorted launch
+    agwn::batch_key_frame kf{};
+    agwn::dbuf skeys, skeys2, sws, spos, scnt;
     hipGraphExec_t exec = nullptr;
     ~agwn_batch()
     {
@@ -1638,6 +1713,46 @@ int batch_prepare(agwn_batch& B, float const* pos, int64_t const* qoff, float co
     AGWN_TRY(up(B.fitem_q, fiq.data(), fiq.size() * sizeof(int64_t)));
     AGWN_TRY(B.counters.alloc(2 * sizeof(unsigned long long), nullptr));
     AGWN_TRY(B.counts.alloc(size_t(std::max<int64_t>(n, 1)) * sizeof(int32_t), nullptr));
+    // sorted rays for the integer term: whole batches only (the tuner's samples leave other queries without keys),
+    // with room for the mesh index and at least 4 Morton bits per axis in the 32-bit sort key
+    int mbits = 0;
+    while ((int64_t(1) << mbits) < k)
+        ++mbits;
+    int const axis_bits = std::min(15, (32 - mbits) / 2); // mesh_shift = 2 axis_bits <= 30: a defined shift
+    B.sorted = which.empty() && n >= kSortRaysMin && n < (int64_t(1) << 32) && axis_bits >= 4;
+    B.g_sorted_n = n;
+    if (B.sorted)
+    {
+        double const e0 = double(x0f[0]), e1 = double(x0f[1]), e2 = double(x0f[2]);
+        double const el = std::sqrt(e0 * e0 + e1 * e1 + e2 * e2);
+        AGWN_CHECK(el > 0.0, "zero ray direction");
+        double const e[3] = {e0 / el, e1 / el, e2 / el};
+        double const h[3] = {std::fabs(e[0]) < 0.9 ? 1.0 : 0.0, std::fabs(e[0]) < 0.9 ? 0.0 : 1.0, 0.0};
+        double u[3] = {e[1] * h[2] - e[2] * h[1], e[2] * h[0] - e[0] * h[2], e[0] * h[1] - e[1] * h[0]};
+        double const ul = std::sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+        for (double& c : u)
+            c /= ul;
+        double const w[3] = {e[1] * u[2] - e[2] * u[1], e[2] * u[0] - e[0] * u[2], e[0] * u[1] - e[1] * u[0]};
+        for (int a = 0; a < 3; ++a)
+        {
+            B.kf.u[a] = float(u[a]);
+            B.kf.w[a] = float(w[a]);
+        }
+        B.kf.axis_bits = axis_bits;
+        B.kf.mesh_shift = 2 * axis_bits;
+        for (int64_t m = 0; m < k; ++m)
+        {
+            double const r = qmax[size_t(m)] > 0.0f ? double(qmax[size_t(m)]) : 1.0;
+            cd[size_t(m)].kr = float(r);
+            cd[size_t(m)].kscale = float(double((1u << axis_bits) - 1u) / (2.0 * r));
+        }
+        AGWN_TRY(B.skeys.alloc(size_t(n) * sizeof(uint64_t), nullptr));
+        AGWN_TRY(B.skeys2.alloc(size_t(n) * sizeof(uint64_t), nullptr));
+        AGWN_TRY(B.sws.alloc(size_t(radix_sort_words(n)) * sizeof(uint32_t), nullptr));
+        AGWN_TRY(B.spos.alloc(size_t(n) * 3 * sizeof(float), nullptr));
+        AGWN_TRY(B.scnt.alloc(size_t(n) * sizeof(int32_t), nullptr));
+        AGWN_HIP(hipMemcpyAsync(B.cdesc.p, cd.data(), cd.size() * sizeof(batch_mesh_desc), hipMemcpyHostToDevice, s));
+    }
     int64_t const npts = B.pt_off[size_t(k)];
     AGWN_TRY(B.p32.alloc(size_t(std::max<int64_t>(npts, 1)) * 2 * sizeof(float4), nullptr));
     AGWN_HIP(hipStreamSynchronize(s));
@@ -1655,12 +1770,37 @@ int batch_launches(agwn_batch& B, float const* pos, float const x0f[3], int vari
     AGWN_HIP(hipGetLastError());
     if (B.ncitems > 0)
     {
-        // 10 traversal blocks per CU: the 14 KiB LDS stacks allow 11; 5 waves per SIMD hide the traversal's loads
-        int const blocks = int(std::min<int64_t>(B.ncitems, int64_t(B.ncu) * 10));
-        k_count_batch<<<dim3(unsigned(blocks)), dim3(kTB), 0, s>>>(
-            pos, B.cdesc.as<batch_mesh_desc>(), B.citem_m.as<int32_t>(), B.citem_q.as<int64_t>(), B.ncitems,
-            B.counters.as<unsigned long long>(), x0f[0], x0f[1], x0f[2], B.counts.as<int32_t>());
-        AGWN_HIP(hipGetLastError());
+        if (B.sorted)
+        {
+            int64_t const n = B.g_sorted_n;
+            k_batch_keys<<<dim3(unsigned(B.ncitems)), dim3(kTB), 0, s>>>(pos, B.cdesc.as<batch_mesh_desc>(),
+                                                                        B.citem_m.as<int32_t>(),
+                                                                        B.citem_q.as<int64_t>(), B.kf,
+                                                                        B.skeys.as<uint64_t>());
+            AGWN_HIP(hipGetLastError());
+            uint64_t* sorted = nullptr;
+            AGWN_TRY(radix_sort_u64_ws(B.skeys.as<uint64_t>(), B.skeys2.as<uint64_t>(), n, 32, B.sws.as<uint32_t>(),
+                                       s, &sorted));
+            unsigned const g = unsigned((n + kB - 1) / kB);
+            k_ray_gather<float><<<dim3(g), dim3(kB), 0, s>>>(sorted, n, pos, B.spos.as<float>());
+            AGWN_HIP(hipGetLastError());
+            int const blocks = int(std::min<int64_t>(B.ncitems, int64_t(B.ncu) * 16));
+            k_count_batch_packet<<<dim3(unsigned(blocks)), dim3(kTB), 0, s>>>(
+                B.spos.as<float>(), B.cdesc.as<batch_mesh_desc>(), B.citem_m.as<int32_t>(), B.citem_q.as<int64_t>(),
+                B.ncitems, B.counters.as<unsigned long long>(), x0f[0], x0f[1], x0f[2], B.scnt.as<int32_t>());
+            AGWN_HIP(hipGetLastError());
+            k_ray_scatter<<<dim3(g), dim3(kB), 0, s>>>(sorted, n, B.scnt.as<int32_t>(), B.counts.as<int32_t>());
+            AGWN_HIP(hipGetLastError());
+        }
+        else
+        {
+            // 10 traversal blocks per CU: the 14 KiB LDS stacks allow 11; 5 waves per SIMD hide the traversal's loads
+            int const blocks = int(std::min<int64_t>(B.ncitems, int64_t(B.ncu) * 10));
+            k_count_batch<<<dim3(unsigned(blocks)), dim3(kTB), 0, s>>>(
+                pos, B.cdesc.as<batch_mesh_desc>(), B.citem_m.as<int32_t>(), B.citem_q.as<int64_t>(), B.ncitems,
+                B.counters.as<unsigned long long>(), x0f[0], x0f[1], x0f[2], B.counts.as<int32_t>());
+            AGWN_HIP(hipGetLastError());
+        }
     }
     if (B.nfitems > 0)
     {

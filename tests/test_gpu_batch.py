@@ -17,7 +17,7 @@ from antipodal_amd import meshgen as mg
 pytestmark = pytest.mark.gpu
 
 
-def make_batch(k: int, seed: int):
+def make_batch(k: int, seed: int, qmin: int = 1):
     rng = np.random.default_rng(seed)
     meshes, queries = [], []
     for m in range(k):
@@ -29,7 +29,7 @@ def make_batch(k: int, seed: int):
             idx = np.zeros(0, dtype=np.int32)  # no triangles
         v = v.astype(np.float32)
         b = mg_boundary(v, idx)
-        nq = 0 if m == 7 else int(math.exp(rng.uniform(math.log(1), math.log(20000))))
+        nq = 0 if m == 7 else int(math.exp(rng.uniform(math.log(qmin), math.log(20000))))
         lo, hi = mg.expanded_bbox(v)
         q = (lo + (hi - lo) * rng.random((nq, 3))).astype(np.float32)
         meshes.append((v, idx, b))
@@ -91,3 +91,32 @@ def test_batch_rejects_general_weights(gpu):
     b[:, 6] = 0.5
     with pytest.raises(gpu.AgwnError, match="integer boundary weights"):
         gpu.HipMeshBatch([(v, idx, b)])
+
+
+def test_sorted_batch_counts_match_single_mesh_path(gpu):
+    """A whole batch of 2^16 queries or more sorts its rays for the integer term (per mesh, by the projected origin)
+    and counts them in packets: every query's full GWN still equals the single-mesh path's bit for bit, on replay
+    too."""
+    import torch
+
+    meshes, queries = make_batch(24, seed=9, qmin=4000)
+    x0 = mg.example_x0(np.float32)
+    q_off = np.concatenate([[0], np.cumsum([len(q) for q in queries])]).astype(np.int64)
+    assert q_off[-1] >= 1 << 16
+    pos = torch.from_numpy(np.concatenate(queries)).cuda()
+    with gpu.HipMeshBatch(meshes) as batch:
+        out = batch.eval(pos, q_off, x0)
+        got = out.cpu().numpy()
+        for m, (v, idx, b) in enumerate(meshes):
+            if q_off[m + 1] == q_off[m] or len(idx) == 0:
+                assert np.all(got[q_off[m]: q_off[m + 1]] == 0.0) or len(idx) > 0
+                continue
+            p = pos[q_off[m]: q_off[m + 1]].contiguous()
+            want = torch.empty(len(p), dtype=torch.float32, device="cuda")
+            with gpu.HipMesh(v, idx, dtype=np.float32) as mesh:
+                seg = torch.from_numpy(b).cuda() if len(b) else None
+                mesh.full(p, x0, seg, out=want)
+            np.testing.assert_array_equal(got[q_off[m]: q_off[m + 1]], want.cpu().numpy(), err_msg=f"mesh {m}")
+        out2 = torch.full_like(out, -7.0)
+        batch.eval(pos, q_off, x0, out=out2)
+        assert torch.equal(out, out2)
