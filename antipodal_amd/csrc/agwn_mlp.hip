@@ -497,21 +497,37 @@ struct route_params
     surface_mask mask;
 };
 
-__device__ __forceinline__ bool near_surface(surface_mask const& m, float x, float y, float z)
+// the near-surface test in two halves: the fetch of the mask word that holds the point's voxel (a point outside the box,
+// or NaN, gets the word 1 and bit 0: near) and the bit test, so that a caller can issue the load early and test late
+__device__ __forceinline__ void near_surface_fetch(surface_mask const& m, float x, float y, float z, uint32_t& word,
+                                                   uint32_t& bit)
 {
+    word = 0u;
+    bit = 0u;
     if (!m.bits)
-        return false;
+        return;
     float const f[3] = {(x - m.lo[0]) * m.inv[0], (y - m.lo[1]) * m.inv[1], (z - m.lo[2]) * m.inv[2]};
     int iv[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a)
     {
         if (!(f[a] >= 0.0f && f[a] < float(m.n[a]))) // outside the box, or NaN
-            return true;
+        {
+            word = 1u;
+            return;
+        }
         iv[a] = min(int(f[a]), m.n[a] - 1);
     }
     int64_t const v = (int64_t(iv[2]) * m.n[1] + iv[1]) * m.n[0] + iv[0];
-    return (m.bits[v >> 5] >> (v & 31)) & 1u;
+    word = m.bits[v >> 5];
+    bit = uint32_t(v & 31);
+}
+
+__device__ __forceinline__ bool near_surface(surface_mask const& m, float x, float y, float z)
+{
+    uint32_t w, b;
+    near_surface_fetch(m, x, y, z, w, b);
+    return (w >> b) & 1u;
 }
 
 // Marks the voxels within eps of triangle t: every voxel of the triangle's bounding box grown by eps that also passes
@@ -600,6 +616,20 @@ __global__ void __launch_bounds__(256, kInferBlocks) k_mlp_infer(Src const src, 
     int const lane = threadIdx.x & 63;
     int const wave = threadIdx.x >> 6;
     int64_t const base = int64_t(blockIdx.x) * kTileP;
+
+    // the routing's near-surface mask word, fetched now and tested after the output layer, so that the MLP hides the
+    // load's latency
+    uint32_t near_word = 0u, near_bit = 0u;
+    if constexpr (ROUTE)
+    {
+        int64_t const i = base + threadIdx.x;
+        if (threadIdx.x < kTileP && i < n)
+        {
+            float x, y, z;
+            query_at(src, i, x, y, z);
+            near_surface_fetch(RP.mask, x, y, z, near_word, near_bit);
+        }
+    }
 
     // features: a point's 96 pairs in 8 parts of 12; work item (point p, part) -> thread item % 256 (threads of a wave
     // = consecutive points), each part written 4 pairs (16 B) at a time -- the rows' 528 B stride keeps each 16-lane
@@ -738,13 +768,7 @@ __global__ void __launch_bounds__(256, kInferBlocks) k_mlp_infer(Src const src, 
         pred[i] = pv;
     if constexpr (ROUTE)
     {
-        bool near = false;
-        if (live)
-        {
-            float x, y, z;
-            query_at(src, i, x, y, z);
-            near = near_surface(RP.mask, x, y, z);
-        }
+        bool const near = (near_word >> near_bit) & 1u;
         float const d = fabsf(pv - 0.5f);
         bool const band = live && !near && !(d > RP.delta); // NaN routes too
         bool const audit = live && !near && !band && RP.audit_mod != 0 &&
