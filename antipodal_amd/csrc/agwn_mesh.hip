@@ -1054,10 +1054,18 @@ struct agwn_grid
     float* seg7 = nullptr;
     int64_t nseg = 0;
     std::unique_ptr<agwn::frac_plan> plan; // the boundary's chain plan, built once per scene
+    // Routed-query list of the labelling calls (8 bytes per query), kept between calls and grown when a call needs
+    // more: allocating and freeing it per call costs about 3 ms at 512^3 (the free of a 1 GiB block). The calls use it
+    // one at a time under mu, and each finishes its device work before it returns.
+    void* label_list = nullptr;
+    size_t label_list_bytes = 0;
+    int label_list_dev = -1;
     ~agwn_grid()
     {
         if (seg7)
             (void)hipFree(seg7);
+        if (label_list)
+            (void)hipFree(label_list);
     }
 };
 
@@ -1257,7 +1265,8 @@ int label_exact_levels(float const* pts, int32_t const* cnt, int64_t m, frac_pla
 template <class Src>
 int label_surrogate(mesh_t<float>& mesh, agwn_mlp const* mlp, Src const& src, int64_t n, double qmax,
                     float const x0f[3], float const* seg7, int64_t nseg, frac_plan const* plan, float delta,
-                    int64_t audit_every, uint64_t audit_seed, int8_t* labels, int64_t* stats, hipStream_t s)
+                    int64_t audit_every, uint64_t audit_seed, int8_t* labels, int64_t* stats, hipStream_t s,
+                    int64_t* list_ws = nullptr)
 {
     AGWN_CHECK(mlp != nullptr, "mlp handle is null");
     AGWN_CHECK(delta >= 0.0f, "delta must be >= 0 (+inf routes every point)"); // rejects NaN
@@ -1267,12 +1276,18 @@ int label_surrogate(mesh_t<float>& mesh, agwn_mlp const* mlp, Src const& src, in
     unsigned long long cnt[5] = {0, 0, 0, 0, 0};
     if (n > 0)
     {
-        dbuf dc, list;
+        // the routed list has room for every query (list_ws: n entries of the caller's, else a buffer of this call)
+        dbuf dc, list_own;
         AGWN_TRY(dc.alloc(sizeof(cnt), s));
-        AGWN_TRY(list.alloc(size_t(n) * sizeof(int64_t), s));
+        int64_t* list = list_ws;
+        if (!list)
+        {
+            AGWN_TRY(list_own.alloc(size_t(n) * sizeof(int64_t), s));
+            list = list_own.as<int64_t>();
+        }
         AGWN_HIP(hipMemsetAsync(dc.p, 0, sizeof(cnt), s));
         AGWN_TRY(mlp_route(mlp, src, n, delta, uint32_t(audit_every), uint32_t(audit_seed ^ (audit_seed >> 32)), labels,
-                           list.as<int64_t>(), dc.as<unsigned long long>(), s));
+                           list, dc.as<unsigned long long>(), s));
         AGWN_HIP(hipMemcpyAsync(cnt, dc.p, sizeof(cnt), hipMemcpyDeviceToHost, s));
         AGWN_HIP(hipStreamSynchronize(s));
         int64_t const m = int64_t(cnt[3]);
@@ -1287,7 +1302,7 @@ int label_surrogate(mesh_t<float>& mesh, agwn_mlp const* mlp, Src const& src, in
             AGWN_TRY(keys2.alloc(size_t(m) * sizeof(uint64_t), s));
             int64_t const* Li = nullptr;
             uint64_t const* perm = nullptr;
-            AGWN_TRY(route_order(mlp, src, n, list.as<int64_t>(), m, tmp.as<uint64_t>(), keys.as<uint64_t>(),
+            AGWN_TRY(route_order(mlp, src, n, list, m, tmp.as<uint64_t>(), keys.as<uint64_t>(),
                                  keys2.as<uint64_t>(), s, &Li, &perm));
             double const x0d[3] = {double(x0f[0]), double(x0f[1]), double(x0f[2])};
             dbuf pts, c, w, pts_i, c_i, lm;
@@ -1358,8 +1373,27 @@ int grid_label_dev(agwn_grid* g, agwn_mlp const* mlp, float const* start, float 
     AGWN_TRY(grid_setup(start, dx, dy, dz, nx, ny, nz, c));
     std::lock_guard<std::mutex> lk(g->mu);
     AGWN_CHECK(g->mesh != nullptr, "set_scene was not called");
-    return label_surrogate(*g->mesh, mlp, src_grid{c.gd}, c.n, c.qmax, c.x0f, g->seg7, g->nseg, g->plan.get(), delta,
-                           audit_every, audit_seed, labels, stats, s);
+    size_t const need = size_t(c.n) * sizeof(int64_t);
+    int dev = 0;
+    AGWN_HIP(hipGetDevice(&dev));
+    if (g->label_list && (g->label_list_bytes < need || g->label_list_dev != dev))
+    {
+        AGWN_HIP(hipFree(g->label_list));
+        g->label_list = nullptr;
+        g->label_list_bytes = 0;
+    }
+    if (!g->label_list && need > 0)
+    {
+        AGWN_HIP(hipMalloc(&g->label_list, need));
+        g->label_list_bytes = need;
+        g->label_list_dev = dev;
+    }
+    int const rc = label_surrogate(*g->mesh, mlp, src_grid{c.gd}, c.n, c.qmax, c.x0f, g->seg7, g->nseg,
+                                   g->plan.get(), delta, audit_every, audit_seed, labels, stats, s,
+                                   static_cast<int64_t*>(g->label_list));
+    if (rc != ok)
+        (void)hipStreamSynchronize(s); // a failed call may have left work queued on the list: drain it
+    return rc;
 }
 
 int grid_eval_host(agwn_grid* g, float const* start, float const* dx, float const* dy, float const* dz, int nx, int ny,

@@ -194,6 +194,40 @@ def test_grid_labels_follow_the_routing_rule(gpu, c1, trained_c1):
         assert torch.equal(lab_all, exact)
 
 
+def test_grid_labels_across_call_sizes(gpu, c1, trained_c1):
+    """A grid handle keeps its routed-query list between labelling calls and grows it when a call needs more: calls of
+    96^3, then a smaller ragged 40 x 33 x 17 lattice, then 128^3 on one handle each give the routing rule's labels
+    (exact where near or in the band, the network's elsewhere), and with delta = +inf the exact labels."""
+    import torch
+
+    v, idx, lo, hi = c1
+    net, delta, info, bnd, x0 = trained_c1
+    f = np.float32
+    with gpu.HipGridGwn() as grid:
+        grid.set_scene(v.astype(np.float32), idx, bnd)
+        for nx, ny, nz in ((96, 96, 96), (40, 33, 17), (128, 128, 128)):
+            print(f"lattice {nx} x {ny} x {nz}", flush=True)
+            start = lo.astype(f)
+            dx, dy, dz = (np.eye(3, dtype=f)[k] * f((hi[k] - lo[k]) / (n - 1)) for k, n in enumerate((nx, ny, nz)))
+            pts = (((start[None, None, None, :] + dx * np.arange(nx, dtype=f)[None, None, :, None])
+                    + dy * np.arange(ny, dtype=f)[None, :, None, None])
+                   + dz * np.arange(nz, dtype=f)[:, None, None, None]).astype(f).reshape(-1, 3)
+            P = torch.from_numpy(pts).cuda()
+            w = torch.empty(nx * ny * nz, dtype=torch.float32, device="cuda")
+            grid.eval_grid(start, dx, dy, dz, nx, ny, nz, out=w)
+            exact = (w > 0.5).to(torch.int8)
+            near = net.near_surface(P).bool()
+            pred = net.predict(P)
+            band = ~near & ((pred - 0.5).abs() <= delta)
+            want = torch.where(near | band, exact, (pred > 0.5).to(torch.int8))
+            lab, st = net.label_grid(grid, start, dx, dy, dz, nx, ny, nz, delta)
+            assert torch.equal(lab, want)
+            assert st["routed"] == int((near | band).sum())
+            lab_all, st_all = net.label_grid(grid, start, dx, dy, dz, nx, ny, nz, float("inf"))
+            assert st_all["routed"] == nx * ny * nz
+            assert torch.equal(lab_all, exact)
+
+
 def test_point_labels_and_audit(gpu, c1, trained_c1):
     import torch
 
