@@ -1079,6 +1079,7 @@ void loop_tree(std::vector<loop_host> const& loops, std::vector<tree_node>& node
 
 int frac_plan_build(float const* h, int64_t nseg, frac_plan& plan, hipStream_t stream)
 {
+    plan.dec_state.reset();
     plan.chain = false;
     plan.npts = plan.nunit = 0;
     plan.loops.clear();
@@ -1127,6 +1128,108 @@ int frac_plan_build(float const* h, int64_t nseg, frac_plan& plan, hipStream_t s
 // the decision margin covers the FP32 evaluation's own error (measured <= 4e-6 on 430k-segment boundaries)
 constexpr double kDecideMargin = 1e-4;
 
+// The decision's tables for one pole x0: the loop tree's records in the pole frame (device) and the decision stream
+// rotated into that frame (device). They depend on the boundary and x0 only, so a plan keeps them for up to
+// kDecidePoles poles (further poles build theirs per call).
+struct decide_tables
+{
+    double x0[3] = {0.0, 0.0, 0.0};
+    loop_rec* tab = nullptr;
+    int ntab = 0;
+    float4* p32 = nullptr;
+    decide_tables() = default;
+    decide_tables(decide_tables const&) = delete;
+    decide_tables& operator=(decide_tables const&) = delete;
+    ~decide_tables()
+    {
+        if (tab)
+            (void)hipFree(tab);
+        if (p32)
+            (void)hipFree(p32);
+    }
+};
+struct decide_state
+{
+    std::vector<tree_node> nodes; // the loop tree (no pole involved)
+    std::vector<std::unique_ptr<decide_tables>> poles;
+};
+constexpr size_t kDecidePoles = 8;
+
+// a record in the pole frame R; its ball grown by 0.1 % (+ 1e-6 |c|) and its bound numerator by 0.1 % so that the
+// FP32 tests stay conservative
+loop_rec pole_record(frame const& R, loop_host const& L, int64_t start, int64_t count)
+{
+    loop_rec r{};
+    double c[3], a[3], t[9], mr[9];
+    for (int i = 0; i < 3; ++i)
+    {
+        c[i] = R.r[3 * i] * L.c[0] + R.r[3 * i + 1] * L.c[1] + R.r[3 * i + 2] * L.c[2];
+        a[i] = R.r[3 * i] * L.a[0] + R.r[3 * i + 1] * L.a[1] + R.r[3 * i + 2] * L.a[2];
+    }
+    for (int i = 0; i < 3; ++i) // t = R M
+        for (int j = 0; j < 3; ++j)
+            t[3 * i + j] = R.r[3 * i] * L.m[j] + R.r[3 * i + 1] * L.m[3 + j] + R.r[3 * i + 2] * L.m[6 + j];
+    for (int i = 0; i < 3; ++i) // mr = t R^T
+        for (int j = 0; j < 3; ++j)
+            mr[3 * i + j] = t[3 * i] * R.r[3 * j] + t[3 * i + 1] * R.r[3 * j + 1] + t[3 * i + 2] * R.r[3 * j + 2];
+    double const cn = std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+    double const rho = L.rho * 1.001 + 1e-6 * cn;
+    r.cx = float(c[0]);
+    r.cy = float(c[1]);
+    r.cz = float(c[2]);
+    r.rho = float(rho);
+    r.ax = float(a[0]);
+    r.ay = float(a[1]);
+    r.az = float(a[2]);
+    r.k3 = float(3.0 * rho * rho * L.area * 1.001);
+    for (int i = 0; i < 9; ++i)
+        r.m[i] = float(mr[i]);
+    r.trm = float(mr[0] + mr[4] + mr[8]);
+    r.cnorm = float(cn);
+    r.start = int32_t(start);
+    r.count = int32_t(count);
+    return r;
+}
+
+// Builds the tables of pole x0 (frame R) on stream s and waits for them: once kept by the plan they are read by other
+// streams' calls.
+int build_decide_tables(frac_plan const& plan, std::vector<tree_node> const& nodes, double const x0[3],
+                        frame const& R, hipStream_t s, decide_tables& t)
+{
+    for (int a = 0; a < 3; ++a)
+        t.x0[a] = x0[a];
+    std::vector<loop_rec> tab(nodes.size());
+    for (size_t k = 0; k < nodes.size(); ++k)
+    {
+        tree_node const& N = nodes[k];
+        if (N.loop >= 0)
+        {
+            loop_host const& L = plan.loops[size_t(N.loop)];
+            tab[k] = pole_record(R, L, L.start, L.count);
+            tab[k].leaf = 1;
+        }
+        else
+        {
+            tab[k] = pole_record(R, N.L, 0, 0);
+            tab[k].leaf = 0;
+        }
+        tab[k].skip = N.skip;
+    }
+    t.ntab = int(tab.size());
+    AGWN_HIP(hipMalloc(&t.tab, std::max<size_t>(tab.size(), 1) * sizeof(loop_rec)));
+    if (!tab.empty())
+        AGWN_HIP(hipMemcpyAsync(t.tab, tab.data(), tab.size() * sizeof(loop_rec), hipMemcpyHostToDevice, s));
+    AGWN_HIP(hipMalloc(&t.p32, size_t(std::max<int64_t>(plan.dec_npts, 1)) * 2 * sizeof(float4)));
+    if (plan.dec_npts > 0)
+    {
+        k_prep_stream<<<dim3(unsigned((plan.dec_npts + kBlock - 1) / kBlock)), dim3(kBlock), 0, s>>>(
+            plan.dec.as<float4 const>(), plan.dec_npts, R, t.p32);
+        AGWN_HIP(hipGetLastError());
+    }
+    AGWN_HIP(hipStreamSynchronize(s)); // the host table goes out of scope; the tables may be shared from now on
+    return ok;
+}
+
 int frac_decide(float const* pos, int64_t m, int32_t const* counts, frac_plan const& plan, double const x0[3],
                 float beta, float* val, int64_t* undecided, int64_t& nund, hipStream_t s)
 {
@@ -1138,81 +1241,43 @@ int frac_decide(float const* pos, int64_t m, int32_t const* counts, frac_plan co
     AGWN_CHECK(plan.loops.size() < size_t(INT32_MAX) && plan.dec_npts < INT32_MAX, "boundary too large for the decision");
     frame const R = make_frame_impl(x0[0], x0[1], x0[2]);
     pole const P = make_pole(x0[0], x0[1], x0[2]);
-    // a record in the pole frame; its ball grown by 0.1 % (+ 1e-6 |c|) and its bound numerator by 0.1 % so that the
-    // FP32 tests stay conservative
-    auto to_rec = [&](loop_host const& L, int64_t start, int64_t count) {
-        loop_rec r{};
-        double c[3], a[3], t[9], mr[9];
-        for (int i = 0; i < 3; ++i)
-        {
-            c[i] = R.r[3 * i] * L.c[0] + R.r[3 * i + 1] * L.c[1] + R.r[3 * i + 2] * L.c[2];
-            a[i] = R.r[3 * i] * L.a[0] + R.r[3 * i + 1] * L.a[1] + R.r[3 * i + 2] * L.a[2];
-        }
-        for (int i = 0; i < 3; ++i) // t = R M
-            for (int j = 0; j < 3; ++j)
-                t[3 * i + j] = R.r[3 * i] * L.m[j] + R.r[3 * i + 1] * L.m[3 + j] + R.r[3 * i + 2] * L.m[6 + j];
-        for (int i = 0; i < 3; ++i) // mr = t R^T
-            for (int j = 0; j < 3; ++j)
-                mr[3 * i + j] = t[3 * i] * R.r[3 * j] + t[3 * i + 1] * R.r[3 * j + 1] + t[3 * i + 2] * R.r[3 * j + 2];
-        double const cn = std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
-        double const rho = L.rho * 1.001 + 1e-6 * cn;
-        r.cx = float(c[0]);
-        r.cy = float(c[1]);
-        r.cz = float(c[2]);
-        r.rho = float(rho);
-        r.ax = float(a[0]);
-        r.ay = float(a[1]);
-        r.az = float(a[2]);
-        r.k3 = float(3.0 * rho * rho * L.area * 1.001);
-        for (int i = 0; i < 9; ++i)
-            r.m[i] = float(mr[i]);
-        r.trm = float(mr[0] + mr[4] + mr[8]);
-        r.cnorm = float(cn);
-        r.start = int32_t(start);
-        r.count = int32_t(count);
-        return r;
-    };
-    std::vector<tree_node> nodes;
-    loop_tree(plan.loops, nodes);
-    std::vector<loop_rec> tab(nodes.size());
-    for (size_t k = 0; k < nodes.size(); ++k)
+    decide_tables const* T = nullptr;
+    std::unique_ptr<decide_tables> own; // a pole beyond the kept ones: this call's tables
     {
-        tree_node const& T = nodes[k];
-        if (T.loop >= 0)
+        std::lock_guard<std::mutex> lk(plan.dec_mu);
+        if (!plan.dec_state)
         {
-            loop_host const& L = plan.loops[size_t(T.loop)];
-            tab[k] = to_rec(L, L.start, L.count);
-            tab[k].leaf = 1;
+            auto st = std::make_shared<decide_state>();
+            loop_tree(plan.loops, st->nodes);
+            plan.dec_state = st;
         }
-        else
+        auto* st = static_cast<decide_state*>(plan.dec_state.get());
+        for (auto const& e : st->poles)
+            if (e->x0[0] == x0[0] && e->x0[1] == x0[1] && e->x0[2] == x0[2])
+                T = e.get();
+        if (!T)
         {
-            tab[k] = to_rec(T.L, 0, 0);
-            tab[k].leaf = 0;
+            auto e = std::make_unique<decide_tables>();
+            AGWN_TRY(build_decide_tables(plan, st->nodes, x0, R, s, *e));
+            T = e.get();
+            if (st->poles.size() < kDecidePoles)
+                st->poles.push_back(std::move(e));
+            else
+                own = std::move(e);
         }
-        tab[k].skip = T.skip;
     }
-    dbuf dt, p32, dn;
-    AGWN_TRY(dt.alloc(std::max<size_t>(tab.size(), 1) * sizeof(loop_rec), s));
-    if (!tab.empty())
-        AGWN_HIP(hipMemcpyAsync(dt.p, tab.data(), tab.size() * sizeof(loop_rec), hipMemcpyHostToDevice, s));
-    AGWN_TRY(p32.alloc(size_t(std::max<int64_t>(plan.dec_npts, 1)) * 2 * sizeof(float4), s));
-    if (plan.dec_npts > 0)
-    {
-        k_prep_stream<<<dim3(unsigned((plan.dec_npts + kBlock - 1) / kBlock)), dim3(kBlock), 0, s>>>(
-            plan.dec.as<float4 const>(), plan.dec_npts, R, p32.as<float4>());
-        AGWN_HIP(hipGetLastError());
-    }
+    dbuf dn;
     AGWN_TRY(dn.alloc(sizeof(unsigned long long), s));
     AGWN_HIP(hipMemsetAsync(dn.p, 0, sizeof(unsigned long long), s));
     int64_t const blocks = (m + kDecideBlock - 1) / kDecideBlock;
     AGWN_CHECK(blocks < (int64_t(1) << 31), "frac_decide: too many queries (%lld)", (long long)m);
     k_frac_decide<<<dim3(unsigned(blocks)), dim3(kDecideBlock), 0, s>>>(
-        pos, m, counts, dt.as<loop_rec>(), int(tab.size()), p32.as<float4>(), R, P, kTau, beta * kFourPiF,
+        pos, m, counts, T->tab, T->ntab, T->p32, R, P, kTau, beta * kFourPiF,
         kDecideMargin, val, undecided, dn.as<unsigned long long>());
     AGWN_HIP(hipGetLastError());
     unsigned long long h = 0;
     AGWN_HIP(hipMemcpyAsync(&h, dn.p, sizeof(h), hipMemcpyDeviceToHost, s));
-    AGWN_HIP(hipStreamSynchronize(s)); // the host table goes out of scope; nund is needed now
+    AGWN_HIP(hipStreamSynchronize(s)); // nund is needed now (and `own` may be freed)
     nund = int64_t(h);
     return ok;
 }

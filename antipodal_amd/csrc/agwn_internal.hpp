@@ -6,6 +6,8 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -194,6 +196,10 @@ struct frac_plan
     bool loops_ok = false;
     dbuf dec;                     // decision stream: the loops' points, each loop contiguous (flag 0 at its start)
     int64_t dec_npts = 0;
+    // the decision's loop tree and, per pole, its device tables: built by the plan's first decision for that pole
+    // (under dec_mu) and kept with the plan, read-only afterwards (agwn_frac.hip)
+    mutable std::mutex dec_mu;
+    mutable std::shared_ptr<void> dec_state;
     float4 const* pts() const { return buf.as<float4 const>(); }
 };
 // Builds the plan from a host copy of the boundary (stream-ordered upload on `stream`; returns when it is done).

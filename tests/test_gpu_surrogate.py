@@ -228,6 +228,43 @@ def test_grid_labels_across_call_sizes(gpu, c1, trained_c1):
             assert torch.equal(lab_all, exact)
 
 
+def test_grid_labels_over_many_poles(gpu, c1, trained_c1):
+    """The far-field decision keeps its tables per pole with the boundary's plan (eight poles; a ninth builds its own
+    per call). Nine lattices with seeded random orientations -- nine integration axes, so nine poles -- labelled on one
+    grid handle with every point routed give the exact labels of eval_grid on the same lattice; the first lattice,
+    labelled again at the end, gives the same labels as the first time."""
+    import torch
+
+    v, idx, lo, hi = c1
+    net, delta, info, bnd, x0 = trained_c1
+    rng = np.random.default_rng(12)
+    nres = 40
+    centre = 0.5 * (lo + hi)
+    h = float(np.max(hi - lo)) / (nres - 1)
+    with gpu.HipGridGwn() as grid:
+        grid.set_scene(v.astype(np.float32), idx, bnd)
+        first = None
+        frames = []
+        for k in range(9):
+            q, _ = np.linalg.qr(rng.standard_normal((3, 3)))
+            frames.append(q.astype(np.float32) * np.float32(h))
+        frames.append(frames[0])
+        for k, ax in enumerate(frames):
+            print(f"orientation {k}", flush=True)
+            dx, dy, dz = ax[0], ax[1], ax[2]
+            start = (centre - 0.5 * (nres - 1) * (dx + dy + dz)).astype(np.float32)
+            w = torch.empty(nres ** 3, dtype=torch.float32, device="cuda")
+            grid.eval_grid(start, dx, dy, dz, nres, nres, nres, out=w)
+            exact = (w > 0.5).to(torch.int8)
+            lab, st = net.label_grid(grid, start, dx, dy, dz, nres, nres, nres, float("inf"))
+            assert st["routed"] == nres ** 3
+            assert torch.equal(lab, exact), k
+            if k == 0:
+                first = lab.clone()
+            if k == len(frames) - 1:
+                assert torch.equal(lab, first)
+
+
 def test_point_labels_and_audit(gpu, c1, trained_c1):
     import torch
 
