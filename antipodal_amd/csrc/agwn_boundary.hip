@@ -140,27 +140,41 @@ __global__ void __launch_bounds__(kB) k_scan_digits(uint32_t* __restrict__ rowsu
     rowsum[threadIdx.x] = e;
 }
 
-// Stable scatter of one digit pass: within a tile row, lanes with equal digits are ranked by a ballot match; waves
-// of the block are ordered through per-wave digit counts in LDS.
+// Stable scatter of one digit pass. Each wave ranks its own contiguous chunk of kItems rows of 64 keys: lanes with
+// equal digits are matched by ballots, and the wave's running per-digit counts live in LDS (the lowest lane of each
+// match group advances them). The block then orders its waves per digit, places the tile's keys in LDS in digit
+// order (stable: wave, row, lane is index order) and writes the tile out in that order, so that each digit's run of
+// keys goes to consecutive addresses.
 __global__ void __launch_bounds__(kB) k_scatter64(uint64_t const* __restrict__ kin, int64_t n, int shift,
                                                   uint32_t const* __restrict__ offs,
                                                   uint32_t const* __restrict__ rowbase, int nblocks,
                                                   uint64_t* __restrict__ kout)
 {
     constexpr int kWaves = kB / 64;
-    __shared__ uint32_t run[256];
-    __shared__ uint32_t wcnt[kWaves][256];
+    constexpr int kChunk = kItems * 64;
+    static_assert(kB == 256, "one thread per digit");
+    __shared__ uint64_t tile[kTileN];
+    __shared__ uint32_t wc[kWaves][256]; // a wave's running count of each digit, then its offset within the digit
+    __shared__ uint32_t lstart[256];     // the digit's first slot in the tile
+    __shared__ uint32_t gdelta[256];     // the digit's first output index minus lstart
+    __shared__ uint32_t ws[kWaves];
     int const lane = threadIdx.x & 63;
     int const wave = threadIdx.x >> 6;
-    run[threadIdx.x] = offs[int64_t(threadIdx.x) * nblocks + blockIdx.x] + rowbase[threadIdx.x];
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w)
+        wc[w][threadIdx.x] = 0;
+    __syncthreads();
     uint64_t const lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    int64_t const base = int64_t(blockIdx.x) * kTileN;
+    int64_t const base = int64_t(blockIdx.x) * kTileN + int64_t(wave) * kChunk;
+    uint64_t key[kItems];
+    uint32_t rank[kItems]; // within the wave's keys of the same digit
+#pragma unroll
     for (int r = 0; r < kItems; ++r)
     {
-        int64_t const i = base + int64_t(r) * kB + threadIdx.x;
+        int64_t const i = base + int64_t(r) * 64 + lane;
         bool const valid = i < n;
-        uint64_t const key = valid ? kin[i] : 0ull;
-        uint32_t const d = uint32_t(key >> shift) & 255u;
+        key[r] = valid ? kin[i] : 0ull;
+        uint32_t const d = uint32_t(key[r] >> shift) & 255u;
         uint64_t m = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < 8; ++b)
@@ -169,26 +183,41 @@ __global__ void __launch_bounds__(kB) k_scatter64(uint64_t const* __restrict__ k
             uint64_t const bb = __ballot(bit);
             m &= bit ? bb : ~bb;
         }
-        for (int w = 0; w < kWaves; ++w)
-            wcnt[w][threadIdx.x] = 0;
-        __syncthreads();
+        // every lane reads before any leader writes: one LDS read and one write instruction of the same wave
+        uint32_t const before = wc[wave][d];
+        rank[r] = before + uint32_t(__popcll(m & lt));
         if (valid && (m & lt) == 0)
-            wcnt[wave][d] = uint32_t(__popcll(m));
-        __syncthreads();
-        if (valid)
+            wc[wave][d] = before + uint32_t(__popcll(m));
+    }
+    __syncthreads();
+    // thread t = digit t: the waves' offsets within the digit, the digit's count and its start in the tile
+    uint32_t cnt = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w)
+    {
+        uint32_t const c = wc[w][threadIdx.x];
+        wc[w][threadIdx.x] = cnt;
+        cnt += c;
+    }
+    uint32_t total = 0;
+    uint32_t const start = block_scan_excl(cnt, ws, total);
+    lstart[threadIdx.x] = start;
+    gdelta[threadIdx.x] = offs[int64_t(threadIdx.x) * nblocks + blockIdx.x] + rowbase[threadIdx.x] - start;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kItems; ++r)
+        if (base + int64_t(r) * 64 + lane < n)
         {
-            uint32_t pos = run[d] + uint32_t(__popcll(m & lt));
-            for (int w = 0; w < wave; ++w)
-                pos += wcnt[w][d];
-            if (pos < uint32_t(n)) // always true when the histogram matches; guards memory regardless
-                kout[pos] = key;
+            uint32_t const d = uint32_t(key[r] >> shift) & 255u;
+            tile[lstart[d] + wc[wave][d] + rank[r]] = key[r];
         }
-        __syncthreads();
-        uint32_t tot = 0;
-        for (int w = 0; w < kWaves; ++w)
-            tot += wcnt[w][threadIdx.x];
-        run[threadIdx.x] += tot;
-        __syncthreads();
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j < total; j += kB)
+    {
+        uint64_t const k = tile[j];
+        uint32_t const pos = gdelta[uint32_t(k >> shift) & 255u] + j;
+        if (pos < uint32_t(n)) // always true when the histogram matches; guards memory regardless
+            kout[pos] = k;
     }
 }
 
