@@ -135,7 +135,28 @@ struct grid_desc
 {
     float start[3], dx[3], dy[3], dz[3];
     int64_t nx, ny, nz;
+    double inx, iny; // 1 / nx, 1 / ny (the lattice index is split with them; nx * ny * nz <= 2^52)
 };
+
+// q = a / d, r = a % d for 0 <= a < 2^52 and 0 < d < 2^31, with inv = 1 / d rounded to double: a * inv is within a
+// relative 2^-52 of a / d, so its truncation is the quotient or one off it, and one correction step makes it exact.
+// (The 64-bit integer division it replaces is a long emulated sequence on this GPU.)
+__device__ __forceinline__ int64_t div_recip(int64_t a, int64_t d, double inv, int64_t& r)
+{
+    int64_t q = int64_t(double(a) * inv);
+    r = a - q * d;
+    if (r < 0)
+    {
+        --q;
+        r += d;
+    }
+    else if (r >= d)
+    {
+        ++q;
+        r -= d;
+    }
+    return q;
+}
 struct src_grid
 {
     grid_desc g;
@@ -152,10 +173,9 @@ __device__ __forceinline__ void query_at(src_array<T> const& s, int64_t i, T& x,
 __device__ __forceinline__ void query_at(src_grid const& s, int64_t i, float& x, float& y, float& z)
 {
 #pragma clang fp contract(off)
-    int64_t const ix = i % s.g.nx;
-    int64_t const t = i / s.g.nx;
-    int64_t const iy = t % s.g.ny;
-    int64_t const iz = t / s.g.ny;
+    int64_t ix, iy;
+    int64_t const t = div_recip(i, s.g.nx, s.g.inx, ix);
+    int64_t const iz = div_recip(t, s.g.ny, s.g.iny, iy);
     float const fx = float(ix), fy = float(iy), fz = float(iz);
     x = ((s.g.start[0] + s.g.dx[0] * fx) + s.g.dy[0] * fy) + s.g.dz[0] * fz;
     y = ((s.g.start[1] + s.g.dx[1] * fx) + s.g.dy[1] * fy) + s.g.dz[1] * fz;

@@ -1133,9 +1133,12 @@ int grid_setup(float const* start, float const* dx, float const* dy, float const
         c.gd.dy[k] = dy[k];
         c.gd.dz[k] = dz[k];
     }
+    AGWN_CHECK(int64_t(nx) * ny <= (int64_t(1) << 52) / nz, "grid of %d x %d x %d cells is too large", nx, ny, nz);
     c.gd.nx = nx;
     c.gd.ny = ny;
     c.gd.nz = nz;
+    c.gd.inx = 1.0 / double(nx);
+    c.gd.iny = 1.0 / double(ny);
     c.n = int64_t(nx) * ny * nz;
     // integration / pole axis: fewest cells, ties to the lower axis (gwn_mesh_opengl.hh:524-531)
     float const* dirs[3] = {dx, dy, dz};
