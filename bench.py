@@ -211,6 +211,88 @@ def bench_batched(args, agwn, mg, torch) -> int:
     return 0
 
 
+def bench_plain256(args, agwn, mg, torch) -> int:
+    """The plain method at scale: the 1M-triangle holed torus (its 31,376-segment boundary) and the 256^3 lattice over
+    its 10 %-expanded box, 16,777,216 queries; the full GWN in FP32 (the grid evaluator, and the array entry on the
+    same float32 lattice points), in FP64 (array entry) and eval_gwnr_mesh_batch_robust with robust::int64_grid on the
+    FP32 points. Each method: a warm-up call on the first 65,536 queries (per-direction preparation), then one timed
+    call on all of them. The labels (GWN > 0.5) of every method are compared with the FP64 ones on every query."""
+    t = time.perf_counter()
+    verts, idx = mg.holed_torus_mesh()
+    v32 = verts.astype(np.float32)
+    seg32 = agwn.build_boundary_segments(v32, idx)
+    seg64 = agwn.build_boundary_segments(verts, idx)
+    lo, hi = mg.expanded_bbox(verts)
+    n = 256
+    nq, nseg = n ** 3, len(seg32)
+    start, dx, dy, dz = mg.grid_axes(lo, hi, n)
+    p32 = torch.from_numpy(mg.grid_points(lo, hi, n, np.float32)).cuda()
+    p64 = torch.from_numpy(mg.grid_points(lo, hi, n)).cuda()
+    s32 = torch.from_numpy(np.ascontiguousarray(seg32, dtype=np.float32)).cuda()
+    s64 = torch.from_numpy(np.ascontiguousarray(seg64, dtype=np.float64)).cuda()
+    x0 = np.array([1.0, 0.0, 0.0])
+    log(f"plain256 inputs: {len(idx) // 3} triangles, {nseg} segments, {nq} queries ({time.perf_counter() - t:.1f} s)")
+    grid = agwn.HipGridGwn().set_scene(v32, idx, seg32)
+    m32 = agwn.HipMesh(v32, idx, dtype=np.float32)
+    m64 = agwn.HipMesh(verts, idx, dtype=np.float64)
+    rob = agwn.RobustMeshGwn(v32, idx, seg32, precision="int64_grid", dtype=np.float32)
+    w = 65536
+    outs = {"fp32_grid": torch.empty(nq, dtype=torch.float32, device="cuda"),
+            "fp32_array": torch.empty(nq, dtype=torch.float32, device="cuda"),
+            "fp64_array": torch.empty(nq, dtype=torch.float64, device="cuda"),
+            "robust_int64_grid": torch.empty(nq, dtype=torch.float32, device="cuda")}
+    # the array methods run in 8 slices of 2^21 queries, each timed and logged (the slowest takes seconds per slice)
+    sl = [(k * nq // 8, (k + 1) * nq // 8) for k in range(8)]
+    runs = {
+        "fp32_grid": (lambda: grid.eval_grid(start, dx, dy, dz, n, n, 4, out=outs["fp32_grid"][: 4 * n * n]),
+                      [lambda: grid.eval_grid(start, dx, dy, dz, n, n, n, out=outs["fp32_grid"])]),
+        "fp32_array": (lambda: m32.full(p32[:w], x0, s32, out=outs["fp32_array"][:w]),
+                       [lambda a=a, b=b: m32.full(p32[a:b], x0, s32, out=outs["fp32_array"][a:b]) for a, b in sl]),
+        "fp64_array": (lambda: m64.full(p64[:w], x0, s64, out=outs["fp64_array"][:w]),
+                       [lambda a=a, b=b: m64.full(p64[a:b], x0, s64, out=outs["fp64_array"][a:b]) for a, b in sl]),
+        "robust_int64_grid": (lambda: rob.eval(p32[:w], out=outs["robust_int64_grid"][:w]),
+                              [lambda a=a, b=b: rob.eval(p32[a:b], out=outs["robust_int64_grid"][a:b])
+                               for a, b in sl]),
+    }
+    res = {}
+    for name, (warm, parts) in runs.items():
+        warm()
+        torch.cuda.synchronize()
+        ms = 0.0
+        for k, part in enumerate(parts):
+            t0 = time.perf_counter()
+            part()
+            torch.cuda.synchronize()
+            ms += (time.perf_counter() - t0) * 1e3
+            if len(parts) > 1:
+                log(f"{name}: slice {k + 1}/{len(parts)} done, {ms:.1f} ms so far")
+        res[name] = {"ms": ms, "queries_per_s": nq / (ms * 1e-3), "pairs_per_s": nq * float(nseg) / (ms * 1e-3)}
+        log(f"{name}: {ms:.1f} ms = {nq / ms * 1e3 / 1e6:.2f} M queries/s")
+    ref = outs["fp64_array"] > 0.5
+    for name in ("fp32_grid", "fp32_array", "robust_int64_grid"):
+        diff = ((outs[name] > 0.5) != ref).nonzero().flatten()
+        res[name]["labels_differing_from_fp64"] = int(diff.numel())
+        first = diff[:16].tolist()  # which queries, with both values (for a look with the CPU reference)
+        res[name]["first_differing"] = [[i, float(outs[name][i]), float(outs["fp64_array"][i])] for i in first]
+    d = (outs["fp32_array"].double() - outs["fp64_array"]).abs()
+    res["fp32_array"]["max_abs_diff_from_fp64"] = float(d.max())
+    log(f"labels differing from FP64: " + ", ".join(f"{k} {v['labels_differing_from_fp64']}" for k, v in res.items()
+                                                    if "labels_differing_from_fp64" in v))
+    line = {
+        "metric": "GWN queries/s, 256^3 grid over the 1M-tri holed torus (plain FP32, FP64, robust int64_grid)",
+        "value": res["fp32_grid"]["queries_per_s"], "unit": "queries/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": res["fp32_grid"]["ms"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (seeded holed torus + 256^3 lattice)",
+        "config": {"workload": f"256^3 lattice over the {len(idx) // 3}-triangle holed torus ({nseg} boundary "
+                               f"segments), full GWN", "parallelism": "single GPU"},
+        "methods": res,
+    }
+    for h in (grid, m32, m64, rob):
+        h.close()
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -234,9 +316,10 @@ def main() -> int:
                          "else train and save there (for profiler runs that should not retrain)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["surrogate", "batched"], default="surrogate",
+    ap.add_argument("--workload", choices=["surrogate", "batched", "plain256"], default="surrogate",
                     help="surrogate: the metric's 512^3 surrogate+fallback vs plain run (default); batched: the "
-                         "uneven many-mesh configuration in persistent launches")
+                         "uneven many-mesh configuration in persistent launches; plain256: the plain FP32, FP64 and "
+                         "robust int64_grid evaluations of the 256^3 lattice over the same torus")
     ap.add_argument("--batch-meshes", type=int, default=4096)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy a 1 GiB device buffer 3 times (the known-size reference for rocprofv3 "
@@ -262,6 +345,8 @@ def main() -> int:
     stream = torch.cuda.current_stream()
     if args.workload == "batched":
         return bench_batched(args, agwn, mg, torch)
+    if args.workload == "plain256":
+        return bench_plain256(args, agwn, mg, torch)
 
     t = time.perf_counter()
     verts, idx = mg.holed_torus_mesh()
