@@ -142,6 +142,20 @@ __device__ __forceinline__ double pair_f64(double v0x, double v0y, double v0z, d
     return atan2(num, den);
 }
 
+// The same pair's (num, den) in FP64, for callers that combine two pairs' angles.
+__device__ __forceinline__ void pair_nd_f64(double v0x, double v0y, double v0z, double v1x, double v1y, double v1z,
+                                            pole const& P, double& num, double& den)
+{
+    double const yz0 = v0y * v0y + v0z * v0z;
+    double const yz1 = v1y * v1y + v1z * v1z;
+    double const l0 = sqrt(v0x * v0x + yz0);
+    double const l1 = sqrt(v1x * v1x + yz1);
+    double const lp0 = (v0x >= 0.0 ? l0 + v0x : yz0 / (l0 - v0x)) + P.sm1 * v0x;
+    double const lp1 = (v1x >= 0.0 ? l1 + v1x : yz1 / (l1 - v1x)) + P.sm1 * v1x;
+    num = P.s * (v0y * v1z - v0z * v1y);
+    den = lp0 * lp1 + (P.c * v0x * v1x + (v0y * v1y + v0z * v1z));
+}
+
 // atan2 for the FP32 kernel: octant reduction to r = min(|y|,|x|) / max(|y|,|x|) in [0, 1] and a degree-15 odd
 // minimax polynomial atan(r) ~ r P(r^2) fitted here (Lawson's iteratively reweighted least squares on [0, 1]),
 // |error| <= 1.1e-7 evaluated in FP32. Quadrant fix-ups follow C's atan2 (signed zeros included); no NaN/inf inputs
@@ -751,15 +765,37 @@ __global__ void __launch_bounds__(kBlock)
     double const py = R.r[3] * x + R.r[4] * y + R.r[5] * z;
     double const pz = R.r[6] * x + R.r[7] * y + R.r[8] * z;
     double sum = 0.0, comp = 0.0;
-#pragma unroll 2
-    for (int64_t s = 0; s < nseg; ++s)
-    {
-        seg_f64 const g = seg[s];
-        double const a = pair_f64(g.ax - px, g.ay - py, g.az - pz, g.bx - px, g.by - py, g.bz - pz, P);
+    auto add = [&](double v) {
         double t, e;
-        two_sum(sum, g.w * a, t, e);
+        two_sum(sum, v, t, e);
         sum = t;
         comp += e;
+    };
+    // Segments two at a time: with equal weights and both denominators positive, each angle lies in (-pi/2, pi/2), so
+    // their sum is the argument of the product (den0 + i num0)(den1 + i num1) -- one atan2 for two pairs. The product
+    // is formed only when both (num, den) magnitudes lie in [2^-400, 2^400] (no overflow or underflow); otherwise, and
+    // for unequal weights or a non-positive denominator, each pair takes its own atan2.
+    int64_t s = 0;
+    for (; s + 1 < nseg; s += 2)
+    {
+        seg_f64 const g0 = seg[s], g1 = seg[s + 1];
+        double n0, d0, n1, d1;
+        pair_nd_f64(g0.ax - px, g0.ay - py, g0.az - pz, g0.bx - px, g0.by - py, g0.bz - pz, P, n0, d0);
+        pair_nd_f64(g1.ax - px, g1.ay - py, g1.az - pz, g1.bx - px, g1.by - py, g1.bz - pz, P, n1, d1);
+        double const m0 = fmax(fabs(n0), d0), m1 = fmax(fabs(n1), d1);
+        constexpr double lo = 0x1p-400, hi = 0x1p400;
+        if (g0.w == g1.w && d0 > 0.0 && d1 > 0.0 && m0 >= lo && m0 <= hi && m1 >= lo && m1 <= hi)
+            add(g0.w * atan2(fma(n0, d1, n1 * d0), fma(d0, d1, -(n0 * n1))));
+        else
+        {
+            add(g0.w * atan2(n0, d0));
+            add(g1.w * atan2(n1, d1));
+        }
+    }
+    if (s < nseg)
+    {
+        seg_f64 const g = seg[s];
+        add(g.w * pair_f64(g.ax - px, g.ay - py, g.az - pz, g.bx - px, g.by - py, g.bz - pz, P));
     }
     if (i < n)
     {
