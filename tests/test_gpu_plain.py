@@ -202,3 +202,23 @@ def test_empty_and_degenerate_inputs(gpu):
     with gpu.HipMesh(tv, ti) as mesh:
         got = gpu.eval_gwnr_mesh_batch_integer(mesh, p, np.array([-1.0, 0.0, 0.0]))
     assert got.tolist() == [-1, 0, 0]
+
+
+@pytest.mark.parametrize("s,dt,tol", [("f64", np.float64, 1e-12), ("f32", np.float32, 5e-5)])
+def test_general_weights_fractional(gpu, oracle, s, dt, tol):
+    """weighted_segment3 weights are any real numbers in the reference (gwn_mesh.hh:34-46 sums w times the pair
+    term): the C1 boundary with seeded weights -- runs of equal weights (paired atan2 in the FP64 kernel), unequal
+    neighbours, negative and non-integer values (the per-segment FP32 kernel) -- against the extended-precision
+    evaluation of the same inputs. FP32 bar 5e-5: the weights reach 3 in magnitude."""
+    g = _c1()
+    seg = np.array(g["seg_f64"], dtype=np.float64)
+    rng = np.random.default_rng(17)
+    choices = np.array([1.0, 1.0, 1.0, -0.5, 2.75, 0.3, -3.0])
+    seg[:, 6] *= choices[rng.integers(0, len(choices), len(seg))]
+    pos = mg.uniform_points(31, 4096, [-1.5] * 3, [1.5] * 3)
+    x0 = mg.example_x0(np.float64)
+    print(f"case general weights {s}", flush=True)
+    frac = gpu.eval_gwnr_mesh_batch_fractional(pos.astype(dt), x0.astype(dt), seg.astype(dt))
+    ref = oracle.frac_ext(pos.astype(dt).astype(np.float64), seg.astype(dt).astype(np.float64),
+                          x0.astype(dt).astype(np.float64))
+    assert np.max(np.abs(frac.astype(np.float64) - ref)) < tol
