@@ -1161,6 +1161,76 @@ int frac_plan_build(float const* h, int64_t nseg, frac_plan& plan, hipStream_t s
     return ok;
 }
 
+// flag = 1 when the word arrays differ anywhere
+__global__ void __launch_bounds__(kBlock) k_words_differ(uint32_t const* __restrict__ a, uint32_t const* __restrict__ b,
+                                                         int64_t n, int* __restrict__ flag)
+{
+    int64_t const i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (i < n && a[i] != b[i])
+        *flag = 1;
+}
+
+int cached_plan(plan_cache& c, float const* seg7, int64_t nseg, hipStream_t s, std::shared_ptr<frac_plan const>& out)
+{
+    AGWN_CHECK(nseg >= 0 && (nseg == 0 || seg7 != nullptr), "boundary is null with nseg=%lld", (long long)nseg);
+    int dev = 0;
+    AGWN_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(c.mu);
+    if (c.plan && c.nseg == nseg && c.dev == dev)
+    {
+        int differ = 0;
+        if (nseg > 0)
+        {
+            int64_t const words = nseg * 7;
+            AGWN_CHECK((words + kBlock - 1) / kBlock < (int64_t(1) << 31), "boundary too large (%lld segments)",
+                       (long long)nseg);
+            dbuf flag;
+            AGWN_TRY(flag.alloc(sizeof(int), s));
+            AGWN_HIP(hipMemsetAsync(flag.p, 0, sizeof(int), s));
+            k_words_differ<<<dim3(unsigned((words + kBlock - 1) / kBlock)), dim3(kBlock), 0, s>>>(
+                reinterpret_cast<uint32_t const*>(c.seg), reinterpret_cast<uint32_t const*>(seg7), words,
+                flag.as<int>());
+            AGWN_HIP(hipGetLastError());
+            AGWN_HIP(hipMemcpyAsync(&differ, flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
+            AGWN_HIP(hipStreamSynchronize(s));
+        }
+        if (!differ)
+        {
+            out = c.plan;
+            return ok;
+        }
+    }
+    std::vector<float> h(size_t(nseg) * 7);
+    if (nseg > 0)
+    {
+        AGWN_HIP(hipMemcpyAsync(h.data(), seg7, h.size() * sizeof(float), hipMemcpyDeviceToHost, s));
+        AGWN_HIP(hipStreamSynchronize(s));
+    }
+    auto p = std::make_shared<frac_plan>();
+    // on the null stream: the plan's buffers are freed on the stream they were allocated on, and the caller's stream
+    // may be gone by then
+    AGWN_TRY(frac_plan_build(h.data(), nseg, *p, nullptr));
+    float* copy = nullptr;
+    if (nseg > 0)
+    {
+        AGWN_HIP(hipMalloc(&copy, h.size() * sizeof(float)));
+        hipError_t const e = hipMemcpy(copy, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice);
+        if (e != hipSuccess)
+        {
+            (void)hipFree(copy);
+            AGWN_HIP(e);
+        }
+    }
+    if (c.seg)
+        (void)hipFree(c.seg);
+    c.seg = copy;
+    c.nseg = nseg;
+    c.dev = dev;
+    c.plan = p;
+    out = p;
+    return ok;
+}
+
 // the decision margin covers the FP32 evaluation's own error (measured <= 4e-6 on 430k-segment boundaries)
 constexpr double kDecideMargin = 1e-4;
 

@@ -225,6 +225,27 @@ struct frac_plan
 // Builds the plan from a host copy of the boundary (stream-ordered upload on `stream`; returns when it is done).
 int frac_plan_build(float const* seg7_host, int64_t nseg, frac_plan& plan, hipStream_t stream);
 
+// A boundary's plan kept for later calls: reused when the device copy of the boundary it was built from equals the
+// call's boundary word for word (compared on the device), rebuilt otherwise. Calls are serialised by mu; a plan handed
+// out stays alive with its holders when a later call replaces it.
+struct plan_cache
+{
+    std::mutex mu;
+    std::shared_ptr<frac_plan> plan;
+    float* seg = nullptr; // device copy of the boundary the plan was built from
+    int64_t nseg = -1;
+    int dev = -1;
+    plan_cache() = default;
+    plan_cache(plan_cache const&) = delete;
+    plan_cache& operator=(plan_cache const&) = delete;
+    ~plan_cache()
+    {
+        if (seg)
+            (void)hipFree(seg);
+    }
+};
+int cached_plan(plan_cache& c, float const* seg7, int64_t nseg, hipStream_t s, std::shared_ptr<frac_plan const>& out);
+
 // Inside/outside decision without the full boundary sum, for m query points (device, FP32 xyz) with their integer
 // terms `counts` (needs plan.chain && plan.loops_ok): per query, loops whose bound is small are replaced by a
 // second-order far-field expansion with a rigorous remainder bound, the others are summed exactly (FP32 chain

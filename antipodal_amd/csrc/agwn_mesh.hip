@@ -825,6 +825,7 @@ struct mesh_t final : mesh_impl
 struct agwn_mesh
 {
     std::unique_ptr<agwn::mesh_impl> impl;
+    mutable agwn::plan_cache boundary_plan; // FP32: the last boundary's chain plan (the boundary comes per call)
 };
 
 namespace agwn
@@ -919,7 +920,11 @@ int full_dev(agwn_mesh const* m, T const* pos, int64_t n, T const* x0, T const* 
     AGWN_TRY(mt->count(pos, n, d, cnt.as<int32_t>(), s));
     double const x[3] = {double(x0[0]), double(x0[1]), double(x0[2])};
     if constexpr (sizeof(T) == 4)
-        return frac_f32(pos, n, seg7, nseg, x, cnt.as<int32_t>(), out, s);
+    {
+        std::shared_ptr<frac_plan const> pl;
+        AGWN_TRY(cached_plan(m->boundary_plan, seg7, nseg, s, pl));
+        return frac_f32_planned(src_array<float>{pos}, n, *pl, seg7, nseg, x, cnt.as<int32_t>(), out, s);
+    }
     else
         return frac_f64(pos, n, seg7, nseg, x, cnt.as<int32_t>(), out, s);
 }
@@ -1471,7 +1476,10 @@ int agwn_label_surrogate_dev(agwn_mesh const* m, agwn_mlp const* mlp, float cons
         double qmax = 0.0;
         if (n > 0)
             AGWN_TRY(mesh_t<float>::max_query_norm(pos, n, st, qmax));
-        return label_surrogate(mesh, mlp, src_array<float>{pos}, n, qmax, x0, seg7, nseg, nullptr, delta,
+        std::shared_ptr<frac_plan const> pl;
+        if (n > 0)
+            AGWN_TRY(cached_plan(m->boundary_plan, seg7, nseg, st, pl));
+        return label_surrogate(mesh, mlp, src_array<float>{pos}, n, qmax, x0, seg7, nseg, pl.get(), delta,
                                audit_every, audit_seed, labels, stats, st);
     });
 }

@@ -222,3 +222,42 @@ def test_general_weights_fractional(gpu, oracle, s, dt, tol):
     ref = oracle.frac_ext(pos.astype(dt).astype(np.float64), seg.astype(dt).astype(np.float64),
                           x0.astype(dt).astype(np.float64))
     assert np.max(np.abs(frac.astype(np.float64) - ref)) < tol
+
+
+def test_mesh_keeps_boundary_plan_only_for_the_same_boundary(gpu):
+    """An FP32 mesh handle keeps the chain plan of the last boundary it was called with and reuses it only when the
+    call's boundary is the same word for word: calls with the C1 boundary A, a shorter boundary B, A's device array
+    changed in place (same pointer and size, every segment reversed) and A again each give exactly what a fresh
+    handle gives, and the reversed boundary negates the fractional term."""
+    import torch
+
+    g = _c1()
+    verts, idx = g["verts"].astype(np.float32), g["idx"]
+    seg = torch.from_numpy(np.array(g["seg_f64"], dtype=np.float32)).cuda()
+    pos = torch.from_numpy(mg.uniform_points(5, 20000, [-1.5] * 3, [1.5] * 3).astype(np.float32)).cuda()
+    x0 = mg.example_x0(np.float32)
+
+    def fresh(b):
+        with gpu.HipMesh(verts, idx, dtype=np.float32) as m:
+            return m.full(pos, x0, b, out=torch.empty(len(pos), dtype=torch.float32, device="cuda")).clone()
+
+    with gpu.HipMesh(verts, idx, dtype=np.float32) as mesh:
+        def kept(b):
+            return mesh.full(pos, x0, b, out=torch.empty(len(pos), dtype=torch.float32, device="cuda")).clone()
+
+        a0 = kept(seg)
+        assert torch.equal(a0, fresh(seg))
+        short = seg[:-5].clone()
+        assert torch.equal(kept(short), fresh(short))
+        frac_a = a0 - gpu.eval_gwnr_mesh_batch_integer(mesh, pos, x0,
+                                                       out=torch.empty(len(pos), dtype=torch.int32,
+                                                                       device="cuda")).float()
+        seg[:, [0, 1, 2, 3, 4, 5]] = seg[:, [3, 4, 5, 0, 1, 2]].clone()  # in place: every segment reversed
+        r = kept(seg)
+        assert torch.equal(r, fresh(seg))
+        frac_r = r - gpu.eval_gwnr_mesh_batch_integer(mesh, pos, x0,
+                                                      out=torch.empty(len(pos), dtype=torch.int32,
+                                                                      device="cuda")).float()
+        assert float((frac_r + frac_a).abs().max()) < 1e-5
+        seg[:, [0, 1, 2, 3, 4, 5]] = seg[:, [3, 4, 5, 0, 1, 2]].clone()  # back to A
+        assert torch.equal(kept(seg), a0)
