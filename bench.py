@@ -472,8 +472,8 @@ def main() -> int:
         raise SystemExit(f"full != fractional + integer (max diff {chk})")
     # the step's exact stage on exactly the routed points: integer term, far-field decision, full boundary sum for the
     # points it leaves open (the point-labelling entry with delta = +inf routes every point; it also runs inference
-    # on them, 5 % of the grid's, builds the boundary's chain plan on the host, which the grid entry keeps per scene,
-    # and its host synchronisations are inside the interval: an upper bound of the step's exact stage)
+    # on them, 5 % of the grid's, checks the boundary against the one its plan was built from, and its host
+    # synchronisations are inside the interval: an upper bound of the step's exact stage)
     lab_r = torch.empty(len(rp_t), dtype=torch.int8, device=dev)
     net.label(mesh, pts_r, x0, seg_t, float("inf"), labels=lab_r, stream=stream)
     ev0.record(stream)
@@ -485,7 +485,7 @@ def main() -> int:
         raise SystemExit("exact-stage labels of the routed points differ from the plain run's")
     log(f"kernels per step: inference {infer_ms:.2f} ms; on the {len(rp)} routed points: exact stage at most {stage_ms:.2f} ms "
         f"(integer terms by packet traversal in index order, decision, full sums; timed through the point-labelling "
-        f"entry, which also runs inference on them and builds the boundary plan per call); on the same points the public "
+        f"entry, which also runs inference on them); on the same points the public "
         f"integer entry (rays sorted by their projected origin, then packets) takes {int_ms:.2f} ms and the full "
         f"boundary sum {frac_ms:.2f} ms")
     if stage_ms > infer_ms:
