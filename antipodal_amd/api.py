@@ -201,7 +201,7 @@ PRECISIONS = {"int64_grid": 0, "int128_grid": 1, "exact_float": 2}
 
 
 def default_precision(dtype) -> str:
-    """robust_predicates.hh:322-323: int64_grid for float, exact_float for double."""
+    """robust_predicates.hh:220-221 (default_precision): int64_grid for float, exact_float for double."""
     return "int64_grid" if np.dtype(dtype) == np.float32 else "exact_float"
 
 

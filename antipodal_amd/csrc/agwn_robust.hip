@@ -1,6 +1,6 @@
 // Robust GWN on gfx950: RobustMeshGwn<T, Precision> (gwn_mesh_robust.hh:170-456) as a prepared device object.
 //
-// Precision policies (robust_predicates.hh:284-317), template parameter C of every kernel:
+// Precision policies (robust_predicates.hh:186-215), template parameter C of every kernel:
 //   int32_t  int64_grid   20-bit quantized coordinates, 64-bit determinants (exact)
 //   int64_t  int128_grid  50-bit quantized coordinates, 128-bit determinants (exact; __int128 lowered to 64-bit ops)
 //   double   exact_float  the doubles themselves; determinant signs exact through a floating-point filter and, when
@@ -141,7 +141,7 @@ __device__ inline int exact_sign6(double const (&a)[6], double const (&b)[6])
     return 0;
 }
 
-// robust_predicates.hh:374-410 -- edge_det: 2x2 determinant of (q - q0, q1 - q0) in (y, z)
+// robust_predicates.hh:272-308 -- edge_det: 2x2 determinant of (q - q0, q1 - q0) in (y, z)
 template <class C>
 __device__ inline wide<C> edge_det(v3<C> q, v3<C> q0, v3<C> q1)
 {
@@ -171,7 +171,7 @@ __device__ inline wide<C> edge_det(v3<C> q, v3<C> q0, v3<C> q1)
     }
 }
 
-// robust_predicates.hh:412-433 -- edge_sign with the (eps2 on y) >> (eps3 on z) perturbation
+// robust_predicates.hh:314-331 -- edge_sign with the (eps2 on y) >> (eps3 on z) perturbation
 template <class C>
 __device__ inline int edge_sign(v3<C> q, v3<C> q0, v3<C> q1)
 {
@@ -185,7 +185,7 @@ __device__ inline int edge_sign(v3<C> q, v3<C> q0, v3<C> q1)
     return 0;
 }
 
-// robust_predicates.hh:441-458 -- atan2 numerator with exact sign (== -edge_det)
+// robust_predicates.hh:339-356 -- atan2 numerator with exact sign (== -edge_det)
 template <class C>
 __device__ inline wide<C> atan2_num(v3<C> q, v3<C> q0, v3<C> q1)
 {
@@ -206,7 +206,7 @@ __device__ inline wide<C> atan2_num(v3<C> q, v3<C> q0, v3<C> q1)
     }
 }
 
-// robust_predicates.hh:470-477 -- dot_yz
+// robust_predicates.hh:368-375 -- dot_yz
 template <class C>
 __device__ inline double dot_yz(v3<C> a, v3<C> b)
 {
@@ -320,7 +320,7 @@ __device__ inline int leaf_count(v3<C> qi, v3<C> const* tri)
 {
 #pragma clang fp contract(off)
     v3<C> const a = tri[0], b = tri[1], c = tri[2];
-    // robust_predicates.hh:492-504 (project_into_tri)
+    // robust_predicates.hh:390-402 (project_into_tri)
     int const e01 = edge_sign(qi, a, b);
     int const e12 = edge_sign(qi, b, c);
     int const e20 = edge_sign(qi, c, a);

@@ -38,7 +38,7 @@ extern "C" {
 #define AGWN_ERR_UNSUPPORTED 5 /* argument combination not supported by this build */
 
 /* Precision policies of the robust evaluator: robust::int64_grid / int128_grid / exact_float
- * (math/robust_predicates.hh:284-317). */
+ * (math/robust_predicates.hh:186-215). */
 #define AGWN_INT64_GRID 0
 #define AGWN_INT128_GRID 1
 #define AGWN_EXACT_FLOAT 2
@@ -103,7 +103,7 @@ int agwn_full_batch_f64_dev(const agwn_mesh* mesh, const double* pos, int64_t n,
  * Robust evaluator. Replaces RobustMeshGwn<T, Precision> (gwn_mesh_robust.hh:170-424) and
  * eval_gwnr_mesh_batch_robust (gwn_mesh_robust.hh:441-456). scalar_bytes = sizeof(T) (4 or 8); precision one of
  * AGWN_INT64_GRID / AGWN_INT128_GRID / AGWN_EXACT_FLOAT (the reference's defaults: int64_grid for float, exact_float
- * for double, robust_predicates.hh:322-323). verts/seg7 are T arrays (host); the constructor's quantization frame,
+ * for double, robust_predicates.hh:220-221). verts/seg7 are T arrays (host); the constructor's quantization frame,
  * swapped quantized boundary and exact predicates are the reference's (gwn_mesh_robust.hh:177-209, 334-378). The
  * ray is baked in (-x, gwn_mesh_robust.hh:224-226); the handle owns its device copies (gwn_mesh_robust.hh:152-156).
  * --------------------------------------------------------------------------------------------------------------- */

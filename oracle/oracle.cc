@@ -219,7 +219,7 @@ static int64_t boundary(T const* verts, int32_t const* idx, int64_t ntri, T* out
 }
 
 // ---------------------------------------------------------------------------------------------------------------------
-// Robust predicates -- math/robust_predicates.hh:264-411.
+// Robust predicates -- math/robust_predicates.hh:225-411.
 // Coordinates C in {int32 (int64_grid), int64 (int128_grid), double (exact_float)}; W = the wide type of edge_det.
 // ---------------------------------------------------------------------------------------------------------------------
 using i128 = __int128;
@@ -314,7 +314,7 @@ static int exact_sign_sum_products(int n, double const* a, double const* b, int 
     return 0;
 }
 
-// robust_predicates.hh:374-410 (edge_det): the 2x2 determinant of (q - q0, q1 - q0) in the (y,z) plane.
+// robust_predicates.hh:272-308 (edge_det): the 2x2 determinant of (q - q0, q1 - q0) in the (y,z) plane.
 // Integers: exact. Doubles: exact sign; magnitude = the rounded double estimate (p1 + p2), raised to DBL_MIN when the
 // exact value is nonzero and the estimate underflows or carries the wrong sign (the reference's filter returns the
 // estimate itself exactly when its sign is certain, and max(|estimate|, DBL_MIN) with the exact sign otherwise).
@@ -341,7 +341,7 @@ static inline W<C> edge_det(V3<C> q, V3<C> q0, V3<C> q1)
     }
 }
 
-// robust_predicates.hh:412-433 (edge_sign): sign of edge_det, tie broken by the perturbation eps2 (y) >> eps3 (z).
+// robust_predicates.hh:314-331 (edge_sign): sign of edge_det, tie broken by the perturbation eps2 (y) >> eps3 (z).
 template <class C>
 static inline int edge_sign(V3<C> q, V3<C> q0, V3<C> q1)
 {
@@ -355,7 +355,7 @@ static inline int edge_sign(V3<C> q, V3<C> q0, V3<C> q1)
     return 0;
 }
 
-// robust_predicates.hh:441-458 (atan2_num): num = v0.z*v1.y - v0.y*v1.z with v = q_i - q; exact sign (= -edge_det).
+// robust_predicates.hh:339-356 (atan2_num): num = v0.z*v1.y - v0.y*v1.z with v = q_i - q; exact sign (= -edge_det).
 template <class C>
 static inline W<C> atan2_num(V3<C> q, V3<C> q0, V3<C> q1)
 {
@@ -374,7 +374,7 @@ static inline W<C> atan2_num(V3<C> q, V3<C> q0, V3<C> q1)
     }
 }
 
-// robust_predicates.hh:470-477 (dot_yz)
+// robust_predicates.hh:368-375 (dot_yz)
 template <class C>
 static inline double dot_yz(V3<C> a, V3<C> b)
 {
@@ -384,7 +384,7 @@ static inline double dot_yz(V3<C> a, V3<C> b)
         return double(wmul<C>(a.y, b.y) + wmul<C>(a.z, b.z));
 }
 
-// robust_predicates.hh:484-504 (project_into_tri)
+// robust_predicates.hh:382-402 (tri_projection, project_into_tri)
 template <class C>
 static inline int project_into_tri(V3<C> q, V3<C> q0, V3<C> q1, V3<C> q2, double* w)
 {
