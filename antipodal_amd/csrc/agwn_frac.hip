@@ -1477,27 +1477,35 @@ int frac_f32_src(Src const& pos, int64_t n, float const* seg7, int64_t nseg, dou
 }
 
 template <class Src>
+int frac_f64_ws(Src const& pos, int64_t n, double const* seg7, int64_t nseg, double const x0[3],
+                int32_t const* counts, double* out, seg_f64* ws, hipStream_t stream)
+{
+    if (n == 0)
+        return ok;
+    int64_t const blocks = (n + kBlock - 1) / kBlock;
+    AGWN_CHECK(blocks < (int64_t(1) << 31), "frac_f64: too many queries (%lld)", (long long)n);
+    frame const R = make_frame_impl(x0[0], x0[1], x0[2]);
+    if (nseg > 0)
+    {
+        k_prep_segments<double><<<dim3(unsigned((nseg + kBlock - 1) / kBlock)), dim3(kBlock), 0, stream>>>(
+            seg7, nseg, nseg, R, nullptr, ws);
+        AGWN_HIP(hipGetLastError());
+    }
+    pole const P = make_pole(x0[0], x0[1], x0[2]);
+    k_frac_f64<Src><<<dim3(unsigned(blocks)), dim3(kBlock), 0, stream>>>(pos, n, ws, nseg, R, P, counts, out);
+    AGWN_HIP(hipGetLastError());
+    return ok;
+}
+
+template <class Src>
 int frac_f64_src(Src const& pos, int64_t n, double const* seg7, int64_t nseg, double const x0[3],
                  int32_t const* counts, double* out, hipStream_t stream)
 {
     if (n == 0)
         return ok;
-    frame const R = make_frame_impl(x0[0], x0[1], x0[2]);
     dbuf s64;
     AGWN_TRY(s64.alloc(size_t(std::max<int64_t>(nseg, 1)) * sizeof(seg_f64), stream));
-    if (nseg > 0)
-    {
-        k_prep_segments<double><<<dim3(unsigned((nseg + kBlock - 1) / kBlock)), dim3(kBlock), 0, stream>>>(
-            seg7, nseg, nseg, R, nullptr, s64.as<seg_f64>());
-        AGWN_HIP(hipGetLastError());
-    }
-    int64_t const blocks = (n + kBlock - 1) / kBlock;
-    AGWN_CHECK(blocks < (int64_t(1) << 31), "frac_f64: too many queries (%lld)", (long long)n);
-    pole const P = make_pole(x0[0], x0[1], x0[2]);
-    k_frac_f64<Src><<<dim3(unsigned(blocks)), dim3(kBlock), 0, stream>>>(pos, n, s64.as<seg_f64>(), nseg, R, P,
-                                                                         counts, out);
-    AGWN_HIP(hipGetLastError());
-    return ok;
+    return frac_f64_ws(pos, n, seg7, nseg, x0, counts, out, s64.as<seg_f64>(), stream);
 }
 
 // (queries per lane, block) shapes of the batched kernel offered to the autotuner
@@ -1550,4 +1558,6 @@ template int frac_f32_src<src_grid>(src_grid const&, int64_t, float const*, int6
                                     float*, hipStream_t);
 template int frac_f64_src<src_array<double>>(src_array<double> const&, int64_t, double const*, int64_t,
                                              double const*, int32_t const*, double*, hipStream_t);
+template int frac_f64_ws<src_array<double>>(src_array<double> const&, int64_t, double const*, int64_t, double const*,
+                                            int32_t const*, double*, seg_f64*, hipStream_t);
 } // namespace agwn

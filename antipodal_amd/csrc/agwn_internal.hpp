@@ -268,6 +268,11 @@ int frac_f32_src(Src const& src, int64_t n, float const* seg7, int64_t nseg, dou
 template <class Src>
 int frac_f64_src(Src const& src, int64_t n, double const* seg7, int64_t nseg, double const x0[3],
                  int32_t const* counts, double* out, hipStream_t stream);
+// The FP64 launcher with the caller's workspace: ws holds max(nseg, 1) seg_f64 (device), the boundary rotated into
+// the pole frame of x0. Launches two kernels on `stream`; allocates nothing. Instantiated for src_array<double>.
+template <class Src>
+int frac_f64_ws(Src const& src, int64_t n, double const* seg7, int64_t nseg, double const x0[3],
+                int32_t const* counts, double* out, seg_f64* ws, hipStream_t stream);
 
 // Surrogate routing (agwn_mlp.hip), instantiated for src_array<float> and src_grid:
 //   mlp_route      network labels of n points -> labels; the band / near-surface / audit points are appended to list
