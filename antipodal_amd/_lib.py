@@ -35,6 +35,7 @@ _SIG = {
     "agwn_frac_batch_f64": (_int, [_p, _i64, _p, _i64, _p, _p]),
     "agwn_frac_batch_f32_dev": (_int, [_p, _i64, _p, _i64, _p, _p, _p]),
     "agwn_frac_batch_f64_dev": (_int, [_p, _i64, _p, _i64, _p, _p, _p]),
+    "agwn_frac_batch_f64_dev_noalloc": (_int, [_p, _i64, _p, _i64, _p, _p, _p]),
     "agwn_mesh_create_f32": (_int, [_p, _i64, _p, _i64, C.POINTER(_p)]),
     "agwn_mesh_create_f64": (_int, [_p, _i64, _p, _i64, C.POINTER(_p)]),
     "agwn_mesh_destroy": (_int, [_p]),

@@ -63,6 +63,15 @@ int agwn_frac_batch_f32_dev(const float* pos, int64_t n, const float* seg7, int6
                             agwn_stream stream);
 int agwn_frac_batch_f64_dev(const double* pos, int64_t n, const double* seg7, int64_t nseg, const double* x0,
                             double* out, agwn_stream stream);
+/* Allocation-free variant of agwn_frac_batch_f64_dev (same kernels, same results, bit for bit). It has no counterpart
+ * in the reference: it is for callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). Its one workspace, the boundary rotated into the pole frame (64 B per
+ * segment), lives in a 32 MiB device array that the library holds on every GPU it runs on until the process ends, so
+ * nseg may not exceed 524,288 (larger boundaries are rejected with AGWN_ERR_INVALID). The caller still passes every
+ * input and output. All calls on one GPU share that array: calls that overlap on one GPU (other host threads, or
+ * streams without ordering between them) race and must be serialised by the caller. */
+int agwn_frac_batch_f64_dev_noalloc(const double* pos, int64_t n, const double* seg7, int64_t nseg, const double* x0,
+                                    double* out, agwn_stream stream);
 
 /* ---------------------------------------------------------------------------------------------------------------
  * Prepared triangle mesh: a GPU-built LBVH (Morton codes, LDS radix sort, Karras hierarchy) over the triangles.
