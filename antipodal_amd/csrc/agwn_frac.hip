@@ -158,8 +158,9 @@ __device__ __forceinline__ void pair_nd_f64(double v0x, double v0y, double v0z, 
 
 // atan2 for the FP32 kernel: octant reduction to r = min(|y|,|x|) / max(|y|,|x|) in [0, 1] and a degree-15 odd
 // minimax polynomial atan(r) ~ r P(r^2) fitted here (Lawson's iteratively reweighted least squares on [0, 1]),
-// |error| <= 1.1e-7 evaluated in FP32. Quadrant fix-ups follow C's atan2 (signed zeros included); no NaN/inf inputs
-// reach it (num and den are finite sums of finite products).
+// |error| <= 1.1e-7 evaluated in FP32. Quadrant fix-ups follow C's atan2 (signed zeros included). Callers pass finite
+// arguments whose max(|x|, |y|) is normal or zero (the pair range described below, FP64 values scaled near 1,
+// atan2_f32_any), so the FLT_MIN floor on the divisor only acts at (0, 0), where r = 0 is exact.
 __device__ __forceinline__ float atan2_f32(float y, float x)
 {
     constexpr float c0 = 0.9999993443489075f, c1 = -0.33329862356185913f, c2 = 0.19946569204330444f,
@@ -215,6 +216,43 @@ __device__ __forceinline__ float pair_f64_atan_f32(double v0x, double v0y, doubl
     return atan2_f32(float(ldexp(num, -e)), float(ldexp(den, -e)));
 }
 
+// Range of the FP32 pair arithmetic. S = lp0 lp1 + (|y0| + |z0|)(|y1| + |z1|) bounds every product in a pair's num
+// and den, and a pair that passes the conditioning test has tau S / sqrt(2) <= |(den, num)| <= 2 S (tau = 1/32). The
+// FP32 kernels work in coordinates scaled by the power of two that brings the boundary's extent into [1, 2)
+// (boundary_scale_exp), so S is relative to the boundary's size and no result depends on a power-of-two scaling of
+// all inputs. A query within 2^26 extents of the centre of the boundary's box then has S <= 2^58 for every pair, so
+// the complex product of two pairs stays below 2^118 and finite. A query farther out is "far": its lane runs with an
+// infinite conditioning threshold (kFarTau), which sends every one of its pairs to the FP64 path, whose own scaling
+// covers any finite value. The test costs a few instructions per query, none per pair. (Below, S is small only for a
+// query within ~2^-40 extents of a boundary point, where the fractional term itself varies too fast for FP32
+// coordinates to pin it.)
+constexpr float kFarTau = 0x1p100f;
+__device__ __forceinline__ float lane_tau(float tau, float x, float y, float z, float4 const& far)
+{
+    float const m = __builtin_fmaxf(__builtin_fmaxf(__builtin_fabsf(x - far.x), __builtin_fabsf(y - far.y)),
+                                    __builtin_fabsf(z - far.z));
+    return m > far.w ? kFarTau : tau;
+}
+// atan2_f32 for any finite arguments: below 2^-60, both are scaled by 2^64 first (exact: subnormals become normal)
+__device__ __forceinline__ float atan2_f32_any(float y, float x)
+{
+    if (__builtin_fmaxf(__builtin_fabsf(x), __builtin_fabsf(y)) < 0x1p-60f)
+    {
+        x *= 0x1p64f;
+        y *= 0x1p64f;
+    }
+    return atan2_f32(y, x);
+}
+// the conditioning threshold of the lane that owns query i (kFarTau for a far query)
+template <class Src>
+__device__ __forceinline__ float query_tau(Src const& src, int64_t i, int64_t n, float tau, float4 const& far)
+{
+    float x = 0.0f, y = 0.0f, z = 0.0f;
+    if (i < n)
+        query_at(src, i, x, y, z);
+    return lane_tau(tau, x, y, z, far);
+}
+
 // x unchanged, but opaque to the optimiser: work that depends on it stays where it is written (used to keep the
 // FP64 conversions of the rare ill-conditioned branch from being hoisted onto the common path). Emits no instruction.
 __device__ __forceinline__ float opaque(float x)
@@ -262,17 +300,18 @@ constexpr int kTile = kBlock;      // segments staged in LDS per pass: one 64-by
 template <int Q, class Src>
 __global__ void __launch_bounds__(kBlock)
     k_frac_f32(Src const src, int64_t n, seg_f32 const* __restrict__ seg, seg_f64 const* __restrict__ seg64,
-               int64_t nseg_padded, frame R, pole P, float tau, int32_t const* __restrict__ counts,
+               int64_t nseg_padded, frame R, pole P, float tau, float4 far, int32_t const* __restrict__ counts,
                float* __restrict__ out)
 {
     __shared__ seg_f32 tile[kTile];
     int64_t const base = int64_t(blockIdx.x) * (kBlock * Q) + threadIdx.x;
-    float ph[Q][3], pl[Q][3];
+    float ph[Q][3], pl[Q][3], tq[Q];
     double acc[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q)
     {
         query_split(src, base + int64_t(q) * kBlock, n, R, ph[q], pl[q]);
+        tq[q] = query_tau(src, base + int64_t(q) * kBlock, n, tau, far);
         acc[q] = 0.0;
     }
     float const sf = float(P.s), sm1 = float(P.sm1), cf = float(P.c);
@@ -307,7 +346,7 @@ __global__ void __launch_bounds__(kBlock)
                 float const scale = pp + (__builtin_fabsf(v0y) + __builtin_fabsf(v0z)) *
                                              (__builtin_fabsf(v1y) + __builtin_fabsf(v1z));
                 double a;
-                if (__builtin_expect(__builtin_fabsf(num) + __builtin_fabsf(den) < tau * scale || !(den > 0.0f), 0))
+                if (__builtin_expect(__builtin_fabsf(num) + __builtin_fabsf(den) < tq[q] * scale || !(den > 0.0f), 0))
                 {
                     // ill-conditioned pair: FP64 from the FP64 copies of the segment and the query
                     seg_f64 const h = seg64[t0 + j];
@@ -338,18 +377,27 @@ __global__ void __launch_bounds__(kBlock)
     }
 }
 
-// Rotates the point stream into the pole frame, two float4 per point: (hi x, y, z, flag) and (lo x, y, z, 0).
+// Rotates the point stream into the pole frame, two float4 per point: (hi x, y, z, flag) and (lo x, y, z, 0). With
+// pexp (a batch of meshes), point i is also scaled by 2^-pexp[i] (its mesh's exponent, exact in double); otherwise
+// R already carries the scale.
 __global__ void __launch_bounds__(kBlock) k_prep_stream(float4 const* __restrict__ in, int64_t n, frame R,
-                                                       float4* __restrict__ out)
+                                                       int16_t const* __restrict__ pexp, float4* __restrict__ out)
 {
     int64_t const i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
     if (i >= n)
         return;
     float4 const g = in[i];
     double const x = double(g.x), y = double(g.y), z = double(g.z);
-    double const rx = R.r[0] * x + R.r[1] * y + R.r[2] * z;
-    double const ry = R.r[3] * x + R.r[4] * y + R.r[5] * z;
-    double const rz = R.r[6] * x + R.r[7] * y + R.r[8] * z;
+    double rx = R.r[0] * x + R.r[1] * y + R.r[2] * z;
+    double ry = R.r[3] * x + R.r[4] * y + R.r[5] * z;
+    double rz = R.r[6] * x + R.r[7] * y + R.r[8] * z;
+    if (pexp)
+    {
+        double const sc = __longlong_as_double(int64_t(1023 - int(pexp[i])) << 52); // 2^-e, |e| < 1023
+        rx *= sc;
+        ry *= sc;
+        rz *= sc;
+    }
     float const hx = float(rx), hy = float(ry), hz = float(rz);
     out[2 * i] = make_float4(hx, hy, hz, g.w);
     out[2 * i + 1] = make_float4(float(rx - double(hx)), float(ry - double(hy)), float(rz - double(hz)), 0.0f);
@@ -367,10 +415,11 @@ __global__ void __launch_bounds__(kBlock) k_prep_stream(float4 const* __restrict
 template <int Q, int B, class Src>
 __device__ __forceinline__ void chain_block(Src const& src, int64_t base, int64_t n, float4 const* __restrict__ pts,
                                             int64_t npts, frame const& R, pole const& P, float tau,
-                                            int32_t const* __restrict__ counts, float* __restrict__ out,
-                                            double* __restrict__ part, float4* tile)
+                                            float4 const& far, int32_t const* __restrict__ counts,
+                                            float* __restrict__ out, double* __restrict__ part, float4* tile)
 {
     float ph[Q][3], pl[Q][3];                // the query, hi and lo
+    float tq[Q];                             // the lane's conditioning threshold (kFarTau for a far query)
     float vx[Q], vy[Q], vz[Q], vp[Q], va[Q]; // the previous stream point: v, l + s X, |y| + |z|
     float pn[Q], pd[Q];                      // the pending pair's (num, den), when `pending`
     double acc[Q];
@@ -378,6 +427,7 @@ __device__ __forceinline__ void chain_block(Src const& src, int64_t base, int64_
     for (int q = 0; q < Q; ++q)
     {
         query_split(src, base + int64_t(q) * B, n, R, ph[q], pl[q]);
+        tq[q] = query_tau(src, base + int64_t(q) * B, n, tau, far);
         acc[q] = 0.0;
         vx[q] = vy[q] = vz[q] = vp[q] = va[q] = 0.0f;
         pn[q] = 0.0f;
@@ -416,7 +466,7 @@ __device__ __forceinline__ void chain_block(Src const& src, int64_t base, int64_
                     float num = sf * (vy[q] * nz - vz[q] * ny);
                     float const pp = vp[q] * np;
                     float den = pp + (cf * (vx[q] * nx) + (vy[q] * ny + vz[q] * nz));
-                    bool const ill = __builtin_fabsf(num) + __builtin_fabsf(den) < tau * (pp + va[q] * na);
+                    bool const ill = __builtin_fabsf(num) + __builtin_fabsf(den) < tq[q] * (pp + va[q] * na);
                     if (__builtin_expect(ill || !(den > 0.0f), 0))
                     {
                         // the segment's start point: in this tile, or (j == 0) the last point of the previous one --
@@ -477,7 +527,7 @@ __device__ __forceinline__ void chain_block(Src const& src, int64_t base, int64_
     for (int q = 0; q < Q; ++q)
     {
         if (pending)
-            acc[q] += double(atan2_f32(pn[q], pd[q]));
+            acc[q] += double(atan2_f32_any(pn[q], pd[q]));
         int64_t const i = base + int64_t(q) * B;
         if (i < n)
         {
@@ -497,11 +547,11 @@ __device__ __forceinline__ void chain_block(Src const& src, int64_t base, int64_
 template <int Q, class Src>
 __global__ void __launch_bounds__(kBlock)
     k_frac_f32_chain(Src const src, int64_t n, float4 const* __restrict__ pts, int64_t npts, frame R, pole P,
-                     float tau, int32_t const* __restrict__ counts, float* __restrict__ out)
+                     float tau, float4 far, int32_t const* __restrict__ counts, float* __restrict__ out)
 {
     __shared__ float4 tile[2 * kBlock];
     int64_t const base = int64_t(blockIdx.x) * (kBlock * Q) + threadIdx.x;
-    chain_block<Q, kBlock>(src, base, n, pts, npts, R, P, tau, counts, out, nullptr, tile);
+    chain_block<Q, kBlock>(src, base, n, pts, npts, R, P, tau, far, counts, out, nullptr, tile);
 }
 
 // Launches with too few query tiles to fill the GPU: the stream split into chunks at walk starts (cuts, blockIdx.y
@@ -512,12 +562,12 @@ __global__ void __launch_bounds__(kBlock)
 template <int Q, class Src>
 __global__ void __launch_bounds__(kBlock)
     k_frac_f32_chain_part(Src const src, int64_t n, float4 const* __restrict__ pts, int64_t const* __restrict__ cuts,
-                          frame R, pole P, float tau, double* __restrict__ part)
+                          frame R, pole P, float tau, float4 far, double* __restrict__ part)
 {
     __shared__ float4 tile[2 * kBlock];
     int64_t const base = int64_t(blockIdx.x) * (kBlock * Q) + threadIdx.x;
     int64_t const c0 = cuts[blockIdx.y], c1 = cuts[blockIdx.y + 1];
-    chain_block<Q, kBlock>(src, base, n, pts + 2 * c0, c1 - c0, R, P, tau, nullptr, nullptr,
+    chain_block<Q, kBlock>(src, base, n, pts + 2 * c0, c1 - c0, R, P, tau, far, nullptr, nullptr,
                            part + int64_t(blockIdx.y) * n, tile);
 }
 __global__ void __launch_bounds__(kBlock) k_chain_sum(double const* __restrict__ part, int chunks, int64_t n,
@@ -557,8 +607,12 @@ __global__ void __launch_bounds__(B)
         if (it >= nitems)
             break;
         batch_frac_desc const D = meshes[item_mesh[it]];
-        chain_block<Q, B>(src_array<float>{pos}, item_q0[it] + threadIdx.x, D.q_end, pts + 2 * D.pt0, D.npts, R, P,
-                          tau, counts, out, nullptr, tile);
+        frame Rm; // the mesh's scaled frame: its stream was rotated and scaled by the same 2^-e (k_prep_stream)
+#pragma unroll
+        for (int k = 0; k < 9; ++k)
+            Rm.r[k] = R.r[k] * D.qscale;
+        chain_block<Q, B>(src_array<float>{pos}, item_q0[it] + threadIdx.x, D.q_end, pts + 2 * D.pt0, D.npts, Rm, P,
+                          tau, D.far, counts, out, nullptr, tile);
     }
 }
 
@@ -657,7 +711,7 @@ __device__ double loop_exact(float4 const* __restrict__ p, int count, float cons
         }
     }
     if (pending)
-        acc += double(atan2_f32(pn, pd));
+        acc += double(atan2_f32_any(pn, pd));
     return acc;
 }
 
@@ -701,7 +755,7 @@ constexpr float kInnerBeta = 1.0f / 16.0f;
 __global__ void __launch_bounds__(kDecideBlock)
     k_frac_decide(float const* __restrict__ pos, int64_t n, int32_t const* __restrict__ counts,
                   loop_rec const* __restrict__ tree, int nnodes, float4 const* __restrict__ pts, frame R, pole P,
-                  float tau, float beta4pi, double margin, float* __restrict__ val, int64_t* __restrict__ undecided,
+                  float tau, float4 far, float beta4pi, double margin, float* __restrict__ val, int64_t* __restrict__ undecided,
                   unsigned long long* __restrict__ nund)
 {
     __shared__ float4 stage[kDecideBlock / 64][128];
@@ -711,6 +765,7 @@ __global__ void __launch_bounds__(kDecideBlock)
     bool const valid = i < n;
     float ph[3], pl[3];
     query_split(src_array<float>{pos}, i, n, R, ph, pl);
+    float const tl = query_tau(src_array<float>{pos}, i, n, tau, far);
     double fe = 0.0;              // exact loops (radians)
     float est = 0.0f, bnd = 0.0f; // accepted nodes: -Omega estimate and remainder bound (steradians)
     int resume = valid ? 0 : INT32_MAX;
@@ -727,7 +782,7 @@ __global__ void __launch_bounds__(kDecideBlock)
         bool const down = __any(split);
         if (r.leaf && down)
         {
-            double const sl = loop_exact(pts + 2 * int64_t(r.start), r.count, ph, pl, P, tau, st, lane);
+            double const sl = loop_exact(pts + 2 * int64_t(r.start), r.count, ph, pl, P, tl, st, lane);
             if (split)
                 fe += sl;
         }
@@ -823,6 +878,40 @@ constexpr int64_t kSplitTiles = 2048;
 } // namespace
 
 frame make_pole_frame(double x0x, double x0y, double x0z) { return make_frame_impl(x0x, x0y, x0z); }
+
+int boundary_scale_exp(float const* h, int64_t nseg, float4* far)
+{
+    if (far)
+        *far = make_float4(0.0f, 0.0f, 0.0f, HUGE_VALF);
+    double lo[3] = {HUGE_VAL, HUGE_VAL, HUGE_VAL}, hi[3] = {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
+    for (int64_t i = 0; i < nseg; ++i)
+        for (int k = 0; k < 6; ++k)
+        {
+            double const v = double(h[7 * i + k]);
+            lo[k % 3] = std::min(lo[k % 3], v);
+            hi[k % 3] = std::max(hi[k % 3], v);
+        }
+    double ext = 0.0;
+    for (int a = 0; a < 3; ++a)
+        if (hi[a] >= lo[a])
+            ext = std::max(ext, hi[a] - lo[a]);
+    if (!(ext > 0.0) || !std::isfinite(ext))
+        return 0;
+    int const e = std::ilogb(ext);
+    if (far)
+        *far = make_float4(float(0.5 * (lo[0] + hi[0])), float(0.5 * (lo[1] + hi[1])), float(0.5 * (lo[2] + hi[2])),
+                           float(std::ldexp(1.0, std::min(26 + e, 127))));
+    return e;
+}
+
+frame scale_frame(frame const& R, int e)
+{
+    frame S = R;
+    double const sc = std::ldexp(1.0, -e);
+    for (double& v : S.r)
+        v *= sc;
+    return S;
+}
 
 // Unit-weight expansion and chain ordering of a host boundary into a point stream (x, y, z, flag). Weights must be
 // integers of magnitude <= 16 (0 drops the segment); otherwise returns false. Greedy walks in input order: start at
@@ -1122,6 +1211,7 @@ int frac_plan_build(float const* h, int64_t nseg, frac_plan& plan, hipStream_t s
     plan.loops_ok = false;
     plan.dec_npts = 0;
     plan.walk_starts.clear();
+    plan.sexp = boundary_scale_exp(h, nseg, &plan.far);
     std::vector<float4> stream_pts;
     int64_t nunit = 0;
     if (!chain_stream(h, nseg, stream_pts, nunit))
@@ -1261,9 +1351,11 @@ struct decide_state
 };
 constexpr size_t kDecidePoles = 8;
 
-// a record in the pole frame R; its ball grown by 0.1 % (+ 1e-6 |c|) and its bound numerator by 0.1 % so that the
-// FP32 tests stay conservative
-loop_rec pole_record(frame const& R, loop_host const& L, int64_t start, int64_t count)
+// a record in the pole frame R (orthonormal), in coordinates scaled by sc (a power of two): lengths times sc, the
+// vector area times sc^2, the first moment times sc^3 and the bound's numerator 3 rho^2 A times sc^4, so every test
+// and estimate of the decision is the unscaled one; its ball grown by 0.1 % (+ 1e-6 |c|) and its bound numerator by
+// 0.1 % so that the FP32 tests stay conservative
+loop_rec pole_record(frame const& R, double sc, loop_host const& L, int64_t start, int64_t count)
 {
     loop_rec r{};
     double c[3], a[3], t[9], mr[9];
@@ -1280,18 +1372,19 @@ loop_rec pole_record(frame const& R, loop_host const& L, int64_t start, int64_t 
             mr[3 * i + j] = t[3 * i] * R.r[3 * j] + t[3 * i + 1] * R.r[3 * j + 1] + t[3 * i + 2] * R.r[3 * j + 2];
     double const cn = std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
     double const rho = L.rho * 1.001 + 1e-6 * cn;
-    r.cx = float(c[0]);
-    r.cy = float(c[1]);
-    r.cz = float(c[2]);
-    r.rho = float(rho);
-    r.ax = float(a[0]);
-    r.ay = float(a[1]);
-    r.az = float(a[2]);
-    r.k3 = float(3.0 * rho * rho * L.area * 1.001);
+    double const sc2 = sc * sc, sc3 = sc2 * sc;
+    r.cx = float(c[0] * sc);
+    r.cy = float(c[1] * sc);
+    r.cz = float(c[2] * sc);
+    r.rho = float(rho * sc);
+    r.ax = float(a[0] * sc2);
+    r.ay = float(a[1] * sc2);
+    r.az = float(a[2] * sc2);
+    r.k3 = float(3.0 * (rho * sc) * (rho * sc) * (L.area * sc2) * 1.001);
     for (int i = 0; i < 9; ++i)
-        r.m[i] = float(mr[i]);
-    r.trm = float(mr[0] + mr[4] + mr[8]);
-    r.cnorm = float(cn);
+        r.m[i] = float(mr[i] * sc3);
+    r.trm = float((mr[0] + mr[4] + mr[8]) * sc3);
+    r.cnorm = float(cn * sc);
     r.start = int32_t(start);
     r.count = int32_t(count);
     return r;
@@ -1302,6 +1395,8 @@ loop_rec pole_record(frame const& R, loop_host const& L, int64_t start, int64_t 
 int build_decide_tables(frac_plan const& plan, std::vector<tree_node> const& nodes, double const x0[3],
                         frame const& R, hipStream_t s, decide_tables& t)
 {
+    double const sc = std::ldexp(1.0, -plan.sexp);
+    frame const Rs = scale_frame(R, plan.sexp);
     for (int a = 0; a < 3; ++a)
         t.x0[a] = x0[a];
     std::vector<loop_rec> tab(nodes.size());
@@ -1311,12 +1406,12 @@ int build_decide_tables(frac_plan const& plan, std::vector<tree_node> const& nod
         if (N.loop >= 0)
         {
             loop_host const& L = plan.loops[size_t(N.loop)];
-            tab[k] = pole_record(R, L, L.start, L.count);
+            tab[k] = pole_record(R, sc, L, L.start, L.count);
             tab[k].leaf = 1;
         }
         else
         {
-            tab[k] = pole_record(R, N.L, 0, 0);
+            tab[k] = pole_record(R, sc, N.L, 0, 0);
             tab[k].leaf = 0;
         }
         tab[k].skip = N.skip;
@@ -1329,7 +1424,7 @@ int build_decide_tables(frac_plan const& plan, std::vector<tree_node> const& nod
     if (plan.dec_npts > 0)
     {
         k_prep_stream<<<dim3(unsigned((plan.dec_npts + kBlock - 1) / kBlock)), dim3(kBlock), 0, s>>>(
-            plan.dec.as<float4 const>(), plan.dec_npts, R, t.p32);
+            plan.dec.as<float4 const>(), plan.dec_npts, Rs, nullptr, t.p32);
         AGWN_HIP(hipGetLastError());
     }
     AGWN_HIP(hipStreamSynchronize(s)); // the host table goes out of scope; the tables may be shared from now on
@@ -1378,7 +1473,7 @@ int frac_decide(float const* pos, int64_t m, int32_t const* counts, frac_plan co
     int64_t const blocks = (m + kDecideBlock - 1) / kDecideBlock;
     AGWN_CHECK(blocks < (int64_t(1) << 31), "frac_decide: too many queries (%lld)", (long long)m);
     k_frac_decide<<<dim3(unsigned(blocks)), dim3(kDecideBlock), 0, s>>>(
-        pos, m, counts, T->tab, T->ntab, T->p32, R, P, kTau, beta * kFourPiF,
+        pos, m, counts, T->tab, T->ntab, T->p32, scale_frame(R, plan.sexp), P, kTau, plan.far, beta * kFourPiF,
         kDecideMargin, val, undecided, dn.as<unsigned long long>());
     AGWN_HIP(hipGetLastError());
     unsigned long long h = 0;
@@ -1394,7 +1489,7 @@ int frac_f32_planned(Src const& pos, int64_t n, frac_plan const& plan, float con
 {
     if (n == 0)
         return ok;
-    frame const R = make_frame_impl(x0[0], x0[1], x0[2]);
+    frame const R = scale_frame(make_frame_impl(x0[0], x0[1], x0[2]), plan.sexp);
     pole const P = make_pole(x0[0], x0[1], x0[2]);
     int64_t const per_block = int64_t(kBlock) * kQ;
     int64_t const blocks = (n + per_block - 1) / per_block;
@@ -1406,7 +1501,7 @@ int frac_f32_planned(Src const& pos, int64_t n, frac_plan const& plan, float con
         if (plan.npts > 0)
         {
             k_prep_stream<<<dim3(unsigned((plan.npts + kBlock - 1) / kBlock)), dim3(kBlock), 0, stream>>>(
-                plan.pts(), plan.npts, R, p32.as<float4>());
+                plan.pts(), plan.npts, R, nullptr, p32.as<float4>());
             AGWN_HIP(hipGetLastError());
         }
         if (split_small && blocks < kSplitBelow && plan.walk_starts.size() > 1)
@@ -1428,7 +1523,7 @@ int frac_f32_planned(Src const& pos, int64_t n, frac_plan const& plan, float con
             AGWN_HIP(hipMemcpyAsync(dc.p, cuts.data(), cuts.size() * sizeof(int64_t), hipMemcpyHostToDevice, stream));
             AGWN_TRY(part.alloc(size_t(chunks) * size_t(n) * sizeof(double), stream));
             k_frac_f32_chain_part<kQ, Src><<<dim3(unsigned(blocks), unsigned(chunks)), dim3(kBlock), 0, stream>>>(
-                pos, n, p32.as<float4>(), dc.as<int64_t>(), R, P, kTau, part.as<double>());
+                pos, n, p32.as<float4>(), dc.as<int64_t>(), R, P, kTau, plan.far, part.as<double>());
             AGWN_HIP(hipGetLastError());
             k_chain_sum<<<dim3(unsigned((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, stream>>>(
                 part.as<double>(), chunks, n, counts, out);
@@ -1437,7 +1532,7 @@ int frac_f32_planned(Src const& pos, int64_t n, frac_plan const& plan, float con
             return ok;
         }
         k_frac_f32_chain<kQ, Src><<<dim3(unsigned(blocks)), dim3(kBlock), 0, stream>>>(
-            pos, n, p32.as<float4>(), plan.npts, R, P, kTau, counts, out);
+            pos, n, p32.as<float4>(), plan.npts, R, P, kTau, plan.far, counts, out);
         AGWN_HIP(hipGetLastError());
         return ok;
     }
@@ -1453,7 +1548,7 @@ int frac_f32_planned(Src const& pos, int64_t n, frac_plan const& plan, float con
     }
     k_frac_f32<kQ, Src><<<dim3(unsigned(blocks)), dim3(kBlock), 0, stream>>>(pos, n, s32.as<seg_f32>(),
                                                                            s64.as<seg_f64>(), npad, R, P, kTau,
-                                                                           counts, out);
+                                                                           plan.far, counts, out);
     AGWN_HIP(hipGetLastError());
     return ok;
 }
@@ -1519,12 +1614,13 @@ void frac_batch_shape(int v, int& q, int& block)
     block = kBatchShapes[v][1];
 }
 
-int frac_stream_rotate(float4 const* in, int64_t n, double const x0[3], float4* out, hipStream_t s)
+int frac_stream_rotate(float4 const* in, int64_t n, double const x0[3], int16_t const* pexp, float4* out,
+                       hipStream_t s)
 {
     if (n == 0)
         return ok;
     frame const R = make_frame_impl(x0[0], x0[1], x0[2]);
-    k_prep_stream<<<dim3(unsigned((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s>>>(in, n, R, out);
+    k_prep_stream<<<dim3(unsigned((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s>>>(in, n, R, pexp, out);
     AGWN_HIP(hipGetLastError());
     return ok;
 }

@@ -107,6 +107,16 @@ struct frame
 };
 frame make_pole_frame(double x0x, double x0y, double x0z);
 
+// The FP32 kernels' coordinate scale: e such that the boundary's extent (the largest of its endpoints' per-axis
+// max - min, in double) lies in [2^e, 2^(e+1)); 0 for a boundary without extent. Coordinates are multiplied by 2^-e
+// (exact), so the FP32 pair's range limits are relative to the boundary's size and a power-of-two scaling of every
+// input leaves each FP32 operation's rounding, and so every result, unchanged. far (may be null) receives the centre
+// of the endpoints' box (caller frame) and 2^(26 + e): a query farther than that from the centre in any coordinate
+// runs all its pairs through the FP64 path (the FP32 products of its pairs could overflow).
+int boundary_scale_exp(float const* seg7_host, int64_t nseg, float4* far);
+// R with every entry multiplied by 2^-e
+frame scale_frame(frame const& R, int e);
+
 // Packed, rotated boundary segment: endpoints a, b in the pole frame as float pairs (hi = the FP64 coordinate rounded
 // to float, lo = the rounded remainder; hi + lo carries ~48 bits), weight w. 64 B.
 struct alignas(64) seg_f32
@@ -209,6 +219,8 @@ struct frac_plan
     dbuf buf;           // npts float4 entries (device)
     int64_t npts = 0;   // stream entries
     int64_t nunit = 0;  // unit segments
+    int sexp = 0;       // boundary_scale_exp of the boundary: the FP32 kernels' frame is scaled by 2^-sexp
+    float4 far{0.0f, 0.0f, 0.0f, HUGE_VALF}; // its far-query test (centre, threshold)
     bool chain = false;
     std::vector<int64_t> walk_starts; // stream entries with flag 0 after an even number of segments (host copy;
                                       // the cuts of small launches)
@@ -305,12 +317,16 @@ struct batch_frac_desc
 {
     int64_t pt0, npts; // the mesh's point stream within the concatenated stream
     int64_t q_end;     // one past the mesh's last query in the concatenated query array
+    double qscale;     // 2^-e, e = boundary_scale_exp of the mesh's boundary (its queries are scaled by it)
+    float4 far;        // the boundary's far-query test (boundary_scale_exp)
 };
 bool chain_stream(float const* seg7_host, int64_t nseg, std::vector<float4>& stream_pts, int64_t& nunit);
 int frac_batch_variants();                         // number of (queries per lane, block) shapes
 void frac_batch_shape(int variant, int& q, int& block);
-// rotates the concatenated stream into the pole frame of x0: two float4 per point, (hi, flag) and (lo, 0)
-int frac_stream_rotate(float4 const* in, int64_t n, double const x0[3], float4* out, hipStream_t s);
+// rotates the concatenated stream into the pole frame of x0, each point scaled by 2^-pexp[i] (its mesh's exponent;
+// device int16 per point): two float4 per point, (hi, flag) and (lo, 0)
+int frac_stream_rotate(float4 const* in, int64_t n, double const x0[3], int16_t const* pexp, float4* out,
+                       hipStream_t s);
 int frac_batch_launch(int variant, int blocks, float const* pos, batch_frac_desc const* meshes,
                       int32_t const* item_mesh, int64_t const* item_q0, int64_t nitems, unsigned long long* counter,
                       float4 const* pts, double const x0[3], int32_t const* counts, float* out, hipStream_t s);

@@ -1500,7 +1500,10 @@ struct agwn_batch
     std::vector<std::unique_ptr<agwn::mesh_t<float>>> meshes;
     std::vector<int64_t> pt_off; // k + 1 offsets into the point stream
     std::vector<int64_t> nunit;  // unit boundary segments per mesh
+    std::vector<int> sexp;       // per mesh: boundary_scale_exp of its boundary (the FP32 coordinate scale)
+    std::vector<float4> far;     // per mesh: its far-query test
     agwn::dbuf stream;           // the concatenated point streams, caller frame (persistent)
+    agwn::dbuf pexp;             // int16 per stream point: its mesh's sexp
     agwn::dbuf zero;             // an int32 zero: the exhaustive-test count of meshes without a BVH state
     hipStream_t bs = nullptr;    // the batch's own stream (graph capture and replay)
     int variant = -1;            // tuned fractional kernel shape
@@ -1621,7 +1624,10 @@ int batch_create(int64_t k, float const* verts, int64_t const* voff, int32_t con
     b->k = k;
     b->pt_off.assign(size_t(k + 1), 0);
     b->nunit.assign(size_t(k), 0);
+    b->sexp.assign(size_t(k), 0);
+    b->far.assign(size_t(k), make_float4(0.0f, 0.0f, 0.0f, HUGE_VALF));
     std::vector<float4> all;
+    std::vector<int16_t> allexp;
     for (int64_t m = 0; m < k; ++m)
     {
         int64_t const nv = voff[m + 1] - voff[m];
@@ -1637,14 +1643,23 @@ int batch_create(int64_t k, float const* verts, int64_t const* voff, int32_t con
         int64_t nu = 0;
         AGWN_CHECK(chain_stream(seg7 + 7 * soff[m], soff[m + 1] - soff[m], st, nu),
                    "mesh %lld: batched evaluation needs integer boundary weights (|w| <= 16)", (long long)m);
+        float4 far{};
+        int const e = boundary_scale_exp(seg7 + 7 * soff[m], soff[m + 1] - soff[m], &far);
         all.insert(all.end(), st.begin(), st.end());
+        allexp.insert(allexp.end(), st.size(), int16_t(e));
         b->pt_off[size_t(m + 1)] = int64_t(all.size());
         b->nunit[size_t(m)] = nu;
+        b->sexp[size_t(m)] = e;
+        b->far[size_t(m)] = far;
     }
     AGWN_HIP(hipStreamCreateWithFlags(&b->bs, hipStreamNonBlocking));
     AGWN_TRY(b->stream.alloc(std::max<size_t>(all.size(), 1) * sizeof(float4), nullptr));
+    AGWN_TRY(b->pexp.alloc(std::max<size_t>(allexp.size(), 1) * sizeof(int16_t), nullptr));
     if (!all.empty())
+    {
         AGWN_HIP(hipMemcpy(b->stream.p, all.data(), all.size() * sizeof(float4), hipMemcpyHostToDevice));
+        AGWN_HIP(hipMemcpy(b->pexp.p, allexp.data(), allexp.size() * sizeof(int16_t), hipMemcpyHostToDevice));
+    }
     AGWN_TRY(b->zero.alloc(sizeof(int32_t), nullptr));
     AGWN_HIP(hipMemset(b->zero.p, 0, sizeof(int32_t)));
     int dev = 0;
@@ -1712,7 +1727,8 @@ int batch_prepare(agwn_batch& B, float const* pos, int64_t const* qoff, float co
             B.states[size_t(m)] = st;
         }
         cd[size_t(m)] = d;
-        fd[size_t(m)] = batch_frac_desc{B.pt_off[size_t(m)], B.pt_off[size_t(m + 1)] - B.pt_off[size_t(m)], qoff[m + 1]};
+        fd[size_t(m)] = batch_frac_desc{B.pt_off[size_t(m)], B.pt_off[size_t(m + 1)] - B.pt_off[size_t(m)], qoff[m + 1],
+                                        std::ldexp(1.0, -B.sexp[size_t(m)]), B.far[size_t(m)]};
     }
     // work items, longest first: counting by triangle count, the fractional term by stream length
     int fq = 0, fb = 0;
@@ -1810,7 +1826,8 @@ int batch_launches(agwn_batch& B, float const* pos, float const x0f[3], int vari
     hipStream_t const s = B.bs;
     double const x0d[3] = {double(x0f[0]), double(x0f[1]), double(x0f[2])};
     int64_t const npts = B.pt_off[size_t(B.k)];
-    AGWN_TRY(frac_stream_rotate(B.stream.as<float4 const>(), npts, x0d, B.p32.as<float4>(), s));
+    AGWN_TRY(frac_stream_rotate(B.stream.as<float4 const>(), npts, x0d, B.pexp.as<int16_t const>(),
+                                B.p32.as<float4>(), s));
     k_zero_u64<<<dim3(1), dim3(kB), 0, s>>>(B.counters.as<unsigned long long>(), 2);
     AGWN_HIP(hipGetLastError());
     if (B.ncitems > 0)
