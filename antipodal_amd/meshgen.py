@@ -19,6 +19,8 @@ Workloads (sizes as named in the benchmark plan):
   precision_stress()    the precision-stress configuration: the torus family at 3040 x 1063 quads with 5000 holes of
                         radius 0.02 (4,038,497 triangles, 429,869 boundary segments), queries half uniform in the
                         10 %-expanded box and half at |N(0, 1e-3)| along the outward normal of surface points
+  batch_workload()      the uneven many-mesh configuration: 4096 open spheres/tori (100..2e5 triangles log-uniform,
+                        1..200 holes), 1e3..1e6 queries per mesh log-uniform (592,815,454 in all)
 """
 from __future__ import annotations
 
@@ -172,6 +174,14 @@ def surface_points(verts, idx, n: int, seed: int, stream: int = 10):
     return p, nrm[k]
 
 
+def offset_surface_points(verts, idx, n: int, seed: int, stream: int, sigma: float = 1e-3) -> np.ndarray:
+    """n points at |N(0, sigma)| along the outward normal of seeded area-uniform surface points (float64); the
+    surface points use streams stream..stream+2, the offsets stream + 10."""
+    p, nrm = surface_points(verts, idx, n, seed, stream=stream)
+    off = np.abs(normal(seed, stream + 10, n)) * sigma
+    return p + off[:, None] * nrm
+
+
 def precision_stress(n_queries: int = 10_000_000, seed: int = 1):
     """(verts, idx, queries float64 (n, 3)) of the precision-stress configuration."""
     verts, idx = holed_torus_mesh(seed=seed, nu=3040, nv=1063, holes=5000, hole_radius=0.02)
@@ -233,6 +243,33 @@ def open_mesh(kind: str, ntri: int, holes: int, seed: int, noise: float = 0.01):
     return _compact(verts, tri.reshape(-1))
 
 
+def batch_workload(k: int = 4096, seed: int = 7, workers: int = 8):
+    """The uneven many-mesh workload: k seeded open meshes, noisy spheres (even m) and tori (odd m) alternating, with
+    triangle counts log-uniform in [100, 2e5] and 1..200 holes, and a query count per mesh log-uniform in [1e3, 1e6].
+    The draws come from numpy's default_rng(seed), three per mesh in mesh order (triangles, holes, queries), so the
+    workload does not depend on `workers` (threads that build the meshes; numpy and scipy release the GIL).
+    Returns (meshes: [(float32 vertices (nv, 3), int32 indices)], query counts, [(box lo, box hi)] of each mesh's
+    10 %-expanded bounding box)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    rng = np.random.default_rng(seed)
+    specs, qn = [], []
+    for m in range(k):
+        ntri = int(math.exp(rng.uniform(math.log(100), math.log(2e5))))
+        holes = int(rng.integers(1, 201))
+        qn.append(int(math.exp(rng.uniform(math.log(1e3), math.log(1e6)))))
+        specs.append(("sphere" if m % 2 == 0 else "torus", ntri, holes, 100003 + m))
+
+    def one(s):
+        v, idx = open_mesh(*s)
+        v = v.astype(np.float32)
+        return (v, idx), expanded_bbox(v)
+
+    with ThreadPoolExecutor(max(1, workers)) as ex:
+        res = list(ex.map(one, specs))
+    return [r[0] for r in res], qn, [r[1] for r in res]
+
+
 def expanded_bbox(verts: np.ndarray, frac: float = 0.10):
     lo = verts.min(axis=0)
     hi = verts.max(axis=0)
@@ -259,6 +296,20 @@ def grid_points(lo, hi, n: int, dtype=np.float64) -> np.ndarray:
     pts += (dy[None, None, None, :] * r[None, :, None, None])
     pts += (dz[None, None, None, :] * r[:, None, None, None])
     return pts.reshape(-1, 3)
+
+
+def grid_points_at(lo, hi, n: int, flat, dtype=np.float64) -> np.ndarray:
+    """grid_points(lo, hi, n, dtype)[flat] without the whole lattice: the same per-component operations
+    ((start + dx i) + dy j) + dz k in `dtype` (i fastest in the flat index)."""
+    start, dx, dy, dz = (np.asarray(a, dtype=dtype) for a in grid_axes(lo, hi, n))
+    flat = np.asarray(flat, dtype=np.int64)
+    i, j, k = (flat % n).astype(dtype), ((flat // n) % n).astype(dtype), (flat // (n * n)).astype(dtype)
+    pts = np.empty((len(flat), 3), dtype=dtype)
+    pts[...] = start
+    pts += dx[None, :] * i[:, None]
+    pts += dy[None, :] * j[:, None]
+    pts += dz[None, :] * k[:, None]
+    return pts
 
 
 EXAMPLE_X0 = (0.31, 0.57, 0.76)  # examples/main.cc:205 (normalised by the caller)

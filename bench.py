@@ -35,6 +35,7 @@ import json
 import os
 import sys
 import time
+from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
 
 import numpy as np
@@ -151,20 +152,11 @@ def bench_batched(args, agwn, mg, torch) -> int:
     import math
 
     k = args.batch_meshes
-    rng = np.random.default_rng(7)
     t = time.perf_counter()
-    meshes, qn, boxes = [], [], []
-    for m in range(k):
-        ntri = int(math.exp(rng.uniform(math.log(100), math.log(2e5))))
-        holes = int(rng.integers(1, 201))
-        v, idx = mg.open_mesh("sphere" if m % 2 == 0 else "torus", ntri, holes, seed=100003 + m)
-        v = v.astype(np.float32)
-        meshes.append((v, idx, agwn.build_boundary_segments(v, idx)))
-        qn.append(int(math.exp(rng.uniform(math.log(1e3), math.log(1e6)))))
-        lo, hi = mg.expanded_bbox(v)
-        boxes.append((lo, hi))
-        if (m + 1) % 512 == 0:
-            log(f"generated {m + 1}/{k} meshes ({time.perf_counter() - t:.1f} s)")
+    geo, qn, boxes = mg.batch_workload(k, seed=7, workers=min(16, os.cpu_count() or 1))
+    with ThreadPoolExecutor(min(16, os.cpu_count() or 1)) as ex:  # host extraction releases the GIL (ctypes)
+        segs = list(ex.map(lambda g: agwn.build_boundary_segments(*g), geo))
+    meshes = [(v, idx, b) for (v, idx), b in zip(geo, segs)]
     gen_s = time.perf_counter() - t
     q_off = np.concatenate([[0], np.cumsum(qn)]).astype(np.int64)
     nq = int(q_off[-1])

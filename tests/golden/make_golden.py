@@ -16,6 +16,14 @@ the generated arrays) and the reference's outputs on them.
                          random schemes (20-bit int grid, 50-bit grid, 2^-20 double lattice incl. collinear cases)
   torus_sub.npz          the 1M-triangle holed torus (generator parameters + SHA-256), 384 queries of its 256^3 grid:
                          reference fractional (f32, f64) and NaiveIntersector counts (f32, f64)
+  torus_ref.npz          the same torus: 16,384 seeded lattice points of its 256^3 grid + 4,096 points 1e-3 off its
+                         surface (regenerated from the seeds; SHA-256 of the positions kept), the reference's counts,
+                         fractional terms, full GWN and labels in FP32 and FP64, and its boundary multiset's SHA-256
+
+  c5_cap.npz             4096 seeded points of the 512^3 lattice over the config-1 cap: the reference's full GWN and
+                         labels (x0 = +x), FP32 and FP64
+
+    python tests/golden/make_golden.py [fixture ...]   (only the named fixtures, e.g. torus_ref)
 """
 from __future__ import annotations
 
@@ -181,11 +189,65 @@ def torus_sub(ref: Reference) -> None:
     np.savez_compressed(OUT / "torus_sub.npz", **d)
 
 
+def torus_ref(ref: Reference) -> None:
+    """16,384 seeded points of the 256^3 lattice over the 1M-triangle torus plus 4,096 points 1e-3 off its surface:
+    the reference's NaiveIntersector counts, fractional terms and full-GWN labels (GWN > 0.5), FP32 and FP64
+    (OpenMPDispatcher), and the SHA-256 of its boundary multiset (segments sorted lexicographically)."""
+    from oracle import sort_segments
+
+    verts, idx = mg.holed_torus_mesh()
+    lo, hi = mg.expanded_bbox(verts)
+    n = 256
+    flat = np.sort(np.random.default_rng(17).choice(n ** 3, size=16384, replace=False))
+    near = mg.offset_surface_points(verts, idx, 4096, seed=5, stream=50)
+    d = {"mesh_sha": np.bytes_(sha(verts, idx)), "flat": flat, "grid_n": np.int64(n), "n_grid": np.int64(len(flat)),
+         "n_near": np.int64(len(near))}
+    for dt, s in ((np.float64, "f64"), (np.float32, "f32")):
+        pts = np.concatenate([mg.grid_points_at(lo, hi, n, flat, dtype=dt), near.astype(dt)])
+        v = verts.astype(dt)
+        x0 = mg.example_x0(dt)
+        seg = ref.boundary(v, idx, dt)
+        d[f"boundary_sha_{s}"] = np.bytes_(sha(sort_segments(seg)))
+        d[f"nseg_{s}"] = np.int64(len(seg))
+        d[f"pos_sha_{s}"] = np.bytes_(sha(pts))
+        d[f"frac_{s}"] = ref.frac(pts, seg, x0, dt, mode=1)
+        d[f"count_{s}"] = ref.naive_count(v, idx, pts, x0, dt, mode=1).astype(np.int8)
+        full = ref.full(v, idx, pts, x0, seg, dt, mode=1)
+        d[f"label_{s}"] = np.packbits(full > 0.5)
+        d[f"full_{s}"] = full
+        print(f"torus_ref {s}: {len(pts)} points", flush=True)
+    np.savez_compressed(OUT / "torus_ref.npz", **d)
+
+
+def c5_cap(ref: Reference) -> None:
+    """4096 seeded points of the 512^3 lattice over the config-1 cap (its 10 %-expanded box), the lattice the
+    surrogate labels: the reference's full GWN (NaiveIntersector, x0 = +x, the lattice's integration axis) and its
+    labels, FP32 on the float32 lattice points and FP64 on the float64 ones."""
+    verts, idx = mg.icosphere_cap_mesh()
+    lo, hi = mg.expanded_bbox(verts)
+    n = 512
+    flat = np.sort(np.random.default_rng(29).choice(n ** 3, size=4096, replace=False))
+    d = {"flat": flat, "grid_n": np.int64(n)}
+    for dt, s in ((np.float64, "f64"), (np.float32, "f32")):
+        pts = mg.grid_points_at(lo, hi, n, flat, dtype=dt)
+        v = verts.astype(dt)
+        x0 = np.array([1.0, 0.0, 0.0], dtype=dt)
+        seg = ref.boundary(v, idx, dt)
+        full = ref.full(v, idx, pts, x0, seg, dt, mode=1)
+        d[f"pos_sha_{s}"] = np.bytes_(sha(pts))
+        d[f"full_{s}"] = full
+        d[f"label_{s}"] = np.packbits(full > 0.5)
+    np.savez_compressed(OUT / "c5_cap.npz", **d)
+
+
 def main() -> None:
     ref = Reference()
     if not ref.available():
         raise SystemExit("oracle/_ref/libref.so missing: run `make -C oracle` where the reference sources are")
-    for f in (pair_kernel, c1, cube, predicates, torus_sub):
+    only = set(sys.argv[1:])
+    for f in (pair_kernel, c1, cube, predicates, torus_sub, torus_ref, c5_cap):
+        if only and f.__name__ not in only:
+            continue
         f(ref)
         print("wrote", f.__name__)
 

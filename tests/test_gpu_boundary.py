@@ -50,3 +50,24 @@ def test_boundary_edge_cases(gpu):
         gpu.build_boundary_segments(v, torch.tensor([0, 1, 3], dtype=torch.int32, device="cuda"))
     with pytest.raises(ValueError):
         gpu.build_boundary_segments(v, torch.tensor([0, 1], dtype=torch.int32, device="cuda"))
+
+
+@pytest.mark.parametrize("s,dt", [("f64", np.float64), ("f32", np.float32)])
+def test_boundary_matches_reference_golden(gpu, s, dt):
+    """The device extraction against the reference's own build_boundary_segments output (c1_icosphere_cap.npz and
+    cube.npz, written by the reference's code): the same multiset of oriented weighted segments (the reference's order
+    is unordered_map order, so both are sorted before comparing)."""
+    import torch
+
+    from oracle import sort_segments
+
+    c1 = load_golden("c1_icosphere_cap")
+    cube = load_golden("cube")
+    cases = [("c1", c1["verts"], c1["idx"], c1[f"seg_{s}"]),
+             ("cube_open_y", cube["verts"], cube["open_y"], cube[f"boundary_ref_{s}"])]
+    for name, v, idx, ref in cases:
+        vt = torch.from_numpy(np.ascontiguousarray(v, dtype=dt)).cuda()
+        it = torch.from_numpy(np.ascontiguousarray(idx, dtype=np.int32)).cuda()
+        dev = gpu.build_boundary_segments(vt, it).cpu().numpy()
+        assert dev.dtype == ref.dtype and dev.shape == ref.shape, (name, dev.shape, ref.shape)
+        assert np.array_equal(sort_segments(dev), sort_segments(ref)), name

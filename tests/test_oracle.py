@@ -167,3 +167,48 @@ def test_torus_subsample_matches_reference(oracle, s, dt):
     fr = oracle.frac(pos, seg, x0, dt)
     tol = 1e-10 if dt == np.float64 else 2e-3
     assert np.max(np.abs(fr - g[f"frac_{s}"][:96])) < tol
+
+
+def test_torus_ref_fixture_regenerates_and_matches_oracle(oracle):
+    """torus_ref.npz's queries regenerate bit for bit from their seeds, and the oracle restatement reproduces the
+    reference's counts (bit-exact) and fractional terms (FP64 1e-10; its FP32 sum to the reference's FP32 rounding,
+    2e-2 next to the surface) on a 256-point slice of lattice and near-surface queries."""
+    import hashlib
+
+    g = load_golden("torus_ref")
+    verts, idx = mg.holed_torus_mesh()
+    lo, hi = mg.expanded_bbox(verts)
+    near = mg.offset_surface_points(verts, idx, int(g["n_near"]), seed=5, stream=50)
+    pick = np.r_[0:128, 16384:16384 + 128]
+    for s, dt in (("f64", np.float64), ("f32", np.float32)):
+        pos = np.concatenate([mg.grid_points_at(lo, hi, int(g["grid_n"]), g["flat"], dtype=dt), near.astype(dt)])
+        assert hashlib.sha256(pos.tobytes()).hexdigest() == g[f"pos_sha_{s}"].item().decode()
+        p = pos[pick]
+        v = verts.astype(dt)
+        x0 = mg.example_x0(dt)
+        assert np.array_equal(oracle.naive_count(v, idx, p, x0, dt), g[f"count_{s}"][pick].astype(np.int32))
+        seg = oracle.boundary(v, idx, dt)
+        assert len(seg) == int(g[f"nseg_{s}"])
+        fr = oracle.frac(p, seg, x0, dt)
+        assert np.max(np.abs(fr - g[f"frac_{s}"][pick])) < (1e-10 if dt == np.float64 else 2e-2)
+
+
+def test_c5_cap_fixture_matches_oracle(oracle):
+    """c5_cap.npz: the oracle's full GWN (x0 = +x) on the 4096 lattice points equals the reference's to FP64 rounding
+    (1e-12) and gives its labels in both precisions."""
+    import hashlib
+
+    g = load_golden("c5_cap")
+    verts, idx = mg.icosphere_cap_mesh()
+    lo, hi = mg.expanded_bbox(verts)
+    for s, dt in (("f64", np.float64), ("f32", np.float32)):
+        p = mg.grid_points_at(lo, hi, int(g["grid_n"]), g["flat"], dtype=dt)
+        assert hashlib.sha256(p.tobytes()).hexdigest() == g[f"pos_sha_{s}"].item().decode()
+        v = verts.astype(dt)
+        x0 = np.array([1.0, 0.0, 0.0], dtype=dt)
+        seg = oracle.boundary(v, idx, dt)
+        full = oracle.full(v, idx, p, x0, seg, dt)
+        ref = g[f"full_{s}"]
+        assert np.max(np.abs(full - ref)) < (1e-12 if dt == np.float64 else 1e-4)
+        lab = np.unpackbits(g[f"label_{s}"])[:len(p)].astype(bool)
+        assert np.array_equal(full > 0.5, lab)
