@@ -17,7 +17,7 @@ from antipodal_amd import meshgen as mg
 
 pytestmark = pytest.mark.gpu
 
-N_QUERIES = 1 << 20
+N_QUERIES = 10_000_000  # the configuration's query count
 
 
 @pytest.fixture(scope="module")
@@ -25,6 +25,7 @@ def stress(gpu):
     import torch
 
     verts, idx, q = mg.precision_stress(N_QUERIES)
+    print(f"case precision stress: {len(idx) // 3} triangles, {len(q)} queries", flush=True)
     v32 = verts.astype(np.float32)
     vt = torch.from_numpy(v32).cuda()
     it = torch.from_numpy(idx.astype(np.int32)).cuda()
@@ -34,6 +35,7 @@ def stress(gpu):
     return v32, idx, seg32, q32, x0
 
 
+@pytest.mark.timeout(600)
 def test_precision_stress_fp32_vs_fp64(gpu, oracle, stress):
     import torch
 
