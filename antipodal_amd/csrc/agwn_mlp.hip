@@ -631,77 +631,123 @@ __global__ void __launch_bounds__(256, kInferBlocks) k_mlp_infer(Src const src, 
         }
     }
 
-    // features: a point's 96 pairs in 8 parts of 12; work item (point p, part) -> thread item % 256 (threads of a wave
-    // = consecutive points), each part written 4 pairs (16 B) at a time -- the rows' 528 B stride keeps each 16-lane
-    // group of those stores on distinct banks
-    for (int item = threadIdx.x; item < kTileP * 8; item += 256)
+    int const li = lane & 31;
+    int const hl = lane >> 5; // lane half
+
+    // features: the tile's 96 points in 3 chunks of 32, a point's 96 pairs in 24 groups of 4 (axis a = g / 8,
+    // frequencies 4 (g % 8) .. + 3). Wave w takes groups 6w .. 6w + 5 of every chunk, so the group -- and with it the
+    // axis and the frequencies -- is the same across the wave and the frequencies come from scalar registers; lane
+    // (li, hl) computes point li of the chunk at frequencies 4 (g % 8) + 2 hl + {0, 1}: 2 pairs, one 8-byte store.
+    // Same operations as the training features (feature_pair); points past n get the features of the origin, which
+    // nothing reads.
     {
-        int const p = item % kTileP, part = item / kTileP;
-        int64_t const i = base + p;
-        float x = 0.0f, y = 0.0f, z = 0.0f;
-        bool const live = i < n;
-        if (live)
-            query_at(src, i, x, y, z);
-        for (int g = 0; g < 3; ++g)
-        {
-            int const j0 = 12 * part + 4 * g;
-            bf16x8 v;
+        float u[kTileJ][3];
 #pragma unroll
-            for (int q = 0; q < 4; ++q)
+        for (int c = 0; c < kTileJ; ++c)
+        {
+            int64_t const i = base + 32 * c + li;
+            float x = 0.0f, y = 0.0f, z = 0.0f;
+            if (i < n)
+                query_at(src, i, x, y, z);
+            u[c][0] = (x - F.c[0]) * F.inv_h[0];
+            u[c][1] = (y - F.c[1]) * F.inv_h[1];
+            u[c][2] = (z - F.c[2]) * F.inv_h[2];
+        }
+#pragma unroll 1
+        for (int gg = 0; gg < 6; ++gg)
+        {
+            int const g = __builtin_amdgcn_readfirstlane(6 * wave + gg);
+            int const a = g >> 3, k0 = 4 * (g & 7);
+            float const w0 = hl ? F.w[k0 + 2] : F.w[k0], w1 = hl ? F.w[k0 + 3] : F.w[k0 + 1];
+#pragma unroll
+            for (int c = 0; c < kTileJ; ++c)
             {
-                float s = 0.0f, c = 0.0f;
-                if (live)
-                    feature_pair(F, x, y, z, j0 + q, s, c);
-                v[2 * q] = __bf16(s);
-                v[2 * q + 1] = __bf16(c);
+                float const uc = a == 0 ? u[c][0] : (a == 1 ? u[c][1] : u[c][2]);
+                float const t0 = 0.5f * (w0 * uc), t1 = 0.5f * (w1 * uc);
+                bf16x4 v;
+                v[0] = __bf16(__builtin_amdgcn_sinf(t0));
+                v[1] = __bf16(__builtin_amdgcn_cosf(t0));
+                v[2] = __bf16(__builtin_amdgcn_sinf(t1));
+                v[3] = __bf16(__builtin_amdgcn_cosf(t1));
+                *reinterpret_cast<bf16x4*>(act + (32 * c + li) * kLdsStride + 2 * (32 * a + k0 + 2 * hl)) = v;
             }
-            *reinterpret_cast<bf16x8*>(act + p * kLdsStride + 2 * j0) = v;
         }
     }
     __syncthreads();
 
-    int const li = lane & 31;
-    int const kh = (lane >> 5) * 8;
+    int const kh = hl * 8;
     int const n0 = wave * 64;
-    f32x16 acc[2][kTileJ]; // starts at the biases of the lane's units (C[r][c]: unit r = 8(e/4) + 4(l/32) + e%4)
-    // layer l's MFMAs: acc = b_l + Wt_l[units of the wave] act^T
+    f32x16 acc[2][kTileJ]; // C[r][c] of the lane's units: unit r = 8(e/4) + 4(l/32) + e%4
+    // layer l's MFMAs: acc = b_l + Wt_l[units of the wave] act^T. The biases go into acc[i][0] and enter every point
+    // block's first MFMA as its C operand (no copies). The k loop runs two steps per iteration; step s's weight
+    // fragments (f[s & 1]) are fetched two steps ahead and its activation operands (b[s & 1], LDS) one step ahead,
+    // so the MFMAs of a step never wait for either; scheduling barriers keep the compiler from sinking the fetches
+    // back to their uses.
     auto layer = [&](int l) __attribute__((always_inline)) {
         int const K = l == 0 ? kIn : kH;
         // fragments of unit blocks 2w and 2w + 1 (units n0 .. n0 + 63); k step st at + 512 st
         __bf16 const* wa0 = P.wf[l] + (int64_t(n0 / 32) * (K / 16) * 64 + lane) * 8;
         __bf16 const* wa1 = wa0 + int64_t(K / 16) * 512;
+        __bf16 const* brow = act + li * kLdsStride + kh;
+        bf16x8 f0[2], f1[2], b[2][kTileJ];
+        f0[0] = *reinterpret_cast<bf16x8 const*>(wa0);
+        f1[0] = *reinterpret_cast<bf16x8 const*>(wa1);
+        f0[1] = *reinterpret_cast<bf16x8 const*>(wa0 + 512);
+        f1[1] = *reinterpret_cast<bf16x8 const*>(wa1 + 512);
 #pragma unroll
         for (int i = 0; i < 2; ++i)
 #pragma unroll
             for (int q = 0; q < 4; ++q)
             {
-                float4 const bq = *reinterpret_cast<float4 const*>(P.b[l] + n0 + i * 32 + 8 * q + 4 * (lane >> 5));
+                float4 const bq = *reinterpret_cast<float4 const*>(P.b[l] + n0 + i * 32 + 8 * q + 4 * hl);
+                acc[i][0][4 * q] = bq.x;
+                acc[i][0][4 * q + 1] = bq.y;
+                acc[i][0][4 * q + 2] = bq.z;
+                acc[i][0][4 * q + 3] = bq.w;
+            }
+#pragma unroll
+        for (int j = 0; j < kTileJ; ++j)
+            b[0][j] = *reinterpret_cast<bf16x8 const*>(brow + j * 32 * kLdsStride);
+        __builtin_amdgcn_sched_barrier(0);
+        auto step = [&](int k0, int buf, bool first) __attribute__((always_inline)) {
+            if (k0 + 16 < K) // the next step's activation operands
+            {
 #pragma unroll
                 for (int j = 0; j < kTileJ; ++j)
+                    b[buf ^ 1][j] = *reinterpret_cast<bf16x8 const*>(brow + j * 32 * kLdsStride + k0 + 16);
+            }
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+            {
+                bf16x8 const a = i == 0 ? f0[buf] : f1[buf];
+                if (first)
                 {
-                    acc[i][j][4 * q] = bq.x;
-                    acc[i][j][4 * q + 1] = bq.y;
-                    acc[i][j][4 * q + 2] = bq.z;
-                    acc[i][j][4 * q + 3] = bq.w;
+#pragma unroll
+                    for (int j = kTileJ - 1; j >= 0; --j) // acc[i][0] (the biases) is read by every block, last
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b[buf][j], acc[i][0], 0, 0, 0);
+                }
+                else
+                {
+#pragma unroll
+                    for (int j = 0; j < kTileJ; ++j)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b[buf][j], acc[i][j], 0, 0, 0);
                 }
             }
-        bf16x8 a0 = *reinterpret_cast<bf16x8 const*>(wa0);
-        bf16x8 a1 = *reinterpret_cast<bf16x8 const*>(wa1);
-#pragma unroll 1 // unrolled (K is a constant in the last layer's copy), the loads are hoisted and spill
-        for (int k0 = 0; k0 < K; k0 += 16)
-        {
-            int const kn = k0 + 16 < K ? k0 + 16 : k0;
-            bf16x8 const na0 = *reinterpret_cast<bf16x8 const*>(wa0 + int64_t(kn) * 32);
-            bf16x8 const na1 = *reinterpret_cast<bf16x8 const*>(wa1 + int64_t(kn) * 32);
-#pragma unroll
-            for (int j = 0; j < kTileJ; ++j)
+            int const kn = k0 + 32; // the step two ahead, into the buffer just consumed
+            if (kn < K)
             {
-                bf16x8 const b = *reinterpret_cast<bf16x8 const*>(act + (j * 32 + li) * kLdsStride + k0 + kh);
-                acc[0][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b, acc[0][j], 0, 0, 0);
-                acc[1][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b, acc[1][j], 0, 0, 0);
+                f0[buf] = *reinterpret_cast<bf16x8 const*>(wa0 + int64_t(kn) * 32);
+                f1[buf] = *reinterpret_cast<bf16x8 const*>(wa1 + int64_t(kn) * 32);
             }
-            a0 = na0;
-            a1 = na1;
+            __builtin_amdgcn_sched_barrier(0);
+        };
+        step(0, 0, true);
+        step(16, 1, false);
+#pragma unroll 1
+        for (int k0 = 32; k0 < K; k0 += 32)
+        {
+            step(k0, 0, false);
+            step(k0 + 16, 1, false);
         }
     };
 #pragma unroll 1
