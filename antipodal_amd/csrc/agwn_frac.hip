@@ -1432,7 +1432,7 @@ int build_decide_tables(frac_plan const& plan, std::vector<tree_node> const& nod
 }
 
 int frac_decide(float const* pos, int64_t m, int32_t const* counts, frac_plan const& plan, double const x0[3],
-                float beta, float* val, int64_t* undecided, int64_t& nund, hipStream_t s)
+                float beta, float* val, int64_t* undecided, int64_t& nund, hipStream_t s, arena* ws)
 {
     nund = 0;
     if (m == 0)
@@ -1467,17 +1467,24 @@ int frac_decide(float const* pos, int64_t m, int32_t const* counts, frac_plan co
                 own = std::move(e);
         }
     }
-    dbuf dn;
-    AGWN_TRY(dn.alloc(sizeof(unsigned long long), s));
-    AGWN_HIP(hipMemsetAsync(dn.p, 0, sizeof(unsigned long long), s));
+    dbuf dn_own;
+    unsigned long long* dn = nullptr;
+    if (ws)
+        AGWN_TRY(ws->take_n(1, &dn));
+    else
+    {
+        AGWN_TRY(dn_own.alloc(sizeof(unsigned long long), s));
+        dn = dn_own.as<unsigned long long>();
+    }
+    AGWN_HIP(hipMemsetAsync(dn, 0, sizeof(unsigned long long), s));
     int64_t const blocks = (m + kDecideBlock - 1) / kDecideBlock;
     AGWN_CHECK(blocks < (int64_t(1) << 31), "frac_decide: too many queries (%lld)", (long long)m);
     k_frac_decide<<<dim3(unsigned(blocks)), dim3(kDecideBlock), 0, s>>>(
         pos, m, counts, T->tab, T->ntab, T->p32, scale_frame(R, plan.sexp), P, kTau, plan.far, beta * kFourPiF,
-        kDecideMargin, val, undecided, dn.as<unsigned long long>());
+        kDecideMargin, val, undecided, dn);
     AGWN_HIP(hipGetLastError());
     unsigned long long h = 0;
-    AGWN_HIP(hipMemcpyAsync(&h, dn.p, sizeof(h), hipMemcpyDeviceToHost, s));
+    AGWN_HIP(hipMemcpyAsync(&h, dn, sizeof(h), hipMemcpyDeviceToHost, s));
     AGWN_HIP(hipStreamSynchronize(s)); // nund is needed now (and `own` may be freed)
     nund = int64_t(h);
     return ok;
@@ -1485,7 +1492,8 @@ int frac_decide(float const* pos, int64_t m, int32_t const* counts, frac_plan co
 
 template <class Src>
 int frac_f32_planned(Src const& pos, int64_t n, frac_plan const& plan, float const* seg7, int64_t nseg,
-                     double const x0[3], int32_t const* counts, float* out, hipStream_t stream, bool split_small)
+                     double const x0[3], int32_t const* counts, float* out, hipStream_t stream, bool split_small,
+                     arena* ws)
 {
     if (n == 0)
         return ok;
@@ -1496,12 +1504,20 @@ int frac_f32_planned(Src const& pos, int64_t n, frac_plan const& plan, float con
     AGWN_CHECK(blocks < (int64_t(1) << 31), "frac_f32: too many queries (%lld)", (long long)n);
     if (plan.chain)
     {
-        dbuf p32;
-        AGWN_TRY(p32.alloc(size_t(std::max<int64_t>(plan.npts, 1)) * 2 * sizeof(float4), stream));
+        dbuf p32_own;
+        float4* p32 = nullptr;
+        size_t const np32 = size_t(std::max<int64_t>(plan.npts, 1)) * 2;
+        if (ws)
+            AGWN_TRY(ws->take_n(np32, &p32));
+        else
+        {
+            AGWN_TRY(p32_own.alloc(np32 * sizeof(float4), stream));
+            p32 = p32_own.as<float4>();
+        }
         if (plan.npts > 0)
         {
             k_prep_stream<<<dim3(unsigned((plan.npts + kBlock - 1) / kBlock)), dim3(kBlock), 0, stream>>>(
-                plan.pts(), plan.npts, R, nullptr, p32.as<float4>());
+                plan.pts(), plan.npts, R, nullptr, p32);
             AGWN_HIP(hipGetLastError());
         }
         if (split_small && blocks < kSplitBelow && plan.walk_starts.size() > 1)
@@ -1509,7 +1525,11 @@ int frac_f32_planned(Src const& pos, int64_t n, frac_plan const& plan, float con
             // split the stream at walk starts into about kSplitTiles / blocks chunks
             int64_t const want = std::min<int64_t>({int64_t(32), (kSplitTiles + blocks - 1) / blocks,
                                                     int64_t(plan.walk_starts.size())});
-            std::vector<int64_t> cuts{0};
+            // the cuts are staged in host memory that outlives the copy: the workspace's (valid until its next reset,
+            // which follows the owner's synchronisation), else this call's, waited for below
+            std::vector<int64_t> own_cuts;
+            std::vector<int64_t>& cuts = ws ? ws->host : own_cuts;
+            cuts.assign(1, 0);
             for (int64_t c = 1; c < want; ++c)
             {
                 auto it = std::lower_bound(plan.walk_starts.begin(), plan.walk_starts.end(), c * plan.npts / want);
@@ -1518,21 +1538,34 @@ int frac_f32_planned(Src const& pos, int64_t n, frac_plan const& plan, float con
             }
             cuts.push_back(plan.npts);
             int const chunks = int(cuts.size() - 1);
-            dbuf dc, part;
-            AGWN_TRY(dc.alloc(cuts.size() * sizeof(int64_t), stream));
-            AGWN_HIP(hipMemcpyAsync(dc.p, cuts.data(), cuts.size() * sizeof(int64_t), hipMemcpyHostToDevice, stream));
-            AGWN_TRY(part.alloc(size_t(chunks) * size_t(n) * sizeof(double), stream));
+            dbuf dc_own, part_own;
+            int64_t* dc = nullptr;
+            double* part = nullptr;
+            if (ws)
+            {
+                AGWN_TRY(ws->take_n(cuts.size(), &dc));
+                AGWN_TRY(ws->take_n(size_t(chunks) * size_t(n), &part));
+            }
+            else
+            {
+                AGWN_TRY(dc_own.alloc(cuts.size() * sizeof(int64_t), stream));
+                AGWN_TRY(part_own.alloc(size_t(chunks) * size_t(n) * sizeof(double), stream));
+                dc = dc_own.as<int64_t>();
+                part = part_own.as<double>();
+            }
+            AGWN_HIP(hipMemcpyAsync(dc, cuts.data(), cuts.size() * sizeof(int64_t), hipMemcpyHostToDevice, stream));
             k_frac_f32_chain_part<kQ, Src><<<dim3(unsigned(blocks), unsigned(chunks)), dim3(kBlock), 0, stream>>>(
-                pos, n, p32.as<float4>(), dc.as<int64_t>(), R, P, kTau, plan.far, part.as<double>());
+                pos, n, p32, dc, R, P, kTau, plan.far, part);
             AGWN_HIP(hipGetLastError());
-            k_chain_sum<<<dim3(unsigned((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, stream>>>(
-                part.as<double>(), chunks, n, counts, out);
+            k_chain_sum<<<dim3(unsigned((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, stream>>>(part, chunks, n,
+                                                                                                counts, out);
             AGWN_HIP(hipGetLastError());
-            AGWN_HIP(hipStreamSynchronize(stream)); // `cuts` is a host vector
+            if (!ws)
+                AGWN_HIP(hipStreamSynchronize(stream)); // `cuts` is this call's host vector
             return ok;
         }
         k_frac_f32_chain<kQ, Src><<<dim3(unsigned(blocks)), dim3(kBlock), 0, stream>>>(
-            pos, n, p32.as<float4>(), plan.npts, R, P, kTau, plan.far, counts, out);
+            pos, n, p32, plan.npts, R, P, kTau, plan.far, counts, out);
         AGWN_HIP(hipGetLastError());
         return ok;
     }
@@ -1647,9 +1680,10 @@ int frac_batch_launch(int v, int blocks, float const* pos, batch_frac_desc const
 template int frac_f32_src<src_array<float>>(src_array<float> const&, int64_t, float const*, int64_t, double const*,
                                             int32_t const*, float*, hipStream_t);
 template int frac_f32_planned<src_array<float>>(src_array<float> const&, int64_t, frac_plan const&, float const*,
-                                                int64_t, double const*, int32_t const*, float*, hipStream_t, bool);
+                                                int64_t, double const*, int32_t const*, float*, hipStream_t, bool,
+                                                arena*);
 template int frac_f32_planned<src_grid>(src_grid const&, int64_t, frac_plan const&, float const*, int64_t,
-                                        double const*, int32_t const*, float*, hipStream_t, bool);
+                                        double const*, int32_t const*, float*, hipStream_t, bool, arena*);
 template int frac_f32_src<src_grid>(src_grid const&, int64_t, float const*, int64_t, double const*, int32_t const*,
                                     float*, hipStream_t);
 template int frac_f64_src<src_array<double>>(src_array<double> const&, int64_t, double const*, int64_t,

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -92,6 +93,60 @@ struct dbuf
     U* as() const
     {
         return static_cast<U*>(p);
+    }
+};
+
+// Device workspace that a handle keeps between calls: take() hands out 256-byte aligned pieces of a few large blocks,
+// which are hipMalloc'd only when a call needs more than the blocks hold and freed with the arena; reset() makes all
+// of it available again. A piece stays valid until the next reset(), so a call that resets must first have waited for
+// the previous call's work on the pieces; calls using one arena are serialised by its owner. (Per-call stream-ordered
+// allocation and release cost ~3 ms per labelling call of the 512^3 grid in the frees alone.)
+struct arena
+{
+    struct block
+    {
+        char* p;
+        size_t size, used;
+    };
+    std::vector<block> blocks;
+    std::vector<int64_t> host; // host staging whose copy to the device must outlive the call (until the next reset)
+    arena() = default;
+    arena(arena const&) = delete;
+    arena& operator=(arena const&) = delete;
+    ~arena()
+    {
+        for (block& b : blocks)
+            (void)hipFree(b.p);
+    }
+    void reset()
+    {
+        for (block& b : blocks)
+            b.used = 0;
+    }
+    int take(size_t bytes, void** out)
+    {
+        bytes = (std::max<size_t>(bytes, 1) + 255) & ~size_t(255);
+        for (block& b : blocks)
+            if (b.size - b.used >= bytes)
+            {
+                *out = b.p + b.used;
+                b.used += bytes;
+                return ok;
+            }
+        size_t const size = std::max(bytes, blocks.empty() ? size_t(64) << 20 : 2 * blocks.back().size);
+        void* p = nullptr;
+        AGWN_HIP(hipMalloc(&p, size));
+        blocks.push_back(block{static_cast<char*>(p), size, bytes});
+        *out = p;
+        return ok;
+    }
+    template <class U>
+    int take_n(size_t count, U** out)
+    {
+        void* p = nullptr;
+        AGWN_TRY(take(count * sizeof(U), &p));
+        *out = static_cast<U*>(p);
+        return ok;
     }
 };
 
@@ -269,7 +324,7 @@ int cached_plan(plan_cache& c, float const* seg7, int64_t nseg, hipStream_t s, s
 constexpr float kDecideBeta = 0.002f;
 constexpr float kDecideBetaTight = 0.0002f;
 int frac_decide(float const* pos, int64_t m, int32_t const* counts, frac_plan const& plan, double const x0[3],
-                float beta, float* val, int64_t* undecided, int64_t& nund, hipStream_t s);
+                float beta, float* val, int64_t* undecided, int64_t& nund, hipStream_t s, arena* ws = nullptr);
 
 // Fractional launchers (agwn_frac.hip). seg7/out/counts are device pointers. counts may be null (fractional
 // only); otherwise out[i] = fractional + counts[i] (the full GWN of gwn_mesh.hh:213-218).
@@ -300,10 +355,11 @@ int gather_routed(Src const& src, int64_t const* list, int64_t m, float* pts, hi
 //   route_order    orders the m list entries of n queries: *by_index = the entries in index order (list or tmp);
 //                  with perm, *perm[e] = the position in *by_index of the e-th entry in the Morton order of the
 //                  queries in the network's box (null when position and code would not fit 64 bits: index order);
-//                  tmp, keys, keys2: m uint64 of scratch each
+//                  tmp, keys, keys2: m uint64 of scratch each; sort_ws: radix_sort_words(m) uint32 of scratch
 template <class Src>
 int route_order(agwn_mlp const* mlp, Src const& src, int64_t n, int64_t* list, int64_t m, uint64_t* tmp,
-                uint64_t* keys, uint64_t* keys2, hipStream_t s, int64_t const** by_index, uint64_t const** perm);
+                uint64_t* keys, uint64_t* keys2, uint32_t* sort_ws, hipStream_t s, int64_t const** by_index,
+                uint64_t const** perm);
 // LSD radix sort of n uint64 keys over their low `bits` bits (agwn_boundary.hip); *sorted = a or b. The _ws form takes
 // its scratch (radix_sort_words(n) uint32) from the caller and only launches kernels (graph capture).
 int radix_sort_u64(uint64_t* a, uint64_t* b, int64_t n, int bits, hipStream_t s, uint64_t** sorted);
@@ -338,7 +394,7 @@ template <class Src>
 // only for callers that need the side of 0.5, not bit-stable values)
 int frac_f32_planned(Src const& src, int64_t n, frac_plan const& plan, float const* seg7, int64_t nseg,
                      double const x0[3], int32_t const* counts, float* out, hipStream_t stream,
-                     bool split_small = false);
+                     bool split_small = false, arena* ws = nullptr);
 
 inline int frac_f32(float const* pos, int64_t n, float const* seg7, int64_t nseg, double const x0[3],
                     int32_t const* counts, float* out, hipStream_t stream)

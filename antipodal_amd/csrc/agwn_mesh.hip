@@ -826,6 +826,11 @@ struct agwn_mesh
 {
     std::unique_ptr<agwn::mesh_impl> impl;
     mutable agwn::plan_cache boundary_plan; // FP32: the last boundary's chain plan (the boundary comes per call)
+    // point labelling's device workspace (agwn_label_surrogate_dev), reused between calls; the calls use it one at a
+    // time under label_mu and finish their device work before they return
+    mutable std::mutex label_mu;
+    mutable std::unique_ptr<agwn::arena> label_ws;
+    mutable int label_ws_dev = -1;
 };
 
 namespace agwn
@@ -1065,6 +1070,9 @@ struct agwn_grid
     void* label_list = nullptr;
     size_t label_list_bytes = 0;
     int label_list_dev = -1;
+    // the labelling calls' other device temporaries (routed points, sort keys and scratch, undecided lists, ...),
+    // reused the same way: per-call allocation and release cost about 3 ms per call at 512^3, in the releases
+    std::unique_ptr<agwn::arena> label_ws;
     ~agwn_grid()
     {
         if (seg7)
@@ -1237,30 +1245,28 @@ __global__ void __launch_bounds__(kB) k_permute_routed(uint64_t const* __restric
 // level (0: kDecideBeta, 1: kDecideBetaTight); the points it leaves open go to the next level, after the last to the
 // full boundary sum.
 int label_exact_levels(float const* pts, int32_t const* cnt, int64_t m, frac_plan const& pl, float const* seg7,
-                       int64_t nseg, double const x0d[3], int level, float* w, hipStream_t s)
+                       int64_t nseg, double const x0d[3], int level, float* w, hipStream_t s, arena& ws)
 {
     if (m == 0)
         return ok;
     if (level == 2) // labels only: small launches may split the stream
-        return frac_f32_planned(src_array<float>{pts}, m, pl, seg7, nseg, x0d, cnt, w, s, true);
-    dbuf und;
-    AGWN_TRY(und.alloc(size_t(m) * sizeof(int64_t), s));
+        return frac_f32_planned(src_array<float>{pts}, m, pl, seg7, nseg, x0d, cnt, w, s, true, &ws);
+    int64_t* und = nullptr;
+    AGWN_TRY(ws.take_n(size_t(m), &und));
     int64_t nu = 0;
-    AGWN_TRY(frac_decide(pts, m, cnt, pl, x0d, level == 0 ? kDecideBeta : kDecideBetaTight, w, und.as<int64_t>(), nu,
-                         s));
+    AGWN_TRY(frac_decide(pts, m, cnt, pl, x0d, level == 0 ? kDecideBeta : kDecideBetaTight, w, und, nu, s, &ws));
     if (nu == 0)
         return ok;
-    dbuf up, uc, uw;
-    AGWN_TRY(up.alloc(size_t(nu) * 3 * sizeof(float), s));
-    AGWN_TRY(uc.alloc(size_t(nu) * sizeof(int32_t), s));
-    AGWN_TRY(uw.alloc(size_t(nu) * sizeof(float), s));
+    float *up = nullptr, *uw = nullptr;
+    int32_t* uc = nullptr;
+    AGWN_TRY(ws.take_n(size_t(nu) * 3, &up));
+    AGWN_TRY(ws.take_n(size_t(nu), &uc));
+    AGWN_TRY(ws.take_n(size_t(nu), &uw));
     unsigned const gb = unsigned((nu + kB - 1) / kB);
-    k_gather_undecided<<<dim3(gb), dim3(kB), 0, s>>>(pts, cnt, und.as<int64_t>(), nu, up.as<float>(),
-                                                     uc.as<int32_t>());
+    k_gather_undecided<<<dim3(gb), dim3(kB), 0, s>>>(pts, cnt, und, nu, up, uc);
     AGWN_HIP(hipGetLastError());
-    AGWN_TRY(label_exact_levels(up.as<float>(), uc.as<int32_t>(), nu, pl, seg7, nseg, x0d, level + 1, uw.as<float>(),
-                                s));
-    k_scatter_values<<<dim3(gb), dim3(kB), 0, s>>>(und.as<int64_t>(), nu, uw.as<float>(), w);
+    AGWN_TRY(label_exact_levels(up, uc, nu, pl, seg7, nseg, x0d, level + 1, uw, s, ws));
+    k_scatter_values<<<dim3(gb), dim3(kB), 0, s>>>(und, nu, uw, w);
     AGWN_HIP(hipGetLastError());
     return ok;
 }
@@ -1274,7 +1280,7 @@ template <class Src>
 int label_surrogate(mesh_t<float>& mesh, agwn_mlp const* mlp, Src const& src, int64_t n, double qmax,
                     float const x0f[3], float const* seg7, int64_t nseg, frac_plan const* plan, float delta,
                     int64_t audit_every, uint64_t audit_seed, int8_t* labels, int64_t* stats, hipStream_t s,
-                    int64_t* list_ws = nullptr)
+                    arena& ws, int64_t* list_ws = nullptr)
 {
     AGWN_CHECK(mlp != nullptr, "mlp handle is null");
     AGWN_CHECK(delta >= 0.0f, "delta must be >= 0 (+inf routes every point)"); // rejects NaN
@@ -1284,19 +1290,19 @@ int label_surrogate(mesh_t<float>& mesh, agwn_mlp const* mlp, Src const& src, in
     unsigned long long cnt[5] = {0, 0, 0, 0, 0};
     if (n > 0)
     {
-        // the routed list has room for every query (list_ws: n entries of the caller's, else a buffer of this call)
-        dbuf dc, list_own;
-        AGWN_TRY(dc.alloc(sizeof(cnt), s));
+        // every device temporary comes from the caller's workspace, which the previous call (using it under the same
+        // lock) left with its device work finished; the routed list has room for every query (list_ws: n entries of
+        // the caller's, else from the workspace)
+        ws.reset();
+        unsigned long long* dc = nullptr;
+        AGWN_TRY(ws.take_n(5, &dc));
         int64_t* list = list_ws;
         if (!list)
-        {
-            AGWN_TRY(list_own.alloc(size_t(n) * sizeof(int64_t), s));
-            list = list_own.as<int64_t>();
-        }
-        AGWN_HIP(hipMemsetAsync(dc.p, 0, sizeof(cnt), s));
+            AGWN_TRY(ws.take_n(size_t(n), &list));
+        AGWN_HIP(hipMemsetAsync(dc, 0, sizeof(cnt), s));
         AGWN_TRY(mlp_route(mlp, src, n, delta, uint32_t(audit_every), uint32_t(audit_seed ^ (audit_seed >> 32)), labels,
-                           list, dc.as<unsigned long long>(), s));
-        AGWN_HIP(hipMemcpyAsync(cnt, dc.p, sizeof(cnt), hipMemcpyDeviceToHost, s));
+                           list, dc, s));
+        AGWN_HIP(hipMemcpyAsync(cnt, dc, sizeof(cnt), hipMemcpyDeviceToHost, s));
         AGWN_HIP(hipStreamSynchronize(s));
         int64_t const m = int64_t(cnt[3]);
         if (m > 0)
@@ -1304,38 +1310,41 @@ int label_surrogate(mesh_t<float>& mesh, agwn_mlp const* mlp, Src const& src, in
             // the list was appended tile by tile in completion order. The integer terms take it in index order
             // (on a lattice, a wave's rays then run along a few neighbouring rows: the packet traversal shares most
             // of their nodes); the decision takes it in Morton order (a wave's queries then need the same loops).
-            dbuf tmp, keys, keys2;
-            AGWN_TRY(tmp.alloc(size_t(m) * sizeof(uint64_t), s));
-            AGWN_TRY(keys.alloc(size_t(m) * sizeof(uint64_t), s));
-            AGWN_TRY(keys2.alloc(size_t(m) * sizeof(uint64_t), s));
+            uint64_t *tmp = nullptr, *keys = nullptr, *keys2 = nullptr;
+            uint32_t* sort_ws = nullptr;
+            AGWN_TRY(ws.take_n(size_t(m), &tmp));
+            AGWN_TRY(ws.take_n(size_t(m), &keys));
+            AGWN_TRY(ws.take_n(size_t(m), &keys2));
+            AGWN_TRY(ws.take_n(size_t(radix_sort_words(m)), &sort_ws));
             int64_t const* Li = nullptr;
             uint64_t const* perm = nullptr;
-            AGWN_TRY(route_order(mlp, src, n, list, m, tmp.as<uint64_t>(), keys.as<uint64_t>(),
-                                 keys2.as<uint64_t>(), s, &Li, &perm));
+            AGWN_TRY(route_order(mlp, src, n, list, m, tmp, keys, keys2, sort_ws, s, &Li, &perm));
             double const x0d[3] = {double(x0f[0]), double(x0f[1]), double(x0f[2])};
-            dbuf pts, c, w, pts_i, c_i, lm;
-            AGWN_TRY(pts.alloc(size_t(m) * 3 * sizeof(float), s));
-            AGWN_TRY(c.alloc(size_t(m) * sizeof(int32_t), s));
-            AGWN_TRY(w.alloc(size_t(m) * sizeof(float), s));
+            float *pts = nullptr, *w = nullptr;
+            int32_t* c = nullptr;
+            AGWN_TRY(ws.take_n(size_t(m) * 3, &pts));
+            AGWN_TRY(ws.take_n(size_t(m), &c));
+            AGWN_TRY(ws.take_n(size_t(m), &w));
             int64_t const* L = Li;
             if (perm)
             {
-                AGWN_TRY(pts_i.alloc(size_t(m) * 3 * sizeof(float), s));
-                AGWN_TRY(c_i.alloc(size_t(m) * sizeof(int32_t), s));
-                AGWN_TRY(lm.alloc(size_t(m) * sizeof(int64_t), s));
-                AGWN_TRY(gather_routed(src, Li, m, pts_i.as<float>(), s));
-                AGWN_TRY(
-                    mesh.count_src(src_array<float>{pts_i.as<float>()}, m, qmax, x0f, c_i.as<int32_t>(), s, true));
-                k_permute_routed<<<dim3(unsigned((m + kB - 1) / kB)), dim3(kB), 0, s>>>(
-                    perm, m, Li, pts_i.as<float>(), c_i.as<int32_t>(), lm.as<int64_t>(), pts.as<float>(),
-                    c.as<int32_t>());
+                float* pts_i = nullptr;
+                int32_t* c_i = nullptr;
+                int64_t* lm = nullptr;
+                AGWN_TRY(ws.take_n(size_t(m) * 3, &pts_i));
+                AGWN_TRY(ws.take_n(size_t(m), &c_i));
+                AGWN_TRY(ws.take_n(size_t(m), &lm));
+                AGWN_TRY(gather_routed(src, Li, m, pts_i, s));
+                AGWN_TRY(mesh.count_src(src_array<float>{pts_i}, m, qmax, x0f, c_i, s, true));
+                k_permute_routed<<<dim3(unsigned((m + kB - 1) / kB)), dim3(kB), 0, s>>>(perm, m, Li, pts_i, c_i,
+                                                                                        lm, pts, c);
                 AGWN_HIP(hipGetLastError());
-                L = lm.as<int64_t>();
+                L = lm;
             }
             else
             {
-                AGWN_TRY(gather_routed(src, Li, m, pts.as<float>(), s));
-                AGWN_TRY(mesh.count_src(src_array<float>{pts.as<float>()}, m, qmax, x0f, c.as<int32_t>(), s, true));
+                AGWN_TRY(gather_routed(src, Li, m, pts, s));
+                AGWN_TRY(mesh.count_src(src_array<float>{pts}, m, qmax, x0f, c, s, true));
             }
             frac_plan local;
             frac_plan const* pl = plan;
@@ -1355,14 +1364,12 @@ int label_surrogate(mesh_t<float>& mesh, agwn_mlp const* mlp, Src const& src, in
                 // far-field decision, then a tighter one over what it leaves; only the points neither decides take
                 // the full boundary sum. Each level gathers its open points (positions, counts) and scatters the
                 // values back.
-                AGWN_TRY(label_exact_levels(pts.as<float>(), c.as<int32_t>(), m, *pl, seg7, nseg, x0d, 0,
-                                            w.as<float>(), s));
+                AGWN_TRY(label_exact_levels(pts, c, m, *pl, seg7, nseg, x0d, 0, w, s, ws));
             }
             else
-                AGWN_TRY(frac_f32_planned(src_array<float>{pts.as<float>()}, m, *pl, seg7, nseg, x0d, c.as<int32_t>(),
-                                          w.as<float>(), s));
-            AGWN_TRY(route_apply(L, m, w.as<float>(), labels, dc.as<unsigned long long>(), s));
-            AGWN_HIP(hipMemcpyAsync(cnt, dc.p, sizeof(cnt), hipMemcpyDeviceToHost, s));
+                AGWN_TRY(frac_f32_planned(src_array<float>{pts}, m, *pl, seg7, nseg, x0d, c, w, s, false, &ws));
+            AGWN_TRY(route_apply(L, m, w, labels, dc, s));
+            AGWN_HIP(hipMemcpyAsync(cnt, dc, sizeof(cnt), hipMemcpyDeviceToHost, s));
             AGWN_HIP(hipStreamSynchronize(s));
         }
     }
@@ -1390,14 +1397,19 @@ int grid_label_dev(agwn_grid* g, agwn_mlp const* mlp, float const* start, float 
         g->label_list = nullptr;
         g->label_list_bytes = 0;
     }
+    if (g->label_list_dev != dev)
+        g->label_ws.reset(); // blocks of another device
+    if (!g->label_ws)
+        g->label_ws = std::make_unique<arena>();
     if (!g->label_list && need > 0)
     {
         AGWN_HIP(hipMalloc(&g->label_list, need));
         g->label_list_bytes = need;
         g->label_list_dev = dev;
     }
+    g->label_list_dev = dev;
     int const rc = label_surrogate(*g->mesh, mlp, src_grid{c.gd}, c.n, c.qmax, c.x0f, g->seg7, g->nseg,
-                                   g->plan.get(), delta, audit_every, audit_seed, labels, stats, s,
+                                   g->plan.get(), delta, audit_every, audit_seed, labels, stats, s, *g->label_ws,
                                    static_cast<int64_t*>(g->label_list));
     if (rc != ok)
         (void)hipStreamSynchronize(s); // a failed call may have left work queued on the list: drain it
@@ -1479,8 +1491,19 @@ int agwn_label_surrogate_dev(agwn_mesh const* m, agwn_mlp const* mlp, float cons
         std::shared_ptr<frac_plan const> pl;
         if (n > 0)
             AGWN_TRY(cached_plan(m->boundary_plan, seg7, nseg, st, pl));
-        return label_surrogate(mesh, mlp, src_array<float>{pos}, n, qmax, x0, seg7, nseg, pl.get(), delta,
-                               audit_every, audit_seed, labels, stats, st);
+        std::lock_guard<std::mutex> lk(m->label_mu);
+        int dev = 0;
+        AGWN_HIP(hipGetDevice(&dev));
+        if (!m->label_ws || m->label_ws_dev != dev)
+        {
+            m->label_ws = std::make_unique<arena>();
+            m->label_ws_dev = dev;
+        }
+        int const rc = label_surrogate(mesh, mlp, src_array<float>{pos}, n, qmax, x0, seg7, nseg, pl.get(), delta,
+                                       audit_every, audit_seed, labels, stats, st, *m->label_ws);
+        if (rc != ok)
+            (void)hipStreamSynchronize(st); // a failed call may have left work queued on the workspace: drain it
+        return rc;
     });
 }
 } // extern "C"

@@ -1450,7 +1450,8 @@ __global__ void __launch_bounds__(kB) k_strip_keys(uint64_t* __restrict__ keys, 
 
 template <class Src>
 int route_order(agwn_mlp const* h, Src const& src, int64_t n, int64_t* list, int64_t m, uint64_t* tmp,
-                uint64_t* keys, uint64_t* keys2, hipStream_t s, int64_t const** by_index, uint64_t const** perm)
+                uint64_t* keys, uint64_t* keys2, uint32_t* sort_ws, hipStream_t s, int64_t const** by_index,
+                uint64_t const** perm)
 {
     AGWN_CHECK(h != nullptr, "mlp handle is null");
     *by_index = list;
@@ -1462,7 +1463,7 @@ int route_order(agwn_mlp const* h, Src const& src, int64_t n, int64_t* list, int
     while (ebits < 63 && (uint64_t(2 * n) >> ebits) != 0)
         ++ebits;
     uint64_t* sorted = nullptr;
-    AGWN_TRY(radix_sort_u64(reinterpret_cast<uint64_t*>(list), tmp, m, ebits, s, &sorted));
+    AGWN_TRY(radix_sort_u64_ws(reinterpret_cast<uint64_t*>(list), tmp, m, ebits, sort_ws, s, &sorted));
     *by_index = reinterpret_cast<int64_t const*>(sorted);
     int pbits = 1;
     while (pbits < 63 && (uint64_t(m - 1) >> pbits) != 0)
@@ -1481,17 +1482,17 @@ int route_order(agwn_mlp const* h, Src const& src, int64_t n, int64_t* list, int
     k_route_keys<Src><<<dim3(g), dim3(kB), 0, s>>>(src, sorted, m, K, pbits, keys);
     AGWN_HIP(hipGetLastError());
     uint64_t* ks = nullptr;
-    AGWN_TRY(radix_sort_u64(keys, keys2, m, pbits + 30, s, &ks));
+    AGWN_TRY(radix_sort_u64_ws(keys, keys2, m, pbits + 30, sort_ws, s, &ks));
     k_strip_keys<<<dim3(g), dim3(kB), 0, s>>>(ks, m, (uint64_t(1) << pbits) - 1);
     AGWN_HIP(hipGetLastError());
     *perm = ks;
     return ok;
 }
 template int route_order<src_array<float>>(agwn_mlp const*, src_array<float> const&, int64_t, int64_t*, int64_t,
-                                           uint64_t*, uint64_t*, uint64_t*, hipStream_t, int64_t const**,
+                                           uint64_t*, uint64_t*, uint64_t*, uint32_t*, hipStream_t, int64_t const**,
                                            uint64_t const**);
 template int route_order<src_grid>(agwn_mlp const*, src_grid const&, int64_t, int64_t*, int64_t, uint64_t*, uint64_t*,
-                                   uint64_t*, hipStream_t, int64_t const**, uint64_t const**);
+                                   uint64_t*, uint32_t*, hipStream_t, int64_t const**, uint64_t const**);
 
 template <class Src>
 int gather_routed(Src const& src, int64_t const* list, int64_t m, float* pts, hipStream_t s)
