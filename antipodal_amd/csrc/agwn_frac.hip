@@ -1321,7 +1321,11 @@ int cached_plan(plan_cache& c, float const* seg7, int64_t nseg, hipStream_t s, s
     return ok;
 }
 
-// the decision margin covers the FP32 evaluation's own error (measured <= 4e-6 on 430k-segment boundaries)
+// The decision margin covers the FP32 evaluations' own errors: the decision's exact loop sums and the plain path's full
+// sum, whose label the decision must reproduce. It is empirical, not a derived bound: the FP32 full sum differs from
+// the FP64 one by at most 3.8e-6 over 1e7 queries of a 429,869-segment boundary (half of them 1e-3 off the surface)
+// and 3.4e-6 on the benchmark's torus, and a worst-case bound per pair summed over ~3e4 pairs would exceed 0.5. Points
+// within the margin (plus the far-field bound) of 0.5 go to the full sum, whose arithmetic is the plain path's.
 constexpr double kDecideMargin = 1e-4;
 
 // The decision's tables for one pole x0: the loop tree's records in the pole frame (device) and the decision stream

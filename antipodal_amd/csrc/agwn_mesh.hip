@@ -1275,7 +1275,9 @@ int label_exact_levels(float const* pts, int32_t const* cnt, int64_t m, frac_pla
 // mask, band and audit points are gathered and labelled exactly: their integer terms (the same kernel, x0 and query
 // coordinates as the plain evaluation), then the far-field decision (frac_decide) where the boundary is made of
 // closed loops, and the full FP32 boundary sum for the points it leaves open (or for all, otherwise). A decided
-// point's value is an estimate of its GWN proven to lie on the same side of 0.5 as the full evaluation.
+// point's value is an estimate of its GWN farther from 0.5 than the far-field remainder bound (rigorous) plus a margin
+// for the FP32 evaluations' own rounding (empirical: 1e-4, 26 times the largest FP32 error measured), so it lies on the
+// same side of 0.5 as the full evaluation.
 template <class Src>
 int label_surrogate(mesh_t<float>& mesh, agwn_mlp const* mlp, Src const& src, int64_t n, double qmax,
                     float const x0f[3], float const* seg7, int64_t nseg, frac_plan const* plan, float delta,
