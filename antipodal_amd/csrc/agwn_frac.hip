@@ -35,10 +35,12 @@
 // as 1.
 
 #include "agwn_internal.hpp"
+#include "agwn_math.hpp"
 
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 namespace agwn
@@ -127,33 +129,39 @@ struct pole
     double c;   // 1 - s^2
 };
 
-// One pair in FP64, cancellation-free denominator. v0 = a - p, v1 = b - p in the pole frame.
-__device__ __forceinline__ double pair_f64(double v0x, double v0y, double v0z, double v1x, double v1y, double v1z,
-                                           pole const& P)
+// One endpoint's l + s X (v = endpoint - query in the pole frame, l = |v|) in FP64 as a fraction lp = Pn / m with a
+// positive m and no division: X >= 0: Pn = l + X + (s - 1) X, m = 1; X < 0: l + X = (y^2 + z^2) / (l - X), so
+// m = l - X and Pn = y^2 + z^2 + (s - 1) X m.
+__device__ __forceinline__ void lplus_f64(double x, double y, double z, double sm1, double& Pn, double& m)
 {
-    double const yz0 = v0y * v0y + v0z * v0z;
-    double const yz1 = v1y * v1y + v1z * v1z;
-    double const l0 = sqrt(v0x * v0x + yz0);
-    double const l1 = sqrt(v1x * v1x + yz1);
-    double const lp0 = (v0x >= 0.0 ? l0 + v0x : yz0 / (l0 - v0x)) + P.sm1 * v0x;
-    double const lp1 = (v1x >= 0.0 ? l1 + v1x : yz1 / (l1 - v1x)) + P.sm1 * v1x;
-    double const num = P.s * (v0y * v1z - v0z * v1y);
-    double const den = lp0 * lp1 + (P.c * v0x * v1x + (v0y * v1y + v0z * v1z));
-    return atan2(num, den);
+    double const yz = __builtin_fma(y, y, z * z);
+    double const l = sqrt_f64(__builtin_fma(x, x, yz));
+    bool const pos = x >= 0.0;
+    m = pos ? 1.0 : l - x;
+    Pn = __builtin_fma(sm1 * x, m, pos ? l + x : yz);
 }
 
-// The same pair's (num, den) in FP64, for callers that combine two pairs' angles.
+// One pair's (num, den) in FP64, cancellation-free, both multiplied by the positive m0 m1 (the atan2 is unchanged):
+// num = s (y0 z1 - z0 y1) m0 m1, den = Pn0 Pn1 + m0 m1 ((1 - s^2) X0 X1 + y0 y1 + z0 z1). v0 = a - p, v1 = b - p in the
+// pole frame.
 __device__ __forceinline__ void pair_nd_f64(double v0x, double v0y, double v0z, double v1x, double v1y, double v1z,
                                             pole const& P, double& num, double& den)
 {
-    double const yz0 = v0y * v0y + v0z * v0z;
-    double const yz1 = v1y * v1y + v1z * v1z;
-    double const l0 = sqrt(v0x * v0x + yz0);
-    double const l1 = sqrt(v1x * v1x + yz1);
-    double const lp0 = (v0x >= 0.0 ? l0 + v0x : yz0 / (l0 - v0x)) + P.sm1 * v0x;
-    double const lp1 = (v1x >= 0.0 ? l1 + v1x : yz1 / (l1 - v1x)) + P.sm1 * v1x;
-    num = P.s * (v0y * v1z - v0z * v1y);
-    den = lp0 * lp1 + (P.c * v0x * v1x + (v0y * v1y + v0z * v1z));
+    double P0, m0, P1, m1;
+    lplus_f64(v0x, v0y, v0z, P.sm1, P0, m0);
+    lplus_f64(v1x, v1y, v1z, P.sm1, P1, m1);
+    double const w = m0 * m1;
+    num = (P.s * w) * __builtin_fma(v0y, v1z, -(v0z * v1y));
+    den = __builtin_fma(P0, P1, w * __builtin_fma(P.c * v0x, v1x, __builtin_fma(v0y, v1y, v0z * v1z)));
+}
+
+// One pair in FP64 (the angle).
+__device__ __forceinline__ double pair_f64(double v0x, double v0y, double v0z, double v1x, double v1y, double v1z,
+                                           pole const& P)
+{
+    double num, den;
+    pair_nd_f64(v0x, v0y, v0z, v1x, v1y, v1z, P, num, den);
+    return atan2_f64(num, den);
 }
 
 // atan2 for the FP32 kernel: octant reduction to r = min(|y|,|x|) / max(|y|,|x|) in [0, 1] and a degree-15 odd
@@ -840,11 +848,11 @@ __global__ void __launch_bounds__(kBlock)
         double const m0 = fmax(fabs(n0), d0), m1 = fmax(fabs(n1), d1);
         constexpr double lo = 0x1p-400, hi = 0x1p400;
         if (g0.w == g1.w && d0 > 0.0 && d1 > 0.0 && m0 >= lo && m0 <= hi && m1 >= lo && m1 <= hi)
-            add(g0.w * atan2(fma(n0, d1, n1 * d0), fma(d0, d1, -(n0 * n1))));
+            add(g0.w * atan2_f64(fma(n0, d1, n1 * d0), fma(d0, d1, -(n0 * n1))));
         else
         {
-            add(g0.w * atan2(n0, d0));
-            add(g1.w * atan2(n1, d1));
+            add(g0.w * atan2_f64(n0, d0));
+            add(g1.w * atan2_f64(n1, d1));
         }
     }
     if (s < nseg)
@@ -852,6 +860,113 @@ __global__ void __launch_bounds__(kBlock)
         seg_f64 const g = seg[s];
         add(g.w * pair_f64(g.ax - px, g.ay - py, g.az - pz, g.bx - px, g.by - py, g.bz - pz, P));
     }
+    if (i < n)
+    {
+        double r = (sum + comp) / kTwoPi;
+        if (counts)
+            r += double(counts[i]);
+        out[i] = r;
+    }
+}
+
+// The FP64 chain kernel's stream point in the pole frame: (X, y, z, flag), flag 1: a unit segment ends here.
+struct alignas(32) pt64
+{
+    double x, y, z, w;
+};
+
+// Rotates a double stream (x, y, z, flag) into the pole frame.
+__global__ void __launch_bounds__(kBlock) k_prep_stream64(pt64 const* __restrict__ in, int64_t n, frame R,
+                                                         pt64* __restrict__ out)
+{
+    int64_t const i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (i >= n)
+        return;
+    pt64 const g = in[i];
+    out[i] = pt64{R.r[0] * g.x + R.r[1] * g.y + R.r[2] * g.z, R.r[3] * g.x + R.r[4] * g.y + R.r[5] * g.z,
+                  R.r[6] * g.x + R.r[7] * g.y + R.r[8] * g.z, g.w};
+}
+
+// FP64 chain kernel (boundaries of small integer weights, expanded to unit segments and walked as chains): one lane
+// per query, the stream staged in LDS 256 points at a time (one 32-byte load per lane, then broadcast reads). Per
+// stream point a lane computes v = P - q, l + s X as the division-free fraction (Pn, m) and c X once, where the
+// per-segment kernel computes both endpoints of every segment; per unit segment, (num, den) times m0 m1 (see
+// pair_nd_f64). Two consecutive pairs whose den are positive and whose magnitudes lie in [2^-400, 2^400] share one
+// atan2 (the argument of the product of their complex numbers is the exact sum of two angles in (-pi/2, pi/2));
+// others take their own. Two-sum compensated accumulation.
+template <class Src>
+__global__ void __launch_bounds__(kBlock)
+    k_frac_f64_chain(Src const src, int64_t n, pt64 const* __restrict__ pts, int64_t npts, frame R, pole P,
+                     int32_t const* __restrict__ counts, double* __restrict__ out)
+{
+    __shared__ pt64 tile[kBlock];
+    int64_t const i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    double x = 0.0, y = 0.0, z = 0.0;
+    if (i < n)
+        query_at(src, i, x, y, z);
+    double const qx = R.r[0] * x + R.r[1] * y + R.r[2] * z;
+    double const qy = R.r[3] * x + R.r[4] * y + R.r[5] * z;
+    double const qz = R.r[6] * x + R.r[7] * y + R.r[8] * z;
+    double sum = 0.0, comp = 0.0;
+    auto add = [&](double v) {
+        double s, e;
+        two_sum_f64(sum, v, s, e);
+        sum = s;
+        comp += e;
+    };
+    double vx = 0.0, vy = 0.0, vz = 0.0, vP = 1.0, vm = 1.0; // the previous stream point
+    double pn = 0.0, pd = 1.0;                               // the pending pair
+    bool pok = false, pending = false;                       // pending is the same in every lane
+    constexpr double kLo = 0x1p-400, kHi = 0x1p400;
+    for (int64_t t0 = 0; t0 < npts; t0 += kBlock)
+    {
+        __syncthreads(); // the previous tile has been consumed by every wave
+        if (t0 + threadIdx.x < npts)
+            tile[threadIdx.x] = pts[t0 + threadIdx.x];
+        __syncthreads();
+        int const cnt = int(npts - t0 < kBlock ? npts - t0 : kBlock);
+        for (int j = 0; j < cnt; ++j)
+        {
+            pt64 const g = tile[j];
+            bool const seg = __builtin_amdgcn_readfirstlane(int(g.w)) != 0;
+            double const nx = g.x - qx, ny = g.y - qy, nz = g.z - qz;
+            double nP, nm;
+            lplus_f64(nx, ny, nz, P.sm1, nP, nm);
+            if (seg)
+            {
+                double const w = vm * nm;
+                double const num = (P.s * w) * __builtin_fma(vy, nz, -(vz * ny));
+                double const den =
+                    __builtin_fma(vP, nP, w * __builtin_fma(P.c * vx, nx, __builtin_fma(vy, ny, vz * nz)));
+                double const mag = __builtin_fmax(__builtin_fabs(num), __builtin_fabs(den));
+                bool const ok = den > 0.0 && mag >= kLo && mag <= kHi;
+                if (pending)
+                {
+                    if (ok && pok)
+                        add(atan2_f64(__builtin_fma(pn, den, pd * num), __builtin_fma(pd, den, -(pn * num))));
+                    else
+                    {
+                        add(atan2_f64(pn, pd));
+                        add(atan2_f64(num, den));
+                    }
+                }
+                else
+                {
+                    pn = num;
+                    pd = den;
+                    pok = ok;
+                }
+                pending = !pending;
+            }
+            vx = nx;
+            vy = ny;
+            vz = nz;
+            vP = nP;
+            vm = nm;
+        }
+    }
+    if (pending)
+        add(atan2_f64(pn, pd));
     if (i < n)
     {
         double r = (sum + comp) / kTwoPi;
@@ -917,39 +1032,41 @@ frame scale_frame(frame const& R, int e)
 // integers of magnitude <= 16 (0 drops the segment); otherwise returns false. Greedy walks in input order: start at
 // the first unused unit segment, follow any unused segment leaving the current end point (exact coordinate match),
 // stop when there is none. Every unit segment appears exactly once, as a flag-1 point after its start point.
-bool chain_stream(float const* h, int64_t nseg, std::vector<float4>& stream_pts, int64_t& nunit)
+// emit(p, flag) appends the stream point of coordinates p[0..3).
+template <class T, class Emit>
+bool chain_walk(T const* h, int64_t nseg, int64_t& nunit, Emit&& emit)
 {
-    stream_pts.clear();
     nunit = 0;
     int64_t total = 0;
     for (int64_t i = 0; i < nseg; ++i)
     {
-        float const w = h[7 * i + 6];
-        if (!(w == std::rint(w)) || std::fabs(w) > 16.0f)
+        T const w = h[7 * i + 6];
+        if (!(w == std::rint(w)) || std::fabs(w) > T(16))
             return false; // general weights: the per-segment kernel
         total += int64_t(std::fabs(w));
     }
+    using word = std::conditional_t<sizeof(T) == 4, uint32_t, uint64_t>;
     struct key
     {
-        uint32_t b[3];
+        word b[3];
         bool operator<(key const& o) const
         {
             return b[0] != o.b[0] ? b[0] < o.b[0] : (b[1] != o.b[1] ? b[1] < o.b[1] : b[2] < o.b[2]);
         }
         bool operator==(key const& o) const { return b[0] == o.b[0] && b[1] == o.b[1] && b[2] == o.b[2]; }
     };
-    auto key_of = [](float const* p) {
+    auto key_of = [](T const* p) {
         key k;
         std::memcpy(k.b, p, sizeof(k.b));
         return k;
     };
     // unit edges (from, to): pointers to the endpoint coordinates
-    std::vector<float const*> from, to;
+    std::vector<T const*> from, to;
     from.reserve(size_t(total));
     to.reserve(size_t(total));
     for (int64_t i = 0; i < nseg; ++i)
     {
-        float const* g = h + 7 * i;
+        T const* g = h + 7 * i;
         int const w = int(g[6]);
         for (int c = 0; c < (w < 0 ? -w : w); ++c)
         {
@@ -973,7 +1090,7 @@ bool chain_stream(float const* h, int64_t nseg, std::vector<float4>& stream_pts,
     for (int64_t r = 0; r < E; ++r)
         cursor[size_t(r)] = r;
     std::vector<char> used(static_cast<size_t>(E), 0);
-    auto next_from = [&](float const* p) -> int64_t {
+    auto next_from = [&](T const* p) -> int64_t {
         key const k = key_of(p);
         auto it = std::lower_bound(sorted_keys.begin(), sorted_keys.end(), k);
         if (it == sorted_keys.end() || !(*it == k))
@@ -986,24 +1103,31 @@ bool chain_stream(float const* h, int64_t nseg, std::vector<float4>& stream_pts,
             return order[size_t(c)];
         return -1;
     };
-    stream_pts.reserve(size_t(E + E / 8 + 16));
     for (int64_t e0 = 0; e0 < E; ++e0)
     {
         if (used[size_t(e0)])
             continue;
-        float const* a = from[size_t(e0)];
-        stream_pts.push_back(make_float4(a[0], a[1], a[2], 0.0f));
+        emit(from[size_t(e0)], 0);
         int64_t e = e0;
         while (e >= 0)
         {
             used[size_t(e)] = 1;
-            float const* b = to[size_t(e)];
-            stream_pts.push_back(make_float4(b[0], b[1], b[2], 1.0f));
+            T const* b = to[size_t(e)];
+            emit(b, 1);
             e = next_from(b);
         }
     }
     nunit = E;
     return true;
+}
+
+bool chain_stream(float const* h, int64_t nseg, std::vector<float4>& stream_pts, int64_t& nunit)
+{
+    stream_pts.clear();
+    stream_pts.reserve(size_t(nseg + nseg / 8 + 16));
+    return chain_walk(h, nseg, nunit, [&](float const* p, int flag) {
+        stream_pts.push_back(make_float4(p[0], p[1], p[2], float(flag)));
+    });
 }
 
 // Far-field data of one closed loop (points p[0..n), p[n-1] = p[0]): centroid of the distinct points, radius, vector
@@ -1631,10 +1755,44 @@ int frac_f64_ws(Src const& pos, int64_t n, double const* seg7, int64_t nseg, dou
 
 template <class Src>
 int frac_f64_src(Src const& pos, int64_t n, double const* seg7, int64_t nseg, double const x0[3],
-                 int32_t const* counts, double* out, hipStream_t stream)
+                 int32_t const* counts, double* out, hipStream_t stream, bool chain)
 {
     if (n == 0)
         return ok;
+    if (chain && nseg > 0)
+    {
+        // the boundary as a double chain stream when its weights are small integers (planned on the host per call)
+        std::vector<double> h(size_t(nseg) * 7);
+        AGWN_HIP(hipMemcpyAsync(h.data(), seg7, h.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+        AGWN_HIP(hipStreamSynchronize(stream));
+        std::vector<pt64> sp;
+        sp.reserve(size_t(nseg + nseg / 8 + 16));
+        int64_t nunit = 0;
+        if (chain_walk(h.data(), nseg, nunit,
+                       [&](double const* p, int flag) { sp.push_back(pt64{p[0], p[1], p[2], double(flag)}); }))
+        {
+            int64_t const npts = int64_t(sp.size());
+            int64_t const blocks = (n + kBlock - 1) / kBlock;
+            AGWN_CHECK(blocks < (int64_t(1) << 31), "frac_f64: too many queries (%lld)", (long long)n);
+            dbuf raw, rot;
+            AGWN_TRY(raw.alloc(size_t(std::max<int64_t>(npts, 1)) * sizeof(pt64), stream));
+            AGWN_TRY(rot.alloc(size_t(std::max<int64_t>(npts, 1)) * sizeof(pt64), stream));
+            AGWN_HIP(hipMemcpyAsync(raw.p, sp.data(), sp.size() * sizeof(pt64), hipMemcpyHostToDevice, stream));
+            frame const R = make_frame_impl(x0[0], x0[1], x0[2]);
+            pole const P = make_pole(x0[0], x0[1], x0[2]);
+            if (npts > 0)
+            {
+                k_prep_stream64<<<dim3(unsigned((npts + kBlock - 1) / kBlock)), dim3(kBlock), 0, stream>>>(
+                    raw.as<pt64>(), npts, R, rot.as<pt64>());
+                AGWN_HIP(hipGetLastError());
+            }
+            k_frac_f64_chain<Src><<<dim3(unsigned(blocks)), dim3(kBlock), 0, stream>>>(pos, n, rot.as<pt64>(), npts,
+                                                                                      R, P, counts, out);
+            AGWN_HIP(hipGetLastError());
+            AGWN_HIP(hipStreamSynchronize(stream)); // `sp` is a host vector
+            return ok;
+        }
+    }
     dbuf s64;
     AGWN_TRY(s64.alloc(size_t(std::max<int64_t>(nseg, 1)) * sizeof(seg_f64), stream));
     return frac_f64_ws(pos, n, seg7, nseg, x0, counts, out, s64.as<seg_f64>(), stream);
@@ -1691,7 +1849,7 @@ template int frac_f32_planned<src_grid>(src_grid const&, int64_t, frac_plan cons
 template int frac_f32_src<src_grid>(src_grid const&, int64_t, float const*, int64_t, double const*, int32_t const*,
                                     float*, hipStream_t);
 template int frac_f64_src<src_array<double>>(src_array<double> const&, int64_t, double const*, int64_t,
-                                             double const*, int32_t const*, double*, hipStream_t);
+                                             double const*, int32_t const*, double*, hipStream_t, bool);
 template int frac_f64_ws<src_array<double>>(src_array<double> const&, int64_t, double const*, int64_t, double const*,
                                             int32_t const*, double*, seg_f64*, hipStream_t);
 } // namespace agwn

@@ -332,9 +332,11 @@ int frac_decide(float const* pos, int64_t m, int32_t const* counts, frac_plan co
 template <class Src>
 int frac_f32_src(Src const& src, int64_t n, float const* seg7, int64_t nseg, double const x0[3],
                  int32_t const* counts, float* out, hipStream_t stream);
+// chain: a boundary of small integer weights is walked as a double chain stream (each stream point evaluated once,
+// paired arctangents: the full-GWN path); otherwise, and for general weights, segment by segment
 template <class Src>
 int frac_f64_src(Src const& src, int64_t n, double const* seg7, int64_t nseg, double const x0[3],
-                 int32_t const* counts, double* out, hipStream_t stream);
+                 int32_t const* counts, double* out, hipStream_t stream, bool chain = false);
 // The FP64 launcher with the caller's workspace: ws holds max(nseg, 1) seg_f64 (device), the boundary rotated into
 // the pole frame of x0. Launches two kernels on `stream`; allocates nothing. Instantiated for src_array<double>.
 template <class Src>
@@ -402,8 +404,8 @@ inline int frac_f32(float const* pos, int64_t n, float const* seg7, int64_t nseg
     return frac_f32_src(src_array<float>{pos}, n, seg7, nseg, x0, counts, out, stream);
 }
 inline int frac_f64(double const* pos, int64_t n, double const* seg7, int64_t nseg, double const x0[3],
-                    int32_t const* counts, double* out, hipStream_t stream)
+                    int32_t const* counts, double* out, hipStream_t stream, bool chain = false)
 {
-    return frac_f64_src(src_array<double>{pos}, n, seg7, nseg, x0, counts, out, stream);
+    return frac_f64_src(src_array<double>{pos}, n, seg7, nseg, x0, counts, out, stream, chain);
 }
 } // namespace agwn

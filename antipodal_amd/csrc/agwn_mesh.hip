@@ -931,7 +931,7 @@ int full_dev(agwn_mesh const* m, T const* pos, int64_t n, T const* x0, T const* 
         return frac_f32_planned(src_array<float>{pos}, n, *pl, seg7, nseg, x, cnt.as<int32_t>(), out, s);
     }
     else
-        return frac_f64(pos, n, seg7, nseg, x, cnt.as<int32_t>(), out, s);
+        return frac_f64(pos, n, seg7, nseg, x, cnt.as<int32_t>(), out, s, true);
 }
 
 template <class T>
