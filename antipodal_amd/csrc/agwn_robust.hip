@@ -17,6 +17,7 @@
 
 #include "../../include/agwn.h"
 #include "agwn_lbvh.hpp"
+#include "agwn_math.hpp"
 
 #include <algorithm>
 #include <cfloat>
@@ -415,6 +416,39 @@ __device__ inline double l_plus_x(v3<C> v, double l)
     return dot_yz(v, v) / (l - double(v.x));
 }
 
+// One segment's term, in the reference's operations and their order: (num, den) of its atan2 (the generic case,
+// num != 0: returns false), or the term itself in `special` (num == 0: the ray through an endpoint, grazing: returns
+// true). The reference's den cancels for a query next to the segment, so its rounding is kept as is.
+template <class C>
+__device__ inline bool rpair(v3<C> qi, rseg<C> const& g, double& num, double& den, double& special)
+{
+#pragma clang fp contract(off)
+    v3<C> const v0{C(g.p0.x - qi.x), C(g.p0.y - qi.y), C(g.p0.z - qi.z)};
+    v3<C> const v1{C(g.p1.x - qi.x), C(g.p1.y - qi.y), C(g.p1.z - qi.z)};
+    double const d0x = double(v0.x), d0y = double(v0.y), d0z = double(v0.z);
+    double const d1x = double(v1.x), d1y = double(v1.y), d1z = double(v1.z);
+    double const l0 = sqrt(d0x * d0x + d0y * d0y + d0z * d0z);
+    double const l1 = sqrt(d1x * d1x + d1y * d1y + d1z * d1z);
+    wide<C> const wn = atan2_num(qi, g.p0, g.p1);
+    den = l_plus_x(v0, l0) * l_plus_x(v1, l1) + dot_yz(v0, v1);
+    if (wn != 0)
+    {
+        num = to_d(wn);
+        return false;
+    }
+    if (v0.y == 0 && v0.z == 0 && v0.x < 0 && (v1.y != 0 || v1.z != 0))
+        special = v1.z != 0 ? atan2_f64(double(v1.z), -double(v1.y)) : (v1.y > 0 ? -kPi : 0.0);
+    else if (v1.y == 0 && v1.z == 0 && v1.x < 0 && (v0.y != 0 || v0.z != 0))
+        special = v0.z != 0 ? atan2_f64(-double(v0.z), -double(v0.y)) : (v0.y > 0 ? kPi : 0.0);
+    else
+        special = den < 0.0 ? double(-edge_sign(qi, g.p0, g.p1)) * kPi : 0.0;
+    return true;
+}
+
+// Segments two at a time: two generic terms of equal weight whose den are positive and whose magnitudes lie in
+// [2^-400, 2^400] share one atan2 (the argument of the product of their complex numbers is the exact sum of two
+// angles in (-pi/2, pi/2)); any other term takes its own. The arctangent is atan2_f64 (agwn_math.hpp), within 2 ulp
+// of the library's.
 template <class C, class T>
 __global__ void __launch_bounds__(kB)
     k_rfrac(T const* __restrict__ pos, int64_t n, quant Q, rseg<C> const* __restrict__ seg, int64_t nseg,
@@ -430,28 +464,30 @@ __global__ void __launch_bounds__(kB)
         z = double(pos[3 * i + 2]);
     }
     v3<C> const qi = quantize<C>(Q, x, y, z);
+    constexpr double kLo = 0x1p-400, kHi = 0x1p400;
+    auto single = [](bool sp, double nu, double de, double spv) { return sp ? spv : atan2_f64(nu, de); };
     double half_area = 0.0;
-    for (int64_t s = 0; s < nseg; ++s)
+    int64_t s = 0;
+    for (; s + 1 < nseg; s += 2)
     {
-        rseg<C> const g = seg[s];
-        v3<C> const v0{C(g.p0.x - qi.x), C(g.p0.y - qi.y), C(g.p0.z - qi.z)};
-        v3<C> const v1{C(g.p1.x - qi.x), C(g.p1.y - qi.y), C(g.p1.z - qi.z)};
-        double const d0x = double(v0.x), d0y = double(v0.y), d0z = double(v0.z);
-        double const d1x = double(v1.x), d1y = double(v1.y), d1z = double(v1.z);
-        double const l0 = sqrt(d0x * d0x + d0y * d0y + d0z * d0z);
-        double const l1 = sqrt(d1x * d1x + d1y * d1y + d1z * d1z);
-        wide<C> const num = atan2_num(qi, g.p0, g.p1);
-        double const den = l_plus_x(v0, l0) * l_plus_x(v1, l1) + dot_yz(v0, v1);
-        double contrib;
-        if (num != 0)
-            contrib = atan2(to_d(num), den);
-        else if (v0.y == 0 && v0.z == 0 && v0.x < 0 && (v1.y != 0 || v1.z != 0))
-            contrib = v1.z != 0 ? atan2(double(v1.z), -double(v1.y)) : (v1.y > 0 ? -kPi : 0.0);
-        else if (v1.y == 0 && v1.z == 0 && v1.x < 0 && (v0.y != 0 || v0.z != 0))
-            contrib = v0.z != 0 ? atan2(-double(v0.z), -double(v0.y)) : (v0.y > 0 ? kPi : 0.0);
+        rseg<C> const g0 = seg[s], g1 = seg[s + 1];
+        double n0 = 0.0, d0 = 0.0, s0 = 0.0, n1 = 0.0, d1 = 0.0, s1 = 0.0;
+        bool const sp0 = rpair(qi, g0, n0, d0, s0), sp1 = rpair(qi, g1, n1, d1, s1);
+        double const m0 = __builtin_fmax(__builtin_fabs(n0), __builtin_fabs(d0));
+        double const m1 = __builtin_fmax(__builtin_fabs(n1), __builtin_fabs(d1));
+        if (!sp0 && !sp1 && g0.w == g1.w && d0 > 0.0 && d1 > 0.0 && m0 >= kLo && m0 <= kHi && m1 >= kLo && m1 <= kHi)
+            half_area += g0.w * atan2_f64(__builtin_fma(n0, d1, n1 * d0), __builtin_fma(d0, d1, -(n0 * n1)));
         else
-            contrib = den < 0.0 ? double(-edge_sign(qi, g.p0, g.p1)) * kPi : 0.0;
-        half_area += g.w * contrib;
+        {
+            half_area += g0.w * single(sp0, n0, d0, s0);
+            half_area += g1.w * single(sp1, n1, d1, s1);
+        }
+    }
+    if (s < nseg)
+    {
+        double nu = 0.0, de = 0.0, spv = 0.0;
+        bool const sp = rpair(qi, seg[s], nu, de, spv);
+        half_area += seg[s].w * single(sp, nu, de, spv);
     }
     if (i < n)
     {
