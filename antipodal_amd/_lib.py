@@ -72,7 +72,10 @@ _SIG = {
     "agwn_batch_tune": (_int, [_p, _p, _p, _p, _p, _p]),
     "agwn_batch_eval_dev": (_int, [_p, _p, _p, _p, _p, _p]),
     "agwn_batch_info": (_int, [_p, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_int),
-                               C.POINTER(_int), _p]),
+                               C.POINTER(_int), _p, _p]),
+    "agwn_batch_set_tuning": (_int, [_p, _int, _int]),
+    "agwn_batch_frac_shape": (_int, [_int, C.POINTER(_int), C.POINTER(_int), C.POINTER(_int)]),
+    "agwn_batch_leaf_size": (_int, [_int, C.POINTER(_int)]),
     "agwn_mlp_num_params": (_i64, []),
     "agwn_mlp_create": (_int, [_p, _p, C.c_float, C.c_uint64, C.POINTER(_p)]),
     "agwn_mlp_destroy": (_int, [_p]),

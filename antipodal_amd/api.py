@@ -396,13 +396,44 @@ class HipMeshBatch:
                                       _stream_ptr(stream)))
         return out
 
+    # the tuner's search space (agwn.h: AGWN_BATCH_FRAC_SHAPES, AGWN_BATCH_LEAF_SIZES)
+    FRAC_SHAPES = 15
+    LEAF_SIZES = 5
+
+    @staticmethod
+    def frac_shape(v: int) -> tuple[int, int, int]:
+        """(queries per lane, workgroup size, stream points per LDS stage in workgroup sizes) of shape v."""
+        q, b, s = C.c_int(), C.c_int(), C.c_int()
+        check(lib.agwn_batch_frac_shape(int(v), C.byref(q), C.byref(b), C.byref(s)))
+        return q.value, b.value, s.value
+
+    @staticmethod
+    def leaf_size(l: int) -> int:
+        """Largest LBVH subtree (triangles) the counting traversal tests as one leaf bucket, for size index l."""
+        v = C.c_int()
+        check(lib.agwn_batch_leaf_size(int(l), C.byref(v)))
+        return v.value
+
+    def set_tuning(self, frac_shape: int, leaf: int) -> None:
+        """Uses fractional shape `frac_shape` and leaf-bucket size index `leaf` instead of tuning."""
+        check(lib.agwn_batch_set_tuning(self._h, int(frac_shape), int(leaf)))
+
     def info(self) -> dict:
         k, t, u = C.c_int64(), C.c_int64(), C.c_int64()
-        fq, fb = C.c_int(), C.c_int()
-        ms = np.zeros(8, dtype=np.float64)
-        check(lib.agwn_batch_info(self._h, C.byref(k), C.byref(t), C.byref(u), C.byref(fq), C.byref(fb), ptr(ms)))
-        return {"meshes": k.value, "triangles": t.value, "unit_segments": u.value, "queries_per_lane": fq.value,
-                "block": fb.value, "tune_ms": [float(x) for x in ms if x > 0]}
+        fv, lv = C.c_int(), C.c_int()
+        fms = np.zeros(self.FRAC_SHAPES, dtype=np.float64)
+        lms = np.zeros(self.LEAF_SIZES, dtype=np.float64)
+        check(lib.agwn_batch_info(self._h, C.byref(k), C.byref(t), C.byref(u), C.byref(fv), C.byref(lv), ptr(fms),
+                                  ptr(lms)))
+        q, b, s = self.frac_shape(fv.value) if fv.value >= 0 else (0, 0, 0)
+        return {"meshes": k.value, "triangles": t.value, "unit_segments": u.value, "frac_shape": fv.value,
+                "queries_per_lane": q, "block": b, "stage": s,
+                "leaf_size": self.leaf_size(lv.value) if lv.value >= 0 else 0,
+                "frac_table": [{"shape": v, "q": self.frac_shape(v)[0], "block": self.frac_shape(v)[1],
+                                "stage": self.frac_shape(v)[2], "ms": float(fms[v])} for v in range(self.FRAC_SHAPES)
+                               if fms[v] > 0],
+                "leaf_table": [{"leaf": self.leaf_size(l), "ms": float(lms[l])} for l in range(self.LEAF_SIZES)
+                               if lms[l] > 0]}
 
     def close(self) -> None:
         if getattr(self, "_h", None):

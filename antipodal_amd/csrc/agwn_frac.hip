@@ -34,6 +34,7 @@
 // so the product's argument is the exact sum of the two with no 2 pi wrap. Pairs that go to FP64 enter the product
 // as 1.
 
+#include "../../include/agwn.h"
 #include "agwn_internal.hpp"
 #include "agwn_math.hpp"
 
@@ -42,6 +43,7 @@
 #include <cstring>
 #include <type_traits>
 #include <vector>
+#include <utility>
 
 namespace agwn
 {
@@ -412,7 +414,8 @@ __global__ void __launch_bounds__(kBlock) k_prep_stream(float4 const* __restrict
 }
 
 // The chain kernel's work for one block of B lanes x Q queries: queries base + q*B (lane's share), valid below n;
-// the point stream pts[0, 2 npts) (pole frame, hi/lo float4 per point) staged through `tile` (2B float4 of LDS).
+// the point stream pts[0, 2 npts) (pole frame, hi/lo float4 per point) staged S B points at a time through `tile`
+// (2 S B float4 of LDS: more points per barrier, fewer resident blocks).
 // Per stream point a lane keeps v = P - q (X, y, z), lp = l + s X and |y| + |z|; per unit segment
 //   num = s (y0 z1 - z0 y1),  den = lp0 lp1 + (1 - s^2) X0 X1 + (y0 y1 + z0 z1)
 // -- the reference's den, rearranged exactly so that no term cancels when an endpoint lies along the pole. A pair is
@@ -420,7 +423,7 @@ __global__ void __launch_bounds__(kBlock) k_prep_stream(float4 const* __restrict
 // num and den: the query is near the segment, or the segment passes near the pole direction) and goes to FP64 (the
 // coordinates hi + lo, exact in double); so do pairs with den <= 0, where atan2 sits on its branch cut and only an
 // accurate num gives the sign.
-template <int Q, int B, class Src>
+template <int Q, int B, int S = 1, class Src>
 __device__ __forceinline__ void chain_block(Src const& src, int64_t base, int64_t n, float4 const* __restrict__ pts,
                                             int64_t npts, frame const& R, pole const& P, float tau,
                                             float4 const& far, int32_t const* __restrict__ counts,
@@ -445,16 +448,16 @@ __device__ __forceinline__ void chain_block(Src const& src, int64_t base, int64_
     bool pending = false; // uniform: the segment structure is the same for every lane
     int64_t const nf = 2 * npts;
 
-    for (int64_t t0 = 0; t0 < npts; t0 += B)
+    for (int64_t t0 = 0; t0 < npts; t0 += S * B)
     {
         __syncthreads(); // the previous tile has been consumed by every wave
         int64_t const f0 = 2 * t0;
-        if (f0 + threadIdx.x < nf)
-            tile[threadIdx.x] = pts[f0 + threadIdx.x];
-        if (f0 + B + threadIdx.x < nf)
-            tile[B + threadIdx.x] = pts[f0 + B + threadIdx.x];
+#pragma unroll
+        for (int r = 0; r < 2 * S; ++r)
+            if (f0 + r * B + threadIdx.x < nf)
+                tile[r * B + threadIdx.x] = pts[f0 + r * B + threadIdx.x];
         __syncthreads();
-        int const cnt = int(npts - t0 < B ? npts - t0 : B);
+        int const cnt = int(npts - t0 < S * B ? npts - t0 : S * B);
         // one stream point; the loop below takes two per iteration, so that the previous point's registers are
         // renamed rather than copied
         auto step = [&](int const j) __attribute__((always_inline)) {
@@ -595,31 +598,32 @@ __global__ void __launch_bounds__(kBlock) k_chain_sum(double const* __restrict__
 
 // Persistent chain kernel over a batch of meshes: workgroups take work items (mesh, first query of a B x Q tile)
 // from an atomic counter until none are left; each mesh has its own point stream and query range, and counts (the
-// integer term) are added per query. Every workgroup ends at the same test.
-template <int Q, int B>
+// integer term) are added per query. Every workgroup ends at the same test. S B stream points per LDS stage.
+template <int Q, int B, int S>
 __global__ void __launch_bounds__(B)
     k_frac_f32_chain_batch(float const* __restrict__ pos, batch_frac_desc const* __restrict__ meshes,
                            int32_t const* __restrict__ item_mesh, int64_t const* __restrict__ item_q0, int64_t nitems,
                            unsigned long long* __restrict__ counter, float4 const* __restrict__ pts, frame R, pole P,
                            float tau, int32_t const* __restrict__ counts, float* __restrict__ out)
 {
-    __shared__ float4 tile[2 * B];
+    __shared__ float4 tile[2 * S * B];
     __shared__ unsigned long long cur;
     while (true)
     {
         if (threadIdx.x == 0)
             cur = atomicAdd(counter, 1ull);
         __syncthreads();
-        int64_t const it = int64_t(cur);
+        unsigned long long const ticket = cur;
         __syncthreads();
-        if (it >= nitems)
+        if (ticket >= static_cast<unsigned long long>(nitems)) // unsigned: any counter value past the list ends
             break;
+        int64_t const it = int64_t(ticket);
         batch_frac_desc const D = meshes[item_mesh[it]];
         frame Rm; // the mesh's scaled frame: its stream was rotated and scaled by the same 2^-e (k_prep_stream)
 #pragma unroll
         for (int k = 0; k < 9; ++k)
             Rm.r[k] = R.r[k] * D.qscale;
-        chain_block<Q, B>(src_array<float>{pos}, item_q0[it] + threadIdx.x, D.q_end, pts + 2 * D.pt0, D.npts, Rm, P,
+        chain_block<Q, B, S>(src_array<float>{pos}, item_q0[it] + threadIdx.x, D.q_end, pts + 2 * D.pt0, D.npts, Rm, P,
                           tau, D.far, counts, out, nullptr, tile);
     }
 }
@@ -1798,15 +1802,27 @@ int frac_f64_src(Src const& pos, int64_t n, double const* seg7, int64_t nseg, do
     return frac_f64_ws(pos, n, seg7, nseg, x0, counts, out, s64.as<seg_f64>(), stream);
 }
 
-// (queries per lane, block) shapes of the batched kernel offered to the autotuner
-constexpr int kBatchShapes[][2] = {{1, 256}, {2, 256}, {4, 256}, {2, 128}, {4, 128}};
+// (queries per lane, block, LDS stage in blocks of points) shapes of the batched kernel offered to the autotuner
+constexpr int kBatchShapes[][3] = {{1, 256, 1}, {2, 256, 1}, {4, 256, 1}, {2, 128, 1}, {4, 128, 1},
+                                   {1, 256, 2}, {2, 256, 2}, {4, 256, 2}, {2, 128, 2}, {4, 128, 2},
+                                   {1, 256, 4}, {2, 256, 4}, {4, 256, 4}, {2, 128, 4}, {4, 128, 4}};
+constexpr int kNumBatchShapes = int(sizeof(kBatchShapes) / sizeof(kBatchShapes[0]));
+static_assert(kNumBatchShapes == AGWN_BATCH_FRAC_SHAPES && kNumBatchShapes <= 32, "the header's count of shapes");
 
-int frac_batch_variants() { return int(sizeof(kBatchShapes) / sizeof(kBatchShapes[0])); }
+int frac_batch_variants() { return kNumBatchShapes; }
 
-void frac_batch_shape(int v, int& q, int& block)
+void frac_batch_shape(int v, int& q, int& block, int& stage)
 {
     q = kBatchShapes[v][0];
     block = kBatchShapes[v][1];
+    stage = kBatchShapes[v][2];
+}
+
+int frac_batch_blocks_per_cu(int v)
+{
+    int const b = kBatchShapes[v][1], lds = 32 * kBatchShapes[v][2] * b + 64; // tile + the work counter
+    int const waves = b == 256 ? 8 : 16; // 32 waves per CU at most
+    return std::max(1, std::min(waves, (160 * 1024) / lds));
 }
 
 int frac_stream_rotate(float4 const* in, int64_t n, double const x0[3], int16_t const* pexp, float4* out,
@@ -1820,21 +1836,46 @@ int frac_stream_rotate(float4 const* in, int64_t n, double const x0[3], int16_t 
     return ok;
 }
 
+namespace
+{
+struct batch_args
+{
+    float const* pos;
+    batch_frac_desc const* meshes;
+    int32_t const* item_mesh;
+    int64_t const* item_q0;
+    int64_t nitems;
+    unsigned long long* counter;
+    float4 const* pts;
+    frame R;
+    pole P;
+    int32_t const* counts;
+    float* out;
+};
+
+template <int V>
+void batch_launch_shape(dim3 g, hipStream_t s, batch_args const& a)
+{
+    constexpr int Q = kBatchShapes[V][0], B = kBatchShapes[V][1], S = kBatchShapes[V][2];
+    k_frac_f32_chain_batch<Q, B, S><<<g, dim3(B), 0, s>>>(a.pos, a.meshes, a.item_mesh, a.item_q0, a.nitems,
+                                                          a.counter, a.pts, a.R, a.P, kTau, a.counts, a.out);
+}
+
+template <int... V>
+void batch_dispatch(std::integer_sequence<int, V...>, int v, dim3 g, hipStream_t s, batch_args const& a)
+{
+    ((v == V ? batch_launch_shape<V>(g, s, a) : void()), ...);
+}
+} // namespace
+
 int frac_batch_launch(int v, int blocks, float const* pos, batch_frac_desc const* meshes, int32_t const* item_mesh,
                       int64_t const* item_q0, int64_t nitems, unsigned long long* counter, float4 const* pts,
                       double const x0[3], int32_t const* counts, float* out, hipStream_t s)
 {
     AGWN_CHECK(v >= 0 && v < frac_batch_variants(), "unknown batch kernel shape %d", v);
-    frame const R = make_frame_impl(x0[0], x0[1], x0[2]);
-    pole const P = make_pole(x0[0], x0[1], x0[2]);
-    dim3 const g = dim3(static_cast<unsigned>(blocks));
-#define AGWN_BATCH_CASE(QQ, BB)                                                                                        \
-    if (kBatchShapes[v][0] == QQ && kBatchShapes[v][1] == BB)                                                          \
-        k_frac_f32_chain_batch<QQ, BB><<<g, dim3(BB), 0, s>>>(pos, meshes, item_mesh, item_q0, nitems, counter,        \
-                                                              pts, R, P, kTau, counts, out);
-    AGWN_BATCH_CASE(1, 256)
-    else AGWN_BATCH_CASE(2, 256) else AGWN_BATCH_CASE(4, 256) else AGWN_BATCH_CASE(2, 128) else AGWN_BATCH_CASE(4, 128)
-#undef AGWN_BATCH_CASE
+    batch_args const a{pos, meshes, item_mesh, item_q0, nitems, counter, pts, make_frame_impl(x0[0], x0[1], x0[2]),
+                       make_pole(x0[0], x0[1], x0[2]), counts, out};
+    batch_dispatch(std::make_integer_sequence<int, kNumBatchShapes>{}, v, dim3(static_cast<unsigned>(blocks)), s, a);
     AGWN_HIP(hipGetLastError());
     return ok;
 }

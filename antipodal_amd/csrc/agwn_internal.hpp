@@ -81,8 +81,14 @@ struct dbuf
         if (p)
             (void)hipFreeAsync(p, s);
     }
+    // (re)allocates; a previous allocation is released first (its last use must be complete)
     int alloc(size_t bytes, hipStream_t stream)
     {
+        if (p)
+        {
+            (void)hipFreeAsync(p, s);
+            p = nullptr;
+        }
         s = stream;
         if (bytes == 0)
             bytes = 256;
@@ -379,8 +385,10 @@ struct batch_frac_desc
     float4 far;        // the boundary's far-query test (boundary_scale_exp)
 };
 bool chain_stream(float const* seg7_host, int64_t nseg, std::vector<float4>& stream_pts, int64_t& nunit);
-int frac_batch_variants();                         // number of (queries per lane, block) shapes
-void frac_batch_shape(int variant, int& q, int& block);
+int frac_batch_variants(); // number of (queries per lane, block, LDS stage) shapes
+// shape `variant`: q queries per lane, `block` lanes, stage x block stream points per LDS stage
+void frac_batch_shape(int variant, int& q, int& block, int& stage);
+int frac_batch_blocks_per_cu(int variant); // resident workgroups per CU (wave and LDS limits)
 // rotates the concatenated stream into the pole frame of x0, each point scaled by 2^-pexp[i] (its mesh's exponent;
 // device int16 per point): two float4 per point, (hi, flag) and (lo, 0)
 int frac_stream_rotate(float4 const* in, int64_t n, double const x0[3], int16_t const* pexp, float4* out,

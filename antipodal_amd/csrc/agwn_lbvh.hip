@@ -205,7 +205,8 @@ __device__ __forceinline__ int delta(uint32_t const* __restrict__ k, int64_t n, 
 }
 
 __global__ void __launch_bounds__(kB) k_karras(uint32_t const* __restrict__ k, int64_t n, int32_t* __restrict__ child,
-                                               int32_t* __restrict__ parent, int8_t* __restrict__ big)
+                                               int32_t* __restrict__ parent, int8_t* __restrict__ big,
+                                               uint32_t* __restrict__ csz)
 {
     int64_t const i = int64_t(blockIdx.x) * kB + threadIdx.x;
     if (i >= n - 1)
@@ -236,6 +237,7 @@ __global__ void __launch_bounds__(kB) k_karras(uint32_t const* __restrict__ k, i
     child[2 * i] = left;
     child[2 * i + 1] = right;
     big[i] = (hi - g) > (g - lo + 1) ? 1 : 0; // leaves right: [g+1, hi], left: [lo, g]
+    csz[i] = uint32_t(g - lo + 1 < 255 ? g - lo + 1 : 255) | (uint32_t(hi - g < 255 ? hi - g : 255) << 8);
     parent[left >= 0 ? left : (n - 1) + ~left] = int32_t(i);
     parent[right >= 0 ? right : (n - 1) + ~right] = int32_t(i);
 }
@@ -268,7 +270,7 @@ __device__ __forceinline__ void ld_box(B const* lo, B const* hi, int64_t i, B (&
 template <class B>
 __global__ void __launch_bounds__(kB)
     k_refit(int32_t const* __restrict__ child, int32_t const* __restrict__ parent, int8_t const* __restrict__ big,
-            int64_t n, B const* leaf_lo, B const* leaf_hi, B* node_lo, B* node_hi, bvh_node<B>* __restrict__ nodes,
+            uint32_t const* __restrict__ csz, int64_t n, B const* leaf_lo, B const* leaf_hi, B* node_lo, B* node_hi, bvh_node<B>* __restrict__ nodes,
             int32_t* flags)
 {
     int64_t const k = int64_t(blockIdx.x) * kB + threadIdx.x;
@@ -305,6 +307,7 @@ __global__ void __launch_bounds__(kB)
         nd.c0 = c[0];
         nd.c1 = c[1];
         nd.big = big[p];
+        nd.sz = csz[p];
         nodes[p] = nd;
         p = parent[p];
     }
@@ -323,6 +326,7 @@ int lbvh_build_topology(float const* centroids, int64_t n, lbvh_topology& topo, 
     AGWN_HIP(hipMalloc(&topo.parent, size_t(2 * n) * sizeof(int32_t)));
     AGWN_HIP(hipMalloc(&topo.child, size_t(std::max<int64_t>(2 * (n - 1), 1)) * sizeof(int32_t)));
     AGWN_HIP(hipMalloc(&topo.big, size_t(std::max<int64_t>(n - 1, 1)) * sizeof(int8_t)));
+    AGWN_HIP(hipMalloc(&topo.csz, size_t(std::max<int64_t>(n - 1, 1)) * sizeof(uint32_t)));
     if (n == 1)
     {
         int32_t const zero = 0, none = -1;
@@ -364,7 +368,8 @@ int lbvh_build_topology(float const* centroids, int64_t n, lbvh_topology& topo, 
     }
     // after 4 passes the sorted data is back in (k0, order)
     AGWN_HIP(hipMemsetAsync(topo.parent, 0xff, size_t(2 * n) * sizeof(int32_t), s));
-    k_karras<<<dim3(unsigned((n - 1 + kB - 1) / kB)), dim3(kB), 0, s>>>(ka, n, topo.child, topo.parent, topo.big);
+    k_karras<<<dim3(unsigned((n - 1 + kB - 1) / kB)), dim3(kB), 0, s>>>(ka, n, topo.child, topo.parent, topo.big,
+                                                                                topo.csz);
     AGWN_HIP(hipGetLastError());
     AGWN_HIP(hipMemsetAsync(md.p, 0, sizeof(int), s));
     k_depth<<<dim3(g), dim3(kB), 0, s>>>(topo.parent, n, md.as<int>());
@@ -386,6 +391,8 @@ void lbvh_release(lbvh_topology& topo)
         (void)hipFree(topo.parent);
     if (topo.big)
         (void)hipFree(topo.big);
+    if (topo.csz)
+        (void)hipFree(topo.csz);
     topo = lbvh_topology{};
 }
 
@@ -397,8 +404,8 @@ int lbvh_refit(lbvh_topology const& topo, B const* leaf_lo, B const* leaf_hi, bv
     if (n <= 1)
         return ok;
     AGWN_HIP(hipMemsetAsync(flags, 0, size_t(n - 1) * sizeof(int32_t), s));
-    k_refit<B><<<dim3(unsigned((n + kB - 1) / kB)), dim3(kB), 0, s>>>(topo.child, topo.parent, topo.big, n, leaf_lo,
-                                                                     leaf_hi, node_lo, node_hi, nodes, flags);
+    k_refit<B><<<dim3(unsigned((n + kB - 1) / kB)), dim3(kB), 0, s>>>(topo.child, topo.parent, topo.big, topo.csz, n,
+                                                                     leaf_lo, leaf_hi, node_lo, node_hi, nodes, flags);
     AGWN_HIP(hipGetLastError());
     return ok;
 }

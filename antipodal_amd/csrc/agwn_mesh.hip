@@ -228,13 +228,24 @@ __device__ __forceinline__ float inv_dir(double d)
 
 // Signed crossing count of one ray (p, d) against a prepared mesh: LBVH short-stack traversal (stack column `lane` of
 // an LDS array [kMaxStack][kTB]) with the Moeller-Trumbore replica at the leaves, plus the exhaustively tested
-// triangles. The batched kernel's traversal: its queries are scattered, where a wave's rays share few nodes.
-template <class T>
+// triangles. The batched kernel's traversal: its queries are scattered, where a wave's rays share few nodes. With
+// Buckets, internal children of at most L leaves are taken as leaf buckets (bucket_span): their leaves are tested
+// without their own boxes. A leaf box only ever rejects triangles the test would not count (it holds every point where
+// the replica can report a crossing), so the counts do not depend on L.
+template <bool Buckets, class T>
 __device__ __forceinline__ int count_one(T px, T py, T pz, T dx, T dy, T dz, bvh_node<float> const* __restrict__ nodes,
                                          leaf_rec<T> const* __restrict__ rec, int32_t const* __restrict__ brute,
-                                         int32_t nb, int use_bvh, int32_t (*stack)[kTB], int lane)
+                                         int32_t nb, int use_bvh, int L, int32_t (*stack)[kTB], int lane)
 {
     int sum = 0;
+    auto leaves = [&](int32_t first, int32_t count) __attribute__((always_inline)) {
+        for (int32_t k = first; k < first + count; ++k)
+        {
+            leaf_rec<T> const r = rec[k];
+            if (r.live)
+                sum += mt_test(r, px, py, pz, dx, dy, dz);
+        }
+    };
     if (use_bvh)
     {
         float const ox = float(px), oy = float(py), oz = float(pz);
@@ -248,14 +259,11 @@ __device__ __forceinline__ int count_one(T px, T py, T pz, T dx, T dy, T dz, bvh
             bool const h1 = slab(nd.lo1, nd.hi1, ox, oy, oz, ix, iy, iz);
             int32_t next = -1;
             bool have = false;
+            int32_t first = 0, count = 0;
             if (h0)
             {
-                if (nd.c0 < 0)
-                {
-                    leaf_rec<T> const r = rec[~nd.c0];
-                    if (r.live)
-                        sum += mt_test(r, px, py, pz, dx, dy, dz);
-                }
+                if (bucket_span<Buckets>(nd.c0, nd.sz & 0xFFu, false, L, first, count))
+                    leaves(first, count);
                 else
                 {
                     next = nd.c0;
@@ -264,12 +272,8 @@ __device__ __forceinline__ int count_one(T px, T py, T pz, T dx, T dy, T dz, bvh
             }
             if (h1)
             {
-                if (nd.c1 < 0)
-                {
-                    leaf_rec<T> const r = rec[~nd.c1];
-                    if (r.live)
-                        sum += mt_test(r, px, py, pz, dx, dy, dz);
-                }
+                if (bucket_span<Buckets>(nd.c1, nd.sz >> 8, true, L, first, count))
+                    leaves(first, count);
                 else if (have)
                 {
                     // both children internal: descend into the one with fewer leaves, push the other (the stack
@@ -303,19 +307,28 @@ __device__ __forceinline__ int count_one(T px, T py, T pz, T dx, T dy, T dz, bvh
 }
 
 // The same count for a wave of rays at once (packet traversal): the wave walks the LBVH together, carrying the mask
-// of lanes whose rays reach the current node; a node's two boxes are tested by those lanes, a leaf's triangle by the
-// lanes whose rays hit its box. Every lane thus tests exactly the triangles the per-lane traversal would, with the
-// same arithmetic, so the counts are the same; nodes are read once per wave (scalar loads) and the control flow is
-// uniform, which pays when a wave's rays are close and parallel (lattice rows, spatially ordered queries). The stack
-// (node, mask) lives in the wave's LDS slots; the larger subtree is pushed, as in count_one, so it stays within
-// kMaxStack.
-template <class T>
+// of lanes whose rays reach the current node; a node's two boxes are tested by those lanes, a leaf's triangle (a
+// bucket's triangles) by the lanes whose rays hit its box. Every lane thus tests exactly the triangles the per-lane
+// traversal would, with the same arithmetic, so the counts are the same; nodes are read once per wave (scalar loads)
+// and the control flow is uniform, which pays when a wave's rays are close and parallel (lattice rows, spatially
+// ordered queries). The stack (node, mask) lives in the wave's LDS slots; the larger subtree is pushed, as in
+// count_one, so it stays within kMaxStack.
+template <bool Buckets, class T>
 __device__ __forceinline__ int count_packet(T px, T py, T pz, bool valid, T dx, T dy, T dz,
                                             bvh_node<float> const* __restrict__ nodes,
                                             leaf_rec<T> const* __restrict__ rec, int32_t const* __restrict__ brute,
-                                            int32_t nb, int use_bvh, int32_t* stk_node, uint64_t* stk_mask, int lane)
+                                            int32_t nb, int use_bvh, int L, int32_t* stk_node, uint64_t* stk_mask,
+                                            int lane)
 {
     int sum = 0;
+    auto leaves = [&](int32_t first, int32_t count) __attribute__((always_inline)) {
+        for (int32_t k = first; k < first + count; ++k)
+        {
+            leaf_rec<T> const r = rec[k];
+            if (r.live)
+                sum += mt_test(r, px, py, pz, dx, dy, dz);
+        }
+    };
     if (use_bvh)
     {
         float const ox = float(px), oy = float(py), oz = float(pz);
@@ -332,16 +345,13 @@ __device__ __forceinline__ int count_packet(T px, T py, T pz, bool valid, T dx, 
             uint64_t const m0 = __ballot(h0), m1 = __ballot(h1);
             int32_t next = -1;
             uint64_t nm = 0;
+            int32_t first = 0, count = 0;
             if (m0)
             {
-                if (nd.c0 < 0)
+                if (bucket_span<Buckets>(nd.c0, nd.sz & 0xFFu, false, L, first, count))
                 {
                     if (h0)
-                    {
-                        leaf_rec<T> const r = rec[~nd.c0];
-                        if (r.live)
-                            sum += mt_test(r, px, py, pz, dx, dy, dz);
-                    }
+                        leaves(first, count);
                 }
                 else
                 {
@@ -351,14 +361,10 @@ __device__ __forceinline__ int count_packet(T px, T py, T pz, bool valid, T dx, 
             }
             if (m1)
             {
-                if (nd.c1 < 0)
+                if (bucket_span<Buckets>(nd.c1, nd.sz >> 8, true, L, first, count))
                 {
                     if (h1)
-                    {
-                        leaf_rec<T> const r = rec[~nd.c1];
-                        if (r.live)
-                            sum += mt_test(r, px, py, pz, dx, dy, dz);
-                    }
+                        leaves(first, count);
                 }
                 else if (next >= 0)
                 {
@@ -469,8 +475,8 @@ __global__ void __launch_bounds__(kTB)
         if (valid)
             query_at(src, i, px, py, pz);
         int const w = int(threadIdx.x >> 6);
-        int const c = count_packet(px, py, pz, valid, dx, dy, dz, nodes, rec, brute, *nbrute_p, use_bvh, stk_node[w],
-                                   stk_mask[w], int(threadIdx.x & 63));
+        int const c = count_packet<false>(px, py, pz, valid, dx, dy, dz, nodes, rec, brute, *nbrute_p, use_bvh, 1,
+                                          stk_node[w], stk_mask[w], int(threadIdx.x & 63));
         if (valid)
             out[i] = c;
     }
@@ -481,7 +487,8 @@ __global__ void __launch_bounds__(kTB)
             return;
         T px, py, pz;
         query_at(src, i, px, py, pz);
-        out[i] = count_one(px, py, pz, dx, dy, dz, nodes, rec, brute, *nbrute_p, use_bvh, stack, int(threadIdx.x));
+        out[i] = count_one<false>(px, py, pz, dx, dy, dz, nodes, rec, brute, *nbrute_p, use_bvh, 1, stack,
+                                  int(threadIdx.x));
     }
 }
 
@@ -499,11 +506,15 @@ struct batch_mesh_desc
 };
 
 // Persistent counting kernel over a batch: workgroups take work items (mesh, first query; kTB queries at most, not
-// past the mesh's end) from an atomic counter until none are left -- every workgroup ends at the same test.
+// past the mesh's end) from an atomic counter until none are left -- every workgroup ends at the same test. Buckets:
+// internal children of at most L leaves are tested as leaf buckets (L is tuned per batch; the counts do not depend
+// on it).
+template <bool Buckets>
 __global__ void __launch_bounds__(kTB)
     k_count_batch(float const* __restrict__ pos, batch_mesh_desc const* __restrict__ meshes,
                   int32_t const* __restrict__ item_mesh, int64_t const* __restrict__ item_q0, int64_t nitems,
-                  unsigned long long* __restrict__ counter, float dx, float dy, float dz, int32_t* __restrict__ out)
+                  unsigned long long* __restrict__ counter, float dx, float dy, float dz, int L,
+                  int32_t* __restrict__ out)
 {
     __shared__ int32_t stack[kMaxStack][kTB];
     __shared__ unsigned long long cur;
@@ -512,16 +523,17 @@ __global__ void __launch_bounds__(kTB)
         if (threadIdx.x == 0)
             cur = atomicAdd(counter, 1ull);
         __syncthreads();
-        int64_t const it = int64_t(cur);
+        unsigned long long const ticket = cur;
         __syncthreads();
-        if (it >= nitems)
+        if (ticket >= static_cast<unsigned long long>(nitems)) // unsigned: any counter value past the list ends
             break;
+        int64_t const it = int64_t(ticket);
         int32_t const m = item_mesh[it];
         batch_mesh_desc const D = meshes[m];
         int64_t const q = item_q0[it] + threadIdx.x;
         if (q < D.q_end)
-            out[q] = count_one(pos[3 * q], pos[3 * q + 1], pos[3 * q + 2], dx, dy, dz, D.nodes, D.rec, D.brute,
-                               D.nbrute ? *D.nbrute : 0, D.use_bvh, stack, int(threadIdx.x));
+            out[q] = count_one<Buckets>(pos[3 * q], pos[3 * q + 1], pos[3 * q + 2], dx, dy, dz, D.nodes, D.rec,
+                                        D.brute, D.nbrute ? *D.nbrute : 0, D.use_bvh, L, stack, int(threadIdx.x));
     }
 }
 
@@ -557,10 +569,11 @@ __global__ void __launch_bounds__(kTB) k_batch_keys(float const* __restrict__ po
 
 // The persistent counting kernel on sorted queries (the same work items: each mesh keeps its index range), by packet
 // traversal: a work item's kTB queries are the block's two waves' packets.
+template <bool Buckets>
 __global__ void __launch_bounds__(kTB)
     k_count_batch_packet(float const* __restrict__ pos, batch_mesh_desc const* __restrict__ meshes,
                          int32_t const* __restrict__ item_mesh, int64_t const* __restrict__ item_q0, int64_t nitems,
-                         unsigned long long* __restrict__ counter, float dx, float dy, float dz,
+                         unsigned long long* __restrict__ counter, float dx, float dy, float dz, int L,
                          int32_t* __restrict__ out)
 {
     __shared__ int32_t stk_node[kTB / 64][kMaxStack];
@@ -572,10 +585,11 @@ __global__ void __launch_bounds__(kTB)
         if (threadIdx.x == 0)
             cur = atomicAdd(counter, 1ull);
         __syncthreads();
-        int64_t const it = int64_t(cur);
+        unsigned long long const ticket = cur;
         __syncthreads();
-        if (it >= nitems)
+        if (ticket >= static_cast<unsigned long long>(nitems)) // unsigned: any counter value past the list ends
             break;
+        int64_t const it = int64_t(ticket);
         int32_t const m = item_mesh[it];
         batch_mesh_desc const D = meshes[m];
         int64_t const q = item_q0[it] + threadIdx.x;
@@ -587,8 +601,9 @@ __global__ void __launch_bounds__(kTB)
             py = pos[3 * q + 1];
             pz = pos[3 * q + 2];
         }
-        int const c = count_packet(px, py, pz, valid, dx, dy, dz, D.nodes, D.rec, D.brute, D.nbrute ? *D.nbrute : 0,
-                                   D.use_bvh, stk_node[w], stk_mask[w], int(threadIdx.x & 63));
+        int const c = count_packet<Buckets>(px, py, pz, valid, dx, dy, dz, D.nodes, D.rec, D.brute,
+                                            D.nbrute ? *D.nbrute : 0, D.use_bvh, L, stk_node[w], stk_mask[w],
+                                            int(threadIdx.x & 63));
         if (valid)
             out[q] = c;
     }
@@ -1514,9 +1529,10 @@ int agwn_label_surrogate_dev(agwn_mesh const* m, agwn_mlp const* mlp, float cons
 // Batched evaluation of many meshes with uneven query counts: the full GWN of every query of every mesh, in two
 // persistent launches (the integer term, then the fractional term plus the counts) captured once in a HIP graph and
 // replayed while the inputs stay the same. Each mesh keeps its own LBVH and chain-ordered boundary stream; work items
-// (mesh, query tile) are ordered longest mesh first and taken from atomic counters. The fractional kernel's tile shape
-// (queries per lane x block) is chosen by an autotuner that times every shape on a sample of the meshes. Per query the
-// arithmetic is exactly the single-mesh path's (k_count, k_frac_f32_chain with x0 shared by the batch).
+// (mesh, query tile) are ordered longest mesh first and taken from atomic counters. An autotuner picks the fractional
+// kernel's shape (queries per lane x block x stream points per LDS stage), timing every shape on a sample of the
+// meshes, and the counting traversal's leaf-bucket size, timing every size on the whole batch. Per query the arithmetic
+// is exactly the single-mesh path's (k_count, k_frac_f32_chain with x0 shared by the batch) for every choice.
 // =====================================================================================================================
 struct agwn_batch
 {
@@ -1531,15 +1547,17 @@ struct agwn_batch
     agwn::dbuf pexp;             // int16 per stream point: its mesh's sexp
     agwn::dbuf zero;             // an int32 zero: the exhaustive-test count of meshes without a BVH state
     hipStream_t bs = nullptr;    // the batch's own stream (graph capture and replay)
-    int variant = -1;            // tuned fractional kernel shape
-    double tune_ms[8] = {};
+    int variant = -1;            // tuned fractional kernel shape (index into the shape table)
+    int leaf = -1;               // tuned leaf-bucket size (index into kLeafSizes)
+    double tune_ms[32] = {};     // the tuner's time per fractional shape
+    double leaf_ms[8] = {};      // the tuner's time per leaf-bucket size
     int ncu = 0;
     // the cached launch: inputs it was built for, its buffers and graph
     float const* g_pos = nullptr;
     float* g_out = nullptr;
     float g_x0[3] = {0, 0, 0};
     std::vector<int64_t> g_qoff;
-    int g_variant = -1;
+    int g_variant = -1, g_leaf = -1;
     std::vector<std::shared_ptr<agwn::dir_state<float>>> states;
     agwn::dbuf cdesc, fdesc, citem_m, citem_q, fitem_m, fitem_q, counters, counts, p32;
     int64_t ncitems = 0, nfitems = 0;
@@ -1563,6 +1581,11 @@ struct agwn_batch
 
 namespace agwn
 {
+// leaf-bucket sizes offered to the autotuner (1: plain leaves)
+constexpr int kLeafSizes[] = {1, 2, 4, 8, 16};
+constexpr int kNumLeafSizes = int(sizeof(kLeafSizes) / sizeof(kLeafSizes[0]));
+static_assert(kLeafSizes[kNumLeafSizes - 1] <= kMaxBucket, "bucket size");
+static_assert(kNumLeafSizes == AGWN_BATCH_LEAF_SIZES, "the header's count of leaf-bucket sizes");
 namespace
 {
 __global__ void __launch_bounds__(kB) k_zero_u64(unsigned long long* p, int n)
@@ -1756,8 +1779,8 @@ int batch_prepare(agwn_batch& B, float const* pos, int64_t const* qoff, float co
                                         std::ldexp(1.0, -B.sexp[size_t(m)]), B.far[size_t(m)]};
     }
     // work items, longest first: counting by triangle count, the fractional term by stream length
-    int fq = 0, fb = 0;
-    frac_batch_shape(variant, fq, fb);
+    int fq = 0, fb = 0, fs = 0;
+    frac_batch_shape(variant, fq, fb, fs);
     std::vector<int64_t> order(static_cast<size_t>(k));
     for (int64_t m = 0; m < k; ++m)
         order[size_t(m)] = m;
@@ -1798,6 +1821,7 @@ int batch_prepare(agwn_batch& B, float const* pos, int64_t const* qoff, float co
     AGWN_TRY(up(B.fitem_m, fim.data(), fim.size() * sizeof(int32_t)));
     AGWN_TRY(up(B.fitem_q, fiq.data(), fiq.size() * sizeof(int64_t)));
     AGWN_TRY(B.counters.alloc(2 * sizeof(unsigned long long), nullptr));
+    AGWN_HIP(hipMemsetAsync(B.counters.p, 0, 2 * sizeof(unsigned long long), s)); // prepared outside any capture
     AGWN_TRY(B.counts.alloc(size_t(std::max<int64_t>(n, 1)) * sizeof(int32_t), nullptr));
     // sorted rays for the integer term: whole batches only (the tuner's samples leave other queries without keys),
     // with room for the mesh index and at least 4 Morton bits per axis in the 32-bit sort key
@@ -1845,18 +1869,44 @@ int batch_prepare(agwn_batch& B, float const* pos, int64_t const* qoff, float co
     return ok;
 }
 
-// The batch's launches for the prepared lists: stream rotation, counter reset, counting, fractional + counts.
-int batch_launches(agwn_batch& B, float const* pos, float const x0f[3], int variant, float* out)
+// The batch's launches for the prepared lists, by phase: kPhaseCount -- counter reset and the integer term (sorted
+// rays: keys, sort, gather, packet traversal, scatter; else the per-lane traversal) with leaf buckets of
+// kLeafSizes[leaf]; kPhaseFrac -- stream rotation, counter reset, fractional term + counts with shape `variant`. The
+// fractional kernel reads the counts, so a whole evaluation runs both, in this order. kWarm: only the phases'
+// persistent kernels, one workgroup with no work items each (it exits at once; loads the kernel's code).
+constexpr int kPhaseCount = 1, kPhaseFrac = 2, kWarm = 4;
+
+int batch_launches(agwn_batch& B, float const* pos, float const x0f[3], int variant, int leaf, float* out,
+                   int phases = kPhaseCount | kPhaseFrac)
 {
     hipStream_t const s = B.bs;
     double const x0d[3] = {double(x0f[0]), double(x0f[1]), double(x0f[2])};
-    int64_t const npts = B.pt_off[size_t(B.k)];
-    AGWN_TRY(frac_stream_rotate(B.stream.as<float4 const>(), npts, x0d, B.pexp.as<int16_t const>(),
-                                B.p32.as<float4>(), s));
-    k_zero_u64<<<dim3(1), dim3(kB), 0, s>>>(B.counters.as<unsigned long long>(), 2);
-    AGWN_HIP(hipGetLastError());
-    if (B.ncitems > 0)
+    int const L = kLeafSizes[leaf];
+    if (phases & kWarm)
     {
+        unsigned long long* const ctr = B.counters.as<unsigned long long>();
+        k_zero_u64<<<dim3(1), dim3(kB), 0, s>>>(ctr, 2); // the warm-up kernels take a ticket each
+        AGWN_HIP(hipGetLastError());
+        if (phases & kPhaseCount)
+        {
+            auto kc = L > 1 ? k_count_batch_packet<true> : k_count_batch_packet<false>;
+            auto ku = L > 1 ? k_count_batch<true> : k_count_batch<false>;
+            (B.sorted ? kc : ku)<<<dim3(1), dim3(kTB), 0, s>>>(pos, B.cdesc.as<batch_mesh_desc>(),
+                                                               B.citem_m.as<int32_t>(), B.citem_q.as<int64_t>(), 0,
+                                                               ctr, x0f[0], x0f[1], x0f[2], L,
+                                                               B.counts.as<int32_t>());
+            AGWN_HIP(hipGetLastError());
+        }
+        if (phases & kPhaseFrac)
+            AGWN_TRY(frac_batch_launch(variant, 1, pos, B.fdesc.as<batch_frac_desc>(), B.fitem_m.as<int32_t>(),
+                                       B.fitem_q.as<int64_t>(), 0, ctr + 1, B.p32.as<float4>(), x0d,
+                                       B.counts.as<int32_t>(), out, s));
+        return ok;
+    }
+    if ((phases & kPhaseCount) && B.ncitems > 0)
+    {
+        k_zero_u64<<<dim3(1), dim3(kB), 0, s>>>(B.counters.as<unsigned long long>(), 1);
+        AGWN_HIP(hipGetLastError());
         if (B.sorted)
         {
             int64_t const n = B.g_sorted_n;
@@ -1872,9 +1922,11 @@ int batch_launches(agwn_batch& B, float const* pos, float const x0f[3], int vari
             k_ray_gather<float><<<dim3(g), dim3(kB), 0, s>>>(sorted, n, pos, B.spos.as<float>());
             AGWN_HIP(hipGetLastError());
             int const blocks = int(std::min<int64_t>(B.ncitems, int64_t(B.ncu) * 16));
-            k_count_batch_packet<<<dim3(unsigned(blocks)), dim3(kTB), 0, s>>>(
-                B.spos.as<float>(), B.cdesc.as<batch_mesh_desc>(), B.citem_m.as<int32_t>(), B.citem_q.as<int64_t>(),
-                B.ncitems, B.counters.as<unsigned long long>(), x0f[0], x0f[1], x0f[2], B.scnt.as<int32_t>());
+            auto kc = L > 1 ? k_count_batch_packet<true> : k_count_batch_packet<false>;
+            kc<<<dim3(unsigned(blocks)), dim3(kTB), 0, s>>>(B.spos.as<float>(), B.cdesc.as<batch_mesh_desc>(),
+                                                            B.citem_m.as<int32_t>(), B.citem_q.as<int64_t>(),
+                                                            B.ncitems, B.counters.as<unsigned long long>(), x0f[0],
+                                                            x0f[1], x0f[2], L, B.scnt.as<int32_t>());
             AGWN_HIP(hipGetLastError());
             k_ray_scatter<<<dim3(g), dim3(kB), 0, s>>>(sorted, n, B.scnt.as<int32_t>(), B.counts.as<int32_t>());
             AGWN_HIP(hipGetLastError());
@@ -1883,18 +1935,22 @@ int batch_launches(agwn_batch& B, float const* pos, float const x0f[3], int vari
         {
             // 10 traversal blocks per CU: the 14 KiB LDS stacks allow 11; 5 waves per SIMD hide the traversal's loads
             int const blocks = int(std::min<int64_t>(B.ncitems, int64_t(B.ncu) * 10));
-            k_count_batch<<<dim3(unsigned(blocks)), dim3(kTB), 0, s>>>(
-                pos, B.cdesc.as<batch_mesh_desc>(), B.citem_m.as<int32_t>(), B.citem_q.as<int64_t>(), B.ncitems,
-                B.counters.as<unsigned long long>(), x0f[0], x0f[1], x0f[2], B.counts.as<int32_t>());
+            auto kc = L > 1 ? k_count_batch<true> : k_count_batch<false>;
+            kc<<<dim3(unsigned(blocks)), dim3(kTB), 0, s>>>(pos, B.cdesc.as<batch_mesh_desc>(), B.citem_m.as<int32_t>(),
+                                                            B.citem_q.as<int64_t>(), B.ncitems,
+                                                            B.counters.as<unsigned long long>(), x0f[0], x0f[1],
+                                                            x0f[2], L, B.counts.as<int32_t>());
             AGWN_HIP(hipGetLastError());
         }
     }
-    if (B.nfitems > 0)
+    if ((phases & kPhaseFrac) && B.nfitems > 0)
     {
-        int fq = 0, fb = 0;
-        frac_batch_shape(variant, fq, fb);
-        int const per_cu = fb == 256 ? 8 : 16;
-        int const blocks = int(std::min<int64_t>(B.nfitems, int64_t(B.ncu) * per_cu));
+        int64_t const npts = B.pt_off[size_t(B.k)];
+        AGWN_TRY(frac_stream_rotate(B.stream.as<float4 const>(), npts, x0d, B.pexp.as<int16_t const>(),
+                                    B.p32.as<float4>(), s));
+        k_zero_u64<<<dim3(1), dim3(kB), 0, s>>>(B.counters.as<unsigned long long>() + 1, 1);
+        AGWN_HIP(hipGetLastError());
+        int const blocks = int(std::min<int64_t>(B.nfitems, int64_t(B.ncu) * frac_batch_blocks_per_cu(variant)));
         AGWN_TRY(frac_batch_launch(variant, blocks, pos, B.fdesc.as<batch_frac_desc>(), B.fitem_m.as<int32_t>(),
                                    B.fitem_q.as<int64_t>(), B.nfitems, B.counters.as<unsigned long long>() + 1,
                                    B.p32.as<float4>(), x0d, B.counts.as<int32_t>(), out, s));
@@ -1902,7 +1958,10 @@ int batch_launches(agwn_batch& B, float const* pos, float const x0f[3], int vari
     return ok;
 }
 
-// Autotuner: times every fractional kernel shape on every 8th mesh (same work for each shape) and keeps the fastest.
+// Autotuner. The two launches are independent in cost, so each is tuned alone: every fractional kernel shape is timed
+// on every 8th mesh (the same work for each shape), then every leaf-bucket size on the whole batch's integer term
+// (the path its evaluations take: sorted packets or per-lane rays). Each kernel is launched once with no work items
+// before it is timed (its code is then loaded). The fastest of each is kept.
 int batch_tune(agwn_batch& B, float const* pos, int64_t const* qoff, float const x0f[3], float* out)
 {
     std::vector<int64_t> sample;
@@ -1920,19 +1979,27 @@ int batch_tune(agwn_batch& B, float const* pos, int64_t const* qoff, float const
             (void)hipEventDestroy(b);
         }
     } eg{e0, e1};
+    auto timed = [&](int variant, int leaf, int phase, double& ms_out) -> int {
+        AGWN_TRY(batch_launches(B, pos, x0f, variant, leaf, out, phase | kWarm));
+        AGWN_HIP(hipEventRecord(e0, B.bs));
+        AGWN_TRY(batch_launches(B, pos, x0f, variant, leaf, out, phase));
+        AGWN_HIP(hipEventRecord(e1, B.bs));
+        AGWN_HIP(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        AGWN_HIP(hipEventElapsedTime(&ms, e0, e1));
+        ms_out = double(ms);
+        return ok;
+    };
     int best = 0;
     double best_ms = 1e300;
     for (int v = 0; v < frac_batch_variants(); ++v)
     {
         AGWN_TRY(batch_prepare(B, pos, qoff, x0f, v, sample));
-        AGWN_TRY(batch_launches(B, pos, x0f, v, out)); // warm-up
-        AGWN_HIP(hipEventRecord(e0, B.bs));
-        AGWN_TRY(batch_launches(B, pos, x0f, v, out));
-        AGWN_HIP(hipEventRecord(e1, B.bs));
-        AGWN_HIP(hipEventSynchronize(e1));
-        float ms = 0.0f;
-        AGWN_HIP(hipEventElapsedTime(&ms, e0, e1));
-        if (v < 8)
+        if (v == 0)
+            AGWN_TRY(batch_launches(B, pos, x0f, v, 0, out, kPhaseCount)); // the counts the fractional kernel reads
+        double ms = 0.0;
+        AGWN_TRY(timed(v, 0, kPhaseFrac, ms));
+        if (v < int(sizeof(B.tune_ms) / sizeof(B.tune_ms[0])))
             B.tune_ms[v] = ms;
         if (ms < best_ms)
         {
@@ -1941,7 +2008,22 @@ int batch_tune(agwn_batch& B, float const* pos, int64_t const* qoff, float const
         }
     }
     B.variant = best;
-    B.g_pos = nullptr; // the lists now describe the sample only
+    AGWN_TRY(batch_prepare(B, pos, qoff, x0f, best, {}));
+    int best_leaf = 0;
+    best_ms = 1e300;
+    for (int l = 0; l < kNumLeafSizes && B.ncitems > 0; ++l)
+    {
+        double ms = 0.0;
+        AGWN_TRY(timed(best, l, kPhaseCount, ms));
+        B.leaf_ms[l] = ms;
+        if (ms < best_ms)
+        {
+            best_ms = ms;
+            best_leaf = l;
+        }
+    }
+    B.leaf = best_leaf;
+    B.g_pos = nullptr; // the lists are rebuilt (and the graph captured) by the next evaluation
     return ok;
 }
 
@@ -1972,7 +2054,7 @@ int batch_eval(agwn_batch* b, float const* pos, int64_t const* qoff, float const
         AGWN_TRY(batch_tune(B, pos, qoff, x0, out));
     std::vector<int64_t> q(qoff, qoff + B.k + 1);
     bool same = B.exec && B.g_pos == pos && B.g_out == out && B.g_x0[0] == x0[0] && B.g_x0[1] == x0[1] &&
-                B.g_x0[2] == x0[2] && B.g_qoff == q && B.g_variant == B.variant;
+                B.g_x0[2] == x0[2] && B.g_qoff == q && B.g_variant == B.variant && B.g_leaf == B.leaf;
     if (same)
     {
         // the graph's prepared BVH states must still cover the queries (their contents may have changed in place)
@@ -1995,7 +2077,7 @@ int batch_eval(agwn_batch* b, float const* pos, int64_t const* qoff, float const
         AGWN_TRY(batch_prepare(B, pos, qoff, x0, B.variant, {}));
         hipGraph_t graph = nullptr;
         AGWN_HIP(hipStreamBeginCapture(B.bs, hipStreamCaptureModeThreadLocal));
-        int const st = batch_launches(B, pos, x0, B.variant, out);
+        int const st = batch_launches(B, pos, x0, B.variant, B.leaf, out);
         hipError_t const ce = hipStreamEndCapture(B.bs, &graph);
         AGWN_TRY(st);
         AGWN_HIP(ce);
@@ -2009,6 +2091,7 @@ int batch_eval(agwn_batch* b, float const* pos, int64_t const* qoff, float const
         B.g_x0[2] = x0[2];
         B.g_qoff = q;
         B.g_variant = B.variant;
+        B.g_leaf = B.leaf;
     }
     AGWN_HIP(hipGraphLaunch(B.exec, B.bs));
     AGWN_HIP(hipStreamSynchronize(B.bs));
@@ -2045,8 +2128,8 @@ int agwn_batch_eval_dev(agwn_batch* b, float const* pos, int64_t const* q_off, f
 {
     return guarded2([&] { return batch_eval(b, pos, q_off, x0, out, hipStream_t(s)); });
 }
-int agwn_batch_info(agwn_batch const* b, int64_t* k, int64_t* ntri, int64_t* nunit, int* frac_q, int* frac_block,
-                    double* tune_ms)
+int agwn_batch_info(agwn_batch const* b, int64_t* k, int64_t* ntri, int64_t* nunit, int* frac_shape, int* leaf,
+                    double* frac_ms, double* leaf_ms)
 {
     return guarded2([&]() -> int {
         AGWN_CHECK(b != nullptr, "batch handle is null");
@@ -2062,16 +2145,55 @@ int agwn_batch_info(agwn_batch const* b, int64_t* k, int64_t* ntri, int64_t* nun
             *ntri = t;
         if (nunit)
             *nunit = u;
-        int q = 0, blk = 0;
-        if (b->variant >= 0)
-            frac_batch_shape(b->variant, q, blk);
+        if (frac_shape)
+            *frac_shape = b->variant;
+        if (leaf)
+            *leaf = b->leaf;
+        if (frac_ms)
+            for (int v = 0; v < frac_batch_variants(); ++v)
+                frac_ms[v] = b->tune_ms[v];
+        if (leaf_ms)
+            for (int l = 0; l < kNumLeafSizes; ++l)
+                leaf_ms[l] = b->leaf_ms[l];
+        return ok;
+    });
+}
+int agwn_batch_set_tuning(agwn_batch* b, int frac_shape, int leaf)
+{
+    return guarded2([&]() -> int {
+        AGWN_CHECK(b != nullptr, "batch handle is null");
+        AGWN_CHECK(frac_shape >= 0 && frac_shape < frac_batch_variants(), "fractional shape %d out of range [0, %d)",
+                   frac_shape, frac_batch_variants());
+        AGWN_CHECK(leaf >= 0 && leaf < kNumLeafSizes, "leaf-bucket size index %d out of range [0, %d)", leaf,
+                   kNumLeafSizes);
+        std::lock_guard<std::mutex> lk(b->mu);
+        b->variant = frac_shape;
+        b->leaf = leaf;
+        return ok;
+    });
+}
+int agwn_batch_frac_shape(int v, int* frac_q, int* frac_block, int* frac_stage)
+{
+    return guarded2([&]() -> int {
+        AGWN_CHECK(v >= 0 && v < frac_batch_variants(), "fractional shape %d out of range [0, %d)", v,
+                   frac_batch_variants());
+        int q = 0, blk = 0, st = 0;
+        frac_batch_shape(v, q, blk, st);
         if (frac_q)
             *frac_q = q;
         if (frac_block)
             *frac_block = blk;
-        if (tune_ms)
-            for (int v = 0; v < frac_batch_variants() && v < 8; ++v)
-                tune_ms[v] = b->tune_ms[v];
+        if (frac_stage)
+            *frac_stage = st;
+        return ok;
+    });
+}
+int agwn_batch_leaf_size(int l, int* leaf)
+{
+    return guarded2([&]() -> int {
+        AGWN_CHECK(l >= 0 && l < kNumLeafSizes, "leaf-bucket size index %d out of range [0, %d)", l, kNumLeafSizes);
+        if (leaf)
+            *leaf = kLeafSizes[l];
         return ok;
     });
 }

@@ -181,17 +181,30 @@ typedef struct agwn_batch agwn_batch;
 int agwn_batch_create(int64_t k, const float* verts, const int64_t* vert_off, const int32_t* idx,
                       const int64_t* idx_off, const float* seg7, const int64_t* seg_off, agwn_batch** out);
 int agwn_batch_destroy(agwn_batch* batch);
-/* Autotuner: times each fractional kernel tile shape (queries per lane x workgroup size) on every 8th mesh and keeps
- * the fastest (run by the first agwn_batch_eval_dev if never called). `out` is used as scratch. Synchronous. */
+/* The autotuner's search space. Fractional kernel shapes v in [0, AGWN_BATCH_FRAC_SHAPES): queries per lane, lanes
+ * per workgroup and stream points per LDS stage (in multiples of the workgroup size), see agwn_batch_frac_shape.
+ * Leaf-bucket sizes l in [0, AGWN_BATCH_LEAF_SIZES) of the counting traversal (LBVH subtrees of at most that many
+ * triangles tested as one leaf), see agwn_batch_leaf_size. Results do not depend on either choice. */
+#define AGWN_BATCH_FRAC_SHAPES 15
+#define AGWN_BATCH_LEAF_SIZES 5
+int agwn_batch_frac_shape(int v, int* frac_q, int* frac_block, int* frac_stage);
+int agwn_batch_leaf_size(int l, int* leaf);
+/* Autotuner: times each fractional kernel shape on every 8th mesh, then each leaf-bucket size on the whole batch's
+ * integer term, and keeps the fastest of each (run by the first agwn_batch_eval_dev unless tuned or set before).
+ * `out` is used as scratch. Synchronous. */
 int agwn_batch_tune(agwn_batch* batch, const float* pos, const int64_t* q_off, const float* x0, float* out,
                     agwn_stream stream);
+/* Sets the shape and leaf-bucket size instead of tuning (e.g. a table saved from an earlier agwn_batch_info). */
+int agwn_batch_set_tuning(agwn_batch* batch, int frac_shape, int leaf);
 /* out[i] = full GWN of query i against its mesh. Ordered after prior work on `stream`; returns when done. */
 int agwn_batch_eval_dev(agwn_batch* batch, const float* pos, const int64_t* q_off, const float* x0, float* out,
                         agwn_stream stream);
-/* Mesh count, triangles, unit boundary segments, the tuned shape (0, 0 before tuning) and the tuner's timings (ms,
- * up to 8 shapes, in shape order); any output may be NULL. */
-int agwn_batch_info(const agwn_batch* batch, int64_t* k, int64_t* ntri, int64_t* nunit, int* frac_q, int* frac_block,
-                    double* tune_ms);
+/* Mesh count, triangles, unit boundary segments, the chosen shape and leaf-bucket size indexes (-1 before tuning),
+ * and the tuner's timings in ms (frac_ms[AGWN_BATCH_FRAC_SHAPES] per shape, on the sample; leaf_ms
+ * [AGWN_BATCH_LEAF_SIZES] per size, the integer term of the whole batch; 0 where not timed); any output may be
+ * NULL. */
+int agwn_batch_info(const agwn_batch* batch, int64_t* k, int64_t* ntri, int64_t* nunit, int* frac_shape, int* leaf,
+                    double* frac_ms, double* leaf_ms);
 
 /* ---------------------------------------------------------------------------------------------------------------
  * Learned inside/outside surrogate (no counterpart in the reference: an accelerator in front of the exact path, whose

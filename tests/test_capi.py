@@ -102,3 +102,19 @@ def test_boundary_torus_matches_oracle(agwn, oracle):
     g = load_golden("torus_sub")
     assert len(got) == int(g["nseg_f64"])
     assert np.array_equal(sort_segments(got), sort_segments(oracle.boundary(verts, idx)))
+
+
+def test_batch_tuning_space(agwn):
+    """The batch autotuner's search space (host tables, no launch): 15 distinct fractional kernel shapes -- queries
+    per lane x workgroup size x stream points per LDS stage -- and leaf-bucket sizes 1..16; indexes outside the space
+    are errors."""
+    from antipodal_amd.api import HipMeshBatch
+
+    shapes = [HipMeshBatch.frac_shape(v) for v in range(HipMeshBatch.FRAC_SHAPES)]
+    assert len(set(shapes)) == 15
+    assert {s[2] for s in shapes} == {1, 2, 4} and {s[1] for s in shapes} == {128, 256}
+    assert [HipMeshBatch.leaf_size(l) for l in range(HipMeshBatch.LEAF_SIZES)] == [1, 2, 4, 8, 16]
+    with pytest.raises(agwn.AgwnError, match="out of range"):
+        HipMeshBatch.frac_shape(15)
+    with pytest.raises(agwn.AgwnError, match="out of range"):
+        HipMeshBatch.leaf_size(-1)

@@ -3,7 +3,8 @@
 Bars: every query's full GWN equals the single-mesh path's (HipMesh.full, FP32, same x0) bit for bit -- the batch
 runs the same per-query arithmetic in persistent launches; graph replays reproduce the first result; a replay after
 the queries moved farther out (same buffers) re-prepares and stays exact; the autotuner reports a time for every
-tile shape. Meshes include a closed one (no boundary), a mesh without triangles and a mesh without queries.
+fractional kernel shape and leaf-bucket size, and every shape and size gives the same bits. Meshes include a closed
+one (no boundary), a mesh without triangles and a mesh without queries.
 """
 from __future__ import annotations
 
@@ -54,7 +55,8 @@ def test_batch_matches_single_mesh_path(gpu):
         out = batch.eval(pos, q_off, x0)
         info = batch.info()
         print("batch:", info, "queries", int(q_off[-1]))
-        assert len(info["tune_ms"]) == 5 and info["queries_per_lane"] in (1, 2, 4)
+        assert len(info["frac_table"]) == gpu.HipMeshBatch.FRAC_SHAPES and info["queries_per_lane"] in (1, 2, 4)
+        assert len(info["leaf_table"]) == gpu.HipMeshBatch.LEAF_SIZES and info["leaf_size"] in (1, 2, 4, 8, 16)
         got = out.cpu().numpy()
         for m, (v, idx, b) in enumerate(meshes):
             if q_off[m + 1] == q_off[m]:
@@ -120,3 +122,33 @@ def test_sorted_batch_counts_match_single_mesh_path(gpu):
         out2 = torch.full_like(out, -7.0)
         batch.eval(pos, q_off, x0, out=out2)
         assert torch.equal(out, out2)
+
+
+@pytest.mark.parametrize("qmin", [1, 4000])
+def test_every_tuning_choice_gives_the_same_bits(gpu, qmin):
+    """Each fractional kernel shape (queries per lane x block x LDS stage) and each leaf-bucket size, set instead of
+    tuned, reproduces the tuned evaluation bit for bit: the shapes change which lanes hold which queries and how many
+    stream points a barrier stages, the buckets which triangles are tested without their own boxes -- not the
+    per-query arithmetic. qmin = 4000: a batch of 2^16 queries or more (sorted rays, packet traversal); qmin = 1 has
+    small meshes (per-lane traversal when the whole batch is small)."""
+    import torch
+
+    meshes, queries = make_batch(24 if qmin > 1 else 16, seed=13, qmin=qmin)
+    x0 = mg.example_x0(np.float32)
+    q_off = np.concatenate([[0], np.cumsum([len(q) for q in queries])]).astype(np.int64)
+    pos = torch.from_numpy(np.concatenate(queries)).cuda()
+    B = gpu.HipMeshBatch
+    with B(meshes) as batch:
+        ref = batch.eval(pos, q_off, x0).clone()
+        info = batch.info()
+        print(f"tuned: shape {info['frac_shape']} leaf {info['leaf_size']}; queries {int(q_off[-1])}", flush=True)
+        out = torch.empty_like(ref)
+        choices = [(v, 0) for v in range(B.FRAC_SHAPES)] + [(0, l) for l in range(1, B.LEAF_SIZES)]
+        for v, l in choices:
+            print(f"shape {B.frac_shape(v)} leaf {B.leaf_size(l)}", flush=True)
+            batch.set_tuning(v, l)
+            out.fill_(-7.0)
+            batch.eval(pos, q_off, x0, out=out)
+            assert torch.equal(out, ref), f"shape {v}, leaf index {l}: {int((out != ref).sum())} differ"
+        with pytest.raises(gpu.AgwnError, match="out of range"):
+            batch.set_tuning(B.FRAC_SHAPES, 0)
