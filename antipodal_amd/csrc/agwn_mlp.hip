@@ -679,21 +679,24 @@ __global__ void __launch_bounds__(256, kInferBlocks) k_mlp_infer(Src const src, 
     int const n0 = wave * 64;
     f32x16 acc[2][kTileJ]; // C[r][c] of the lane's units: unit r = 8(e/4) + 4(l/32) + e%4
     // layer l's MFMAs: acc = b_l + Wt_l[units of the wave] act^T. The biases go into acc[i][0] and enter every point
-    // block's first MFMA as its C operand (no copies). The k loop runs two steps per iteration; step s's weight
-    // fragments (f[s & 1]) are fetched two steps ahead and its activation operands (b[s & 1], LDS) one step ahead,
-    // so the MFMAs of a step never wait for either; scheduling barriers keep the compiler from sinking the fetches
-    // back to their uses.
+    // block's first MFMA as its C operand (no copies). The k loop is unrolled (static buffer indexes; a layer of
+    // K = 192 leaves the last 4 of 16 steps); step s's weight fragments (f[s % 3]) are fetched from L2 three steps
+    // ahead and its activation operands (b[s & 1], LDS) one step ahead, so the MFMAs of a step rarely wait for either
+    // (two steps ahead measured 4 % slower: L2 latency under this kernel's load exceeds two steps); scheduling
+    // barriers keep the compiler from sinking the fetches back to their uses. 168 VGPRs: three waves per SIMD.
     auto layer = [&](int l) __attribute__((always_inline)) {
         int const K = l == 0 ? kIn : kH;
         // fragments of unit blocks 2w and 2w + 1 (units n0 .. n0 + 63); k step st at + 512 st
         __bf16 const* wa0 = P.wf[l] + (int64_t(n0 / 32) * (K / 16) * 64 + lane) * 8;
         __bf16 const* wa1 = wa0 + int64_t(K / 16) * 512;
         __bf16 const* brow = act + li * kLdsStride + kh;
-        bf16x8 f0[2], f1[2], b[2][kTileJ];
-        f0[0] = *reinterpret_cast<bf16x8 const*>(wa0);
-        f1[0] = *reinterpret_cast<bf16x8 const*>(wa1);
-        f0[1] = *reinterpret_cast<bf16x8 const*>(wa0 + 512);
-        f1[1] = *reinterpret_cast<bf16x8 const*>(wa1 + 512);
+        bf16x8 f0[3], f1[3], b[2][kTileJ];
+#pragma unroll
+        for (int t = 0; t < 3; ++t)
+        {
+            f0[t] = *reinterpret_cast<bf16x8 const*>(wa0 + 512 * t);
+            f1[t] = *reinterpret_cast<bf16x8 const*>(wa1 + 512 * t);
+        }
 #pragma unroll
         for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -709,8 +712,14 @@ __global__ void __launch_bounds__(256, kInferBlocks) k_mlp_infer(Src const src, 
         for (int j = 0; j < kTileJ; ++j)
             b[0][j] = *reinterpret_cast<bf16x8 const*>(brow + j * 32 * kLdsStride);
         __builtin_amdgcn_sched_barrier(0);
-        auto step = [&](int k0, int buf, bool first) __attribute__((always_inline)) {
-            if (k0 + 16 < K) // the next step's activation operands
+#pragma unroll
+        for (int st = 0; st < kH / 16; ++st)
+        {
+            int const k0 = 16 * st;
+            if (k0 >= K)
+                break;
+            int const buf = st & 1, fb = st % 3;
+            if (k0 + 16 < K)
             {
 #pragma unroll
                 for (int j = 0; j < kTileJ; ++j)
@@ -719,11 +728,11 @@ __global__ void __launch_bounds__(256, kInferBlocks) k_mlp_infer(Src const src, 
 #pragma unroll
             for (int i = 0; i < 2; ++i)
             {
-                bf16x8 const a = i == 0 ? f0[buf] : f1[buf];
-                if (first)
+                bf16x8 const a = i == 0 ? f0[fb] : f1[fb];
+                if (st == 0)
                 {
 #pragma unroll
-                    for (int j = kTileJ - 1; j >= 0; --j) // acc[i][0] (the biases) is read by every block, last
+                    for (int j = kTileJ - 1; j >= 0; --j)
                         acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b[buf][j], acc[i][0], 0, 0, 0);
                 }
                 else
@@ -733,21 +742,13 @@ __global__ void __launch_bounds__(256, kInferBlocks) k_mlp_infer(Src const src, 
                         acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b[buf][j], acc[i][j], 0, 0, 0);
                 }
             }
-            int const kn = k0 + 32; // the step two ahead, into the buffer just consumed
+            int const kn = k0 + 48;
             if (kn < K)
             {
-                f0[buf] = *reinterpret_cast<bf16x8 const*>(wa0 + int64_t(kn) * 32);
-                f1[buf] = *reinterpret_cast<bf16x8 const*>(wa1 + int64_t(kn) * 32);
+                f0[fb] = *reinterpret_cast<bf16x8 const*>(wa0 + int64_t(kn) * 32);
+                f1[fb] = *reinterpret_cast<bf16x8 const*>(wa1 + int64_t(kn) * 32);
             }
             __builtin_amdgcn_sched_barrier(0);
-        };
-        step(0, 0, true);
-        step(16, 1, false);
-#pragma unroll 1
-        for (int k0 = 32; k0 < K; k0 += 32)
-        {
-            step(k0, 0, false);
-            step(k0 + 16, 1, false);
         }
     };
 #pragma unroll 1
