@@ -451,9 +451,10 @@ __global__ void __launch_bounds__(kB) k_perm(int64_t* __restrict__ perm, int64_t
 }
 
 // ---------------------------------------------------------------------------------------------------------------------
-// Fused inference: a workgroup takes kTileP = 96 points through the features, the 4 hidden layers and the output
-// layer with the activations in one LDS buffer (96 x 264 BF16, 50 KiB: three workgroups per CU -- 128-point tiles,
-// two per CU, measured 1.5 % slower, 64-point tiles 2 % slower). Each layer computes
+// Fused inference: a workgroup takes kTileP = 128 points through the features, the 4 hidden layers and the output
+// layer with the activations in one LDS buffer (128 x 264 BF16, 66 KiB: two workgroups per CU; with weight fragments
+// fetched three k steps ahead, 128-point tiles measured 4 % faster than 96-point tiles at three per CU -- each weight
+// fragment feeds four MFMAs instead of three). Each layer computes
 // out^T[n][m] = sum_k Wt[n][k] act[m][k]: wave w owns units [64w, 64w + 64) (A operand = Wt rows in fragment order,
 // from L2) for all the tile's points (B operand = LDS rows); the accumulators start at the biases, so a lane's 4
 // consecutive accumulators are 4 consecutive units of one point, activated and written back to LDS as one 8-byte
@@ -463,7 +464,7 @@ __global__ void __launch_bounds__(kB) k_perm(int64_t* __restrict__ perm, int64_t
 // the rest.
 // ---------------------------------------------------------------------------------------------------------------------
 constexpr int kLdsStride = kH + 8; // BF16 per LDS row: 528 B rows keep the 16-lane groups of ds_read_b128 conflict-free
-constexpr int kTileP = 96;          // points per workgroup (a multiple of 32)
+constexpr int kTileP = 128;         // points per workgroup (a multiple of 32)
 constexpr int kTileJ = kTileP / 32; // 32-point MFMA column blocks per tile
 constexpr int kInferBlocks = (160 * 1024) / (kTileP * kLdsStride * 2 + 16 * kTileP); // workgroups per CU that the LDS holds
 static_assert(kTileP % 32 == 0 && kInferBlocks >= 1, "tile shape");
@@ -634,7 +635,7 @@ __global__ void __launch_bounds__(256, kInferBlocks) k_mlp_infer(Src const src, 
     int const li = lane & 31;
     int const hl = lane >> 5; // lane half
 
-    // features: the tile's 96 points in 3 chunks of 32, a point's 96 pairs in 24 groups of 4 (axis a = g / 8,
+    // features: the tile's points in kTileJ chunks of 32, a point's 96 pairs in 24 groups of 4 (axis a = g / 8,
     // frequencies 4 (g % 8) .. + 3). Wave w takes groups 6w .. 6w + 5 of every chunk, so the group -- and with it the
     // axis and the frequencies -- is the same across the wave and the frequencies come from scalar registers; lane
     // (li, hl) computes point li of the chunk at frequencies 4 (g % 8) + 2 hl + {0, 1}: 2 pairs, one 8-byte store.
@@ -683,7 +684,7 @@ __global__ void __launch_bounds__(256, kInferBlocks) k_mlp_infer(Src const src, 
     // K = 192 leaves the last 4 of 16 steps); step s's weight fragments (f[s % 3]) are fetched from L2 three steps
     // ahead and its activation operands (b[s & 1], LDS) one step ahead, so the MFMAs of a step rarely wait for either
     // (two steps ahead measured 4 % slower: L2 latency under this kernel's load exceeds two steps); scheduling
-    // barriers keep the compiler from sinking the fetches back to their uses. 168 VGPRs: three waves per SIMD.
+    // barriers keep the compiler from sinking the fetches back to their uses. ~210 VGPRs: two waves per SIMD.
     auto layer = [&](int l) __attribute__((always_inline)) {
         int const K = l == 0 ? kIn : kH;
         // fragments of unit blocks 2w and 2w + 1 (units n0 .. n0 + 63); k step st at + 512 st
