@@ -139,8 +139,9 @@ __device__ __forceinline__ void lplus_f64(double x, double y, double z, double s
     double const yz = __builtin_fma(y, y, z * z);
     double const l = sqrt_f64(__builtin_fma(x, x, yz));
     bool const pos = x >= 0.0;
-    m = pos ? 1.0 : l - x;
-    Pn = __builtin_fma(sm1 * x, m, pos ? l + x : yz);
+    double const lx = l + __builtin_fabs(x); // l + x for x >= 0, l - x below: no cancellation either way
+    m = pos ? 1.0 : lx;
+    Pn = __builtin_fma(sm1 * x, m, pos ? lx : yz);
 }
 
 // One pair's (num, den) in FP64, cancellation-free, both multiplied by the positive m0 m1 (the atan2 is unchanged):

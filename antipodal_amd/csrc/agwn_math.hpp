@@ -32,8 +32,11 @@ __device__ __forceinline__ double atan2_f64(double y, double x)
     mx = __builtin_amdgcn_ldexp(mx, -e);
     mn = __builtin_amdgcn_ldexp(mn, -e);
     bool const lo = mn <= k16 * mx, mid = !lo && mn <= k316 * mx;
-    double const num = lo ? mn : (mid ? __builtin_fma(-k8, mx, mn) : mn - mx);
-    double const den = lo ? mx : (mid ? __builtin_fma(k8, mn, mx) : mn + mx); // in [1/2, 2) unless both are zero
+    // num = mn, mn - k8 mx or mn - mx and den = mx, mx + k8 mn or mn + mx as one FMA each with k = 0, k8 or 1 (the
+    // same single roundings; selects of constants, no divergent branches)
+    double const k = lo ? 0.0 : (mid ? k8 : 1.0);
+    double const num = __builtin_fma(-k, mx, mn);
+    double const den = __builtin_fma(k, mn, mx); // in [1/2, 2) unless both are zero
     double r = __builtin_amdgcn_rcp(den);
     r = __builtin_fma(__builtin_fma(-den, r, 1.0), r, r);
     r = __builtin_fma(__builtin_fma(-den, r, 1.0), r, r);
