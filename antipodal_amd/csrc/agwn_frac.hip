@@ -635,15 +635,16 @@ __global__ void __launch_bounds__(B)
 // along x0 misses the ball (then neither the pole cone of the pair formula nor the ray of the integer term crosses D,
 // so no integer jump is involved). With f(y) = (y - q) / |y - q|^3, x = c - q, d = |x|, u = x / d and D the fan from c
 // (vector area a, area A, first moment M_ij = integral over D of (y - c)_i n_j):
-//   -Omega_D(q) = a.u / d^2 + (tr M - 3 u.M.u) / d^3 + R,  |R| <= 3 rho^2 A / (d - rho)^4
+//   -Omega_D(q) = a.u / d^2 + (tr M - 3 u.M.u) / d^3 + R,  |R| <= 3 I / (d - rho)^4 <= 3 rho^2 A / (d - rho)^4
 // (Taylor expansion of f about c to first order with the integral remainder; |D^2 f[v, v]| <= 6 |v|^2 / r^4, the
-// maximum of |-6 t v + (15 t^2 - 3) x| over t = u.v for unit v). D's orientation: it closes the mesh, so its boundary
+// maximum of |-6 t v + (15 t^2 - 3) x| over t = u.v for unit v; with v = y - c integrated over D, I = the integral of
+// |y - c|^2 dA, D's second moment of area about c). D's orientation: it closes the mesh, so its boundary
 // runs against the loop and a = -(integral of n dA). Loops whose bound exceeds beta, or that fail the ball tests, are
 // summed exactly instead.
 struct alignas(16) loop_rec
 {
     float cx, cy, cz, rho; // ball in the pole frame; rho grown by 0.1 % (+ 1e-6 |c|) so FP32 tests stay conservative
-    float ax, ay, az, k3;  // vector area; k3 = 3 rho^2 A (numerator of the remainder bound)
+    float ax, ay, az, k3;  // vector area; k3 = 3 I (numerator of the remainder bound; I <= rho^2 A, see pole_record)
     float m[9];            // first moment, pole frame
     float trm;             // its trace
     float cnorm;           // |c|
@@ -1156,11 +1157,16 @@ loop_host loop_data(float4 const* p, int64_t n)
         double const t[3] = {0.5 * (u[1] * w[2] - u[2] * w[1]), 0.5 * (u[2] * w[0] - u[0] * w[2]),
                              0.5 * (u[0] * w[1] - u[1] * w[0])}; // n dA of the fan triangle (c, p_i, p_i+1)
         L.rho = std::max(L.rho, std::sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]));
-        L.area += std::sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+        double const at = std::sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+        L.area += at;
+        // over the triangle (0, u, w): the integral of |y|^2 dA = at (|u|^2 + |w|^2 + u.w) / 6, of y dA = at (u + w) / 3
+        L.inertia += at * (u[0] * u[0] + u[1] * u[1] + u[2] * u[2] + w[0] * w[0] + w[1] * w[1] + w[2] * w[2] +
+                           u[0] * w[0] + u[1] * w[1] + u[2] * w[2]) / 6.0;
         for (int r = 0; r < 3; ++r)
         {
             L.a[r] += t[r];
             double const g = (u[r] + w[r]) / 3.0; // triangle centroid - c
+            L.s[r] += at * g;
             for (int k = 0; k < 3; ++k)
                 L.m[3 * r + k] += g * t[k];
         }
@@ -1311,18 +1317,26 @@ void loop_tree(std::vector<loop_host> const& loops, std::vector<tree_node>& node
                 C.c[k] = A.c[k] + t * dv[k];
             C.rho = 0.5 * (D + A.rho + B.rho);
         }
+        double slack = 0.0; // the magnitudes summed into the second moment (its rounding is a tiny fraction of them)
         for (loop_host const* K : {&A, &B}) // guard the rounding of the centre
         {
             double const dc[3] = {K->c[0] - C.c[0], K->c[1] - C.c[1], K->c[2] - C.c[2]};
-            C.rho = std::max(C.rho, std::sqrt(dc[0] * dc[0] + dc[1] * dc[1] + dc[2] * dc[2]) + K->rho);
+            double const dc2 = dc[0] * dc[0] + dc[1] * dc[1] + dc[2] * dc[2];
+            double const ds = dc[0] * K->s[0] + dc[1] * K->s[1] + dc[2] * K->s[2];
+            C.rho = std::max(C.rho, std::sqrt(dc2) + K->rho);
             C.area += K->area;
+            // the child's fans about the new centre: I + 2 dc.s + |dc|^2 area, s + area dc
+            C.inertia += K->inertia + 2.0 * ds + dc2 * K->area;
+            slack += K->inertia + 2.0 * std::fabs(ds) + dc2 * K->area;
             for (int a = 0; a < 3; ++a)
             {
                 C.a[a] += K->a[a];
+                C.s[a] += K->s[a] + K->area * dc[a];
                 for (int j = 0; j < 3; ++j)
                     C.m[3 * a + j] += K->m[3 * a + j] + dc[a] * K->a[j];
             }
         }
+        C.inertia = std::max(C.inertia, 0.0) + 1e-12 * slack;
         nodes[size_t(me)].L = C;
         nodes[size_t(me)].loop = -1;
         nodes[size_t(me)].skip = int32_t(nodes.size());
@@ -1513,7 +1527,10 @@ loop_rec pole_record(frame const& R, double sc, loop_host const& L, int64_t star
     r.ax = float(a[0] * sc2);
     r.ay = float(a[1] * sc2);
     r.az = float(a[2] * sc2);
-    r.k3 = float(3.0 * (rho * sc) * (rho * sc) * (L.area * sc2) * 1.001);
+    // remainder bound numerator 3 I: the second-order Taylor term of the kernel at y is at most 3 |y - c|^2 / gap^4,
+    // integrated over the fans (I <= rho^2 area: the smaller of the two, both valid)
+    double const inertia = std::min(L.inertia, L.rho * L.rho * L.area);
+    r.k3 = float(3.0 * inertia * sc2 * sc2 * 1.001);
     for (int i = 0; i < 9; ++i)
         r.m[i] = float(mr[i] * sc3);
     r.trm = float((mr[0] + mr[4] + mr[8]) * sc3);

@@ -273,6 +273,8 @@ struct loop_host
     double a[3];
     double area;
     double m[9];
+    double inertia; // second moment of area about c: the integral of |y - c|^2 over the fan(s), <= rho^2 area
+    double s[3];    // first moment of area about c: the integral of y - c (for moving the second moment)
     int64_t start, count; // decision-stream entries [start, start + count)
 };
 struct frac_plan

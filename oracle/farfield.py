@@ -8,9 +8,11 @@ sum (Oracle.frac, which follows math/common.hh:181-189 and gwn_mesh.hh:34-46 and
 A closed loop L (points P_0..P_m, P_m = P_0) spanned by the fan D from its centroid c lies in the ball (c, rho). For a
 query q outside the ball such that the line through q along x0 misses the ball, the loop's fractional term is
     frac_L(q) = -Omega_D(q) / 4 pi,   -Omega_D(q) = a.u / d^2 + (tr M - 3 u.M.u) / d^3 + R,
-    |R| <= 3 rho^2 A / (d - rho)^4,
-with x = c - q, d = |x|, u = x / d, a = the loop's vector area (1/2) sum u_i x u_i+1 (u_i = P_i - c), A the fan's area
-and M = sum over fan triangles of (centroid - c) (x) (n dA). The expansion and the bound hold for any surface inside the
+    |R| <= 3 I / (d - rho)^4 <= 3 rho^2 A / (d - rho)^4,
+with x = c - q, d = |x|, u = x / d, a = the loop's vector area (1/2) sum u_i x u_i+1 (u_i = P_i - c), A the fan's area,
+M = sum over fan triangles of (centroid - c) (x) (n dA) and I = the integral of |y - c|^2 dA over the fan (the
+second-order Taylor term of the kernel at y is at most 3 |y - c|^2 / (d - rho)^4; over a triangle (0, u, w) the
+integral of |y|^2 dA is area (|u|^2 + |w|^2 + u.w) / 6). The expansion and the bound hold for any surface inside the
 ball, so a group of loops (cluster_table: the union of their fans, moments about a common centre) is estimated the same
 way.
 """
@@ -74,13 +76,23 @@ def loop_table(points: np.ndarray) -> dict:
     w = p[1:] - c
     t = 0.5 * np.cross(u, w)
     g = (u + w) / 3.0
+    at = np.sqrt((t * t).sum(axis=1))
     return {
         "c": c,
         "rho": float(np.sqrt((u * u).sum(axis=1)).max()),
         "a": t.sum(axis=0),
-        "area": float(np.sqrt((t * t).sum(axis=1)).sum()),
+        "area": float(at.sum()),
         "m": (g[:, :, None] * t[:, None, :]).sum(axis=0),
+        "inertia": float((at * ((u * u).sum(axis=1) + (w * w).sum(axis=1) + (u * w).sum(axis=1)) / 6.0).sum()),
+        "s": (at[:, None] * g).sum(axis=0),
     }
+
+
+def shifted(t: dict, c: np.ndarray) -> tuple:
+    """(second moment, first moment) of a record's fans about another centre c: I + 2 dc.s + |dc|^2 A, s + A dc with
+    dc = c_t - c (the library's rule for its loop-tree nodes)."""
+    dc = t["c"] - c
+    return t["inertia"] + 2.0 * float(dc @ t["s"]) + float(dc @ dc) * t["area"], t["s"] + t["area"] * dc
 
 
 def cluster_table(tabs: list, shift: bool = True) -> dict:
@@ -91,14 +103,20 @@ def cluster_table(tabs: list, shift: bool = True) -> dict:
     so that tests can plant the mistake."""
     c = np.mean([t["c"] for t in tabs], axis=0)
     m = np.zeros((3, 3))
+    inertia, s = 0.0, np.zeros(3)
     for t in tabs:
         m += t["m"] + (np.outer(t["c"] - c, t["a"]) if shift else 0.0)
+        i_t, s_t = shifted(t, c)
+        inertia += i_t
+        s += s_t
     return {
         "c": c,
         "rho": float(max(np.linalg.norm(t["c"] - c) + t["rho"] for t in tabs)),
         "a": np.sum([t["a"] for t in tabs], axis=0),
         "area": float(sum(t["area"] for t in tabs)),
         "m": m,
+        "inertia": inertia,
+        "s": s,
     }
 
 
@@ -115,10 +133,15 @@ def merge_tables(ta: dict, tb: dict) -> dict:
         c = ta["c"] + (dist + tb["rho"] - ta["rho"]) / (2.0 * dist) * dv
         rho = 0.5 * (dist + ta["rho"] + tb["rho"])
     m = np.zeros((3, 3))
+    inertia, s = 0.0, np.zeros(3)
     for t in (ta, tb):
         rho = max(rho, float(np.linalg.norm(t["c"] - c)) + t["rho"])
         m += t["m"] + np.outer(t["c"] - c, t["a"])
-    return {"c": c, "rho": rho, "a": ta["a"] + tb["a"], "area": ta["area"] + tb["area"], "m": m}
+        i_t, s_t = shifted(t, c)
+        inertia += i_t
+        s += s_t
+    return {"c": c, "rho": rho, "a": ta["a"] + tb["a"], "area": ta["area"] + tb["area"], "m": m, "inertia": inertia,
+            "s": s}
 
 
 def loop_segments(points: np.ndarray) -> np.ndarray:
@@ -145,5 +168,5 @@ def estimate(tab: dict, q: np.ndarray, sign: float = 1.0):
     m = tab["m"]
     mq = np.einsum("ni,ij,nj->n", u, m, u)
     om = sign * (u @ tab["a"]) / d ** 2 + (np.trace(m) - 3.0 * mq) / d ** 3
-    bound = 3.0 * tab["rho"] ** 2 * tab["area"] / (d - tab["rho"]) ** 4
+    bound = 3.0 * min(tab["inertia"], tab["rho"] ** 2 * tab["area"]) / (d - tab["rho"]) ** 4
     return om / (4.0 * np.pi), bound / (4.0 * np.pi)
