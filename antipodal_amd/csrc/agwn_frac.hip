@@ -755,10 +755,12 @@ __device__ __forceinline__ bool farfield_near(loop_rec const& r, float const ph[
     return false;
 }
 
-// An inner node of the loop tree is accepted only when its bound is below beta / 16 (a loop: beta). Accepted high in
-// the tree at the full beta, the summed bound grew until more points stayed undecided: the benchmark grid's step took
-// 118.8 ms that way, 112.4 ms at beta / 16 and the same within 0.2 ms from beta / 4 to beta / 32.
-constexpr float kInnerBeta = 1.0f / 16.0f;
+// An inner node of the loop tree is accepted only when its bound is below beta / 8 (a loop: beta). Accepted high in
+// the tree at the full beta, the summed bound grew until more points stayed undecided (with the rho^2 A bound the
+// benchmark grid's step took 118.8 ms that way against 112.4 ms at beta / 16). With the second-moment bound the exact
+// stage of that step took 27.4 ms at beta / 16, 26.7 at beta / 8, 26.8 at beta / 4; beta itself at 0.001 / 0.002 /
+// 0.004: 28.4 / 27.4 / 27.3 ms.
+constexpr float kInnerBeta = 1.0f / 8.0f;
 
 // One lane per query (FP32 positions in the caller's frame, counts = integer terms). The wave walks the loop tree in
 // depth-first order without a stack: a lane whose far-field test passes at node k takes that node's estimate and
