@@ -76,6 +76,31 @@ __device__ __forceinline__ double sqrt_f64(double q)
     return qs > 0.0 ? g : 0.0;
 }
 
+// sqrt_f64 for q = 0 or q >= 2^-1000 (a squared length of integer coordinates, say): the same steps without the
+// scaling of tiny arguments.
+__device__ __forceinline__ double sqrt_f64_normal(double q)
+{
+    double const r = __builtin_amdgcn_rsq(q);
+    double g = q * r, h = 0.5 * r;
+    double e = __builtin_fma(-g, h, 0.5);
+    g = __builtin_fma(g, e, g);
+    h = __builtin_fma(h, e, h);
+    e = __builtin_fma(-g, g, q); // residual
+    g = __builtin_fma(e, h, g);
+    return q > 0.0 ? g : 0.0;
+}
+
+// a / b in FP64 for finite a and b whose magnitudes and quotient stay well inside the normal range (no scaling steps),
+// within ~1 ulp: the hardware reciprocal refined by two Newton steps, then one residual correction of the quotient.
+__device__ __forceinline__ double div_f64_normal(double a, double b)
+{
+    double r = __builtin_amdgcn_rcp(b);
+    r = __builtin_fma(__builtin_fma(-b, r, 1.0), r, r);
+    r = __builtin_fma(__builtin_fma(-b, r, 1.0), r, r);
+    double const q = a * r;
+    return __builtin_fma(__builtin_fma(-b, q, a), r, q);
+}
+
 // a + b = s + e exactly (Knuth's two-sum)
 __device__ __forceinline__ void two_sum_f64(double a, double b, double& s, double& e)
 {

@@ -23,6 +23,8 @@
 #include <cfloat>
 #include <cstring>
 #include <memory>
+#include <type_traits>
+#include <unordered_map>
 #include <vector>
 
 namespace agwn
@@ -407,6 +409,28 @@ __global__ void __launch_bounds__(kTB)
 }
 
 // --- fractional term (gwn_mesh_robust.hh:264-325) -------------------------------------------------------------------
+// The boundary as a chain stream. Stored segments (p0, p1) are linked where one's p1 is the next one's p0 (bit for bit),
+// so consecutive stream points are the endpoints of a segment and each point's per-query quantities (v = P - q,
+// l = |v|, l + v.x) are computed once for the two segments that share it. seg = 1: the segment (previous point, this
+// point) of weight w ends here; seg = 0: a walk starts here. The reference sums the segments' terms in input order;
+// the stream sums the same terms in walk order.
+template <class C>
+struct rpt
+{
+    v3<C> p;
+    int32_t seg;
+    double w;
+};
+
+// one stream point's per-query quantities
+template <class C>
+struct rpoint
+{
+    v3<C> p, v;          // the point and P - q, in predicate coordinates
+    double dx, dy, dz;   // v in FP64
+    double lpx;          // l + v.x without cancellation (l_plus_x)
+};
+
 template <class C>
 __device__ inline double l_plus_x(v3<C> v, double l)
 {
@@ -416,45 +440,86 @@ __device__ inline double l_plus_x(v3<C> v, double l)
     return dot_yz(v, v) / (l - double(v.x));
 }
 
-// One segment's term, in the reference's operations and their order: (num, den) of its atan2 (the generic case,
-// num != 0: returns false), or the term itself in `special` (num == 0: the ray through an endpoint, grazing: returns
-// true). The reference's den cancels for a query next to the segment, so its rounding is kept as is.
-template <class C>
-__device__ inline bool rpair(v3<C> qi, rseg<C> const& g, double& num, double& den, double& special)
+// F = false: the reference's operations (gwn_mesh_robust.hh:264-290). F = true (int64_grid only): the same values in
+// FP64. Mesh coordinates are quantized to |c| < 2^20 and a query is clamped to 2^30; for a query with |q| <= 2^25
+// every coordinate difference is below 2^25 + 2^20 < 2^25.1 in magnitude, so every product of two differences is
+// below 2^50.2, every sum of two products below 2^51.2 and a squared length below 2^51.8: integers that double
+// represents exactly. The 64-bit integer numerator, squared lengths and dot products of the reference
+// (robust_predicates.hh:339-375) then have the same values in FP64, and only sqrt and the division of l_plus_x round
+// (within ~1 ulp, where the reference's are correctly rounded).
+template <bool F, class C>
+__device__ inline rpoint<C> rpoint_at(v3<C> qi, v3<C> P)
 {
 #pragma clang fp contract(off)
-    v3<C> const v0{C(g.p0.x - qi.x), C(g.p0.y - qi.y), C(g.p0.z - qi.z)};
-    v3<C> const v1{C(g.p1.x - qi.x), C(g.p1.y - qi.y), C(g.p1.z - qi.z)};
-    double const d0x = double(v0.x), d0y = double(v0.y), d0z = double(v0.z);
-    double const d1x = double(v1.x), d1y = double(v1.y), d1z = double(v1.z);
-    double const l0 = sqrt(d0x * d0x + d0y * d0y + d0z * d0z);
-    double const l1 = sqrt(d1x * d1x + d1y * d1y + d1z * d1z);
-    wide<C> const wn = atan2_num(qi, g.p0, g.p1);
-    den = l_plus_x(v0, l0) * l_plus_x(v1, l1) + dot_yz(v0, v1);
-    if (wn != 0)
+    rpoint<C> r;
+    r.p = P;
+    r.v = v3<C>{C(P.x - qi.x), C(P.y - qi.y), C(P.z - qi.z)};
+    r.dx = double(r.v.x);
+    r.dy = double(r.v.y);
+    r.dz = double(r.v.z);
+    if constexpr (F)
     {
-        num = to_d(wn);
-        return false;
+        static_assert(std::is_same_v<C, int32_t>, "the FP64 form is int64_grid's");
+        double const yz = __builtin_fma(r.dy, r.dy, r.dz * r.dz);
+        double const l = sqrt_f64_normal(__builtin_fma(r.dx, r.dx, yz));
+        r.lpx = r.dx >= 0.0 ? l + r.dx : div_f64_normal(yz, l - r.dx);
     }
-    if (v0.y == 0 && v0.z == 0 && v0.x < 0 && (v1.y != 0 || v1.z != 0))
-        special = v1.z != 0 ? atan2_f64(double(v1.z), -double(v1.y)) : (v1.y > 0 ? -kPi : 0.0);
-    else if (v1.y == 0 && v1.z == 0 && v1.x < 0 && (v0.y != 0 || v0.z != 0))
-        special = v0.z != 0 ? atan2_f64(-double(v0.z), -double(v0.y)) : (v0.y > 0 ? kPi : 0.0);
     else
-        special = den < 0.0 ? double(-edge_sign(qi, g.p0, g.p1)) * kPi : 0.0;
+    {
+        double const l = sqrt(r.dx * r.dx + r.dy * r.dy + r.dz * r.dz);
+        r.lpx = l_plus_x(r.v, l);
+    }
+    return r;
+}
+
+// The term of the segment (a, b): (num, den) of its atan2 (the generic case, num != 0: returns false), or the term
+// itself in `special` (num == 0: the ray through an endpoint, grazing: returns true). The reference's den cancels for
+// a query next to the segment, so its rounding is kept as is.
+template <bool F, class C>
+__device__ inline bool rterm(v3<C> qi, rpoint<C> const& a, rpoint<C> const& b, double& num, double& den,
+                             double& special)
+{
+#pragma clang fp contract(off)
+    bool nonzero;
+    if constexpr (F)
+    {
+        den = a.lpx * b.lpx + __builtin_fma(a.dy, b.dy, a.dz * b.dz);
+        num = __builtin_fma(a.dz, b.dy, -(a.dy * b.dz));
+        nonzero = num != 0.0;
+    }
+    else
+    {
+        wide<C> const wn = atan2_num(qi, a.p, b.p);
+        den = a.lpx * b.lpx + dot_yz(a.v, b.v);
+        nonzero = wn != 0;
+        if (nonzero)
+            num = to_d(wn);
+    }
+    if (nonzero)
+        return false;
+    v3<C> const v0 = a.v, v1 = b.v;
+    if (v0.y == 0 && v0.z == 0 && v0.x < 0 && (v1.y != 0 || v1.z != 0))
+        special = v1.z != 0 ? atan2_f64(b.dz, -b.dy) : (v1.y > 0 ? -kPi : 0.0);
+    else if (v1.y == 0 && v1.z == 0 && v1.x < 0 && (v0.y != 0 || v0.z != 0))
+        special = v0.z != 0 ? atan2_f64(-a.dz, -a.dy) : (v0.y > 0 ? kPi : 0.0);
+    else
+        special = den < 0.0 ? double(-edge_sign(qi, a.p, b.p)) * kPi : 0.0;
     return true;
 }
 
-// Segments two at a time: two generic terms of equal weight whose den are positive and whose magnitudes lie in
-// [2^-400, 2^400] share one atan2 (the argument of the product of their complex numbers is the exact sum of two
-// angles in (-pi/2, pi/2)); any other term takes its own. The arctangent is atan2_f64 (agwn_math.hpp), within 2 ulp
-// of the library's.
+// One lane per query walks the chain stream, staged through LDS a tile at a time (broadcast reads). Consecutive
+// segments pair up: two generic terms of equal weight whose den are positive and whose magnitudes lie in
+// [2^-400, 2^400] share one atan2 (the argument of the product of their complex numbers is the exact sum of two angles
+// in (-pi/2, pi/2)); any other term takes its own. The arctangent is atan2_f64 (agwn_math.hpp), within 2 ulp of the
+// library's. int64_grid takes the FP64 form when every query of the workgroup lies within 2^25 (the choice is
+// uniform over the workgroup, whose waves share the LDS tiles).
 template <class C, class T>
 __global__ void __launch_bounds__(kB)
-    k_rfrac(T const* __restrict__ pos, int64_t n, quant Q, rseg<C> const* __restrict__ seg, int64_t nseg,
+    k_rfrac(T const* __restrict__ pos, int64_t n, quant Q, rpt<C> const* __restrict__ pts, int64_t npts,
             double f_norm, int32_t const* __restrict__ counts, T* __restrict__ out_frac, T* __restrict__ out_eval)
 {
 #pragma clang fp contract(off)
+    __shared__ rpt<C> tile[kB];
     int64_t const i = int64_t(blockIdx.x) * kB + threadIdx.x;
     double x = 0.0, y = 0.0, z = 0.0;
     if (i < n)
@@ -467,27 +532,72 @@ __global__ void __launch_bounds__(kB)
     constexpr double kLo = 0x1p-400, kHi = 0x1p400;
     auto single = [](bool sp, double nu, double de, double spv) { return sp ? spv : atan2_f64(nu, de); };
     double half_area = 0.0;
-    int64_t s = 0;
-    for (; s + 1 < nseg; s += 2)
-    {
-        rseg<C> const g0 = seg[s], g1 = seg[s + 1];
-        double n0 = 0.0, d0 = 0.0, s0 = 0.0, n1 = 0.0, d1 = 0.0, s1 = 0.0;
-        bool const sp0 = rpair(qi, g0, n0, d0, s0), sp1 = rpair(qi, g1, n1, d1, s1);
-        double const m0 = __builtin_fmax(__builtin_fabs(n0), __builtin_fabs(d0));
-        double const m1 = __builtin_fmax(__builtin_fabs(n1), __builtin_fabs(d1));
-        if (!sp0 && !sp1 && g0.w == g1.w && d0 > 0.0 && d1 > 0.0 && m0 >= kLo && m0 <= kHi && m1 >= kLo && m1 <= kHi)
-            half_area += g0.w * atan2_f64(__builtin_fma(n0, d1, n1 * d0), __builtin_fma(d0, d1, -(n0 * n1)));
-        else
+    auto walk = [&](auto f_tag) {
+        constexpr bool F = decltype(f_tag)::value;
+        rpoint<C> a{}; // the previous stream point
+        double pn = 0.0, pd = 0.0, ps = 0.0, pw = 0.0; // the pending term
+        bool psp = false, pending = false;               // pending is the same in every lane
+        for (int64_t t0 = 0; t0 < npts; t0 += kB)
         {
-            half_area += g0.w * single(sp0, n0, d0, s0);
-            half_area += g1.w * single(sp1, n1, d1, s1);
+            __syncthreads(); // the previous tile has been consumed by every wave
+            if (t0 + threadIdx.x < npts)
+                tile[threadIdx.x] = pts[t0 + threadIdx.x];
+            __syncthreads();
+            int const cnt = int(npts - t0 < kB ? npts - t0 : kB);
+            for (int j = 0; j < cnt; ++j)
+            {
+                rpt<C> const g = tile[j];
+                rpoint<C> const b = rpoint_at<F>(qi, g.p);
+                if (__builtin_amdgcn_readfirstlane(g.seg) != 0)
+                {
+                    double nu = 0.0, de = 0.0, sv = 0.0;
+                    bool const sp = rterm<F>(qi, a, b, nu, de, sv);
+                    double const w = g.w;
+                    if (pending)
+                    {
+                        // F: numerators are nonzero integers below 2^51.2 and den below 2^56, always in range
+                        bool in_range = true;
+                        if constexpr (!F)
+                        {
+                            double const m0 = __builtin_fmax(__builtin_fabs(pn), __builtin_fabs(pd));
+                            double const m1 = __builtin_fmax(__builtin_fabs(nu), __builtin_fabs(de));
+                            in_range = m0 >= kLo && m0 <= kHi && m1 >= kLo && m1 <= kHi;
+                        }
+                        if (!psp && !sp && pw == w && pd > 0.0 && de > 0.0 && in_range)
+                            half_area += w * atan2_f64(__builtin_fma(pn, de, nu * pd), __builtin_fma(pd, de, -(pn * nu)));
+                        else
+                        {
+                            half_area += pw * single(psp, pn, pd, ps);
+                            half_area += w * single(sp, nu, de, sv);
+                        }
+                    }
+                    else
+                    {
+                        pn = nu;
+                        pd = de;
+                        ps = sv;
+                        psp = sp;
+                        pw = w;
+                    }
+                    pending = !pending;
+                }
+                a = b;
+            }
         }
-    }
-    if (s < nseg)
+        if (pending)
+            half_area += pw * single(psp, pn, pd, ps);
+    };
+    if constexpr (std::is_same_v<C, int32_t>)
     {
-        double nu = 0.0, de = 0.0, spv = 0.0;
-        bool const sp = rpair(qi, seg[s], nu, de, spv);
-        half_area += seg[s].w * single(sp, nu, de, spv);
+        int32_t const qm = max(max(abs(qi.x), abs(qi.y)), abs(qi.z));
+        if (__syncthreads_and(i >= n || qm <= (1 << 25)))
+            walk(std::true_type{});
+        else
+            walk(std::false_type{});
+    }
+    else
+    {
+        walk(std::false_type{});
     }
     if (i < n)
     {
@@ -543,6 +653,69 @@ __global__ void __launch_bounds__(kB) k_rgather(v3<C> const* __restrict__ src, i
     for (int j = 0; j < 3; ++j)
         dst[3 * k + j] = src[3 * t + j];
 }
+// Links the quantized segments into walks for the chain stream: a walk starts at the first unused segment (input
+// order) and continues with an unused segment whose p0 equals its last p1 bit for bit, while there is one. Every
+// segment appears once; a closed loop becomes one walk of n + 1 points.
+template <class C>
+std::vector<rpt<C>> chain_stream(std::vector<rseg<C>> const& sg)
+{
+    struct key
+    {
+        uint64_t k[3];
+        bool operator==(key const& o) const { return k[0] == o.k[0] && k[1] == o.k[1] && k[2] == o.k[2]; }
+    };
+    struct key_hash
+    {
+        size_t operator()(key const& x) const
+        {
+            uint64_t h = 0x9e3779b97f4a7c15ull;
+            for (uint64_t v : x.k)
+                h = (h ^ v) * 0xff51afd7ed558ccdull, h ^= h >> 32;
+            return size_t(h);
+        }
+    };
+    auto key_of = [](v3<C> const& p) {
+        key r{};
+        C const c[3] = {p.x, p.y, p.z};
+        for (int j = 0; j < 3; ++j)
+            std::memcpy(&r.k[j], &c[j], sizeof(C));
+        return r;
+    };
+    int64_t const ns = int64_t(sg.size());
+    std::unordered_map<key, std::vector<int64_t>, key_hash> from; // segments by their p0, each list descending
+    from.reserve(size_t(ns));
+    for (int64_t k = ns - 1; k >= 0; --k)
+        from[key_of(sg[size_t(k)].p0)].push_back(k);
+    std::vector<char> used(size_t(ns), 0);
+    std::vector<rpt<C>> out;
+    out.reserve(size_t(ns) + 16);
+    for (int64_t k = 0; k < ns; ++k)
+    {
+        if (used[size_t(k)])
+            continue;
+        out.push_back(rpt<C>{sg[size_t(k)].p0, 0, 0.0});
+        int64_t cur = k;
+        while (cur >= 0)
+        {
+            used[size_t(cur)] = 1;
+            rseg<C> const& g = sg[size_t(cur)];
+            out.push_back(rpt<C>{g.p1, 1, g.w});
+            cur = -1;
+            auto it = from.find(key_of(g.p1));
+            if (it == from.end())
+                break;
+            std::vector<int64_t>& cand = it->second;
+            while (!cand.empty() && used[size_t(cand.back())])
+                cand.pop_back();
+            if (!cand.empty())
+            {
+                cur = cand.back();
+                cand.pop_back();
+            }
+        }
+    }
+    return out;
+}
 } // namespace
 
 // ---------------------------------------------------------------------------------------------------------------------
@@ -567,13 +740,14 @@ template <class C, class T>
 struct robust_t final : robust_impl
 {
     v3<C>* tri = nullptr;     // predicate triangles in LBVH leaf order
-    rseg<C>* seg = nullptr;   // swapped quantized boundary
+    rpt<C>* stream = nullptr; // the swapped quantized boundary as a chain stream
+    int64_t nstream = 0;
     bvh_node<double>* nodes = nullptr;
     lbvh_topology topo;
 
     ~robust_t() override
     {
-        for (void* p : {(void*)tri, (void*)seg, (void*)nodes})
+        for (void* p : {(void*)tri, (void*)stream, (void*)nodes})
             if (p)
                 (void)hipFree(p);
         lbvh_release(topo);
@@ -627,11 +801,21 @@ struct robust_t final : robust_impl
             double const max_coord = double((int64_t(1) << bits) - 1);
             Q.scale = half > 0.0 ? max_coord / half : 1.0;
         }
-        AGWN_HIP(hipMalloc(&seg, size_t(std::max<int64_t>(ns, 1)) * sizeof(rseg<C>)));
         if (ns > 0)
         {
-            k_rquant_seg<C, T><<<dim3(unsigned((ns + kB - 1) / kB)), dim3(kB), 0, s>>>(ds.as<T>(), ns, Q, seg);
+            dbuf dq;
+            AGWN_TRY(dq.alloc(size_t(ns) * sizeof(rseg<C>), s));
+            k_rquant_seg<C, T><<<dim3(unsigned((ns + kB - 1) / kB)), dim3(kB), 0, s>>>(ds.as<T>(), ns, Q,
+                                                                                      dq.as<rseg<C>>());
             AGWN_HIP(hipGetLastError());
+            std::vector<rseg<C>> hq(static_cast<size_t>(ns));
+            AGWN_HIP(hipMemcpyAsync(hq.data(), dq.p, size_t(ns) * sizeof(rseg<C>), hipMemcpyDeviceToHost, s));
+            AGWN_HIP(hipStreamSynchronize(s));
+            std::vector<rpt<C>> const hs = chain_stream(hq);
+            nstream = int64_t(hs.size());
+            AGWN_HIP(hipMalloc(&stream, hs.size() * sizeof(rpt<C>)));
+            AGWN_HIP(hipMemcpyAsync(stream, hs.data(), hs.size() * sizeof(rpt<C>), hipMemcpyHostToDevice, s));
+            AGWN_HIP(hipStreamSynchronize(s));
         }
         if (ntri > 0)
         {
@@ -690,7 +874,7 @@ struct robust_t final : robust_impl
         int64_t const blocks = (n + kB - 1) / kB;
         AGWN_CHECK(blocks < (int64_t(1) << 31), "too many queries (%lld)", (long long)n);
         constexpr double f_norm = 2.0 / (4.0 * kPi);
-        k_rfrac<C, T><<<dim3(unsigned(blocks)), dim3(kB), 0, s>>>(static_cast<T const*>(pos), n, Q, seg, nseg, f_norm,
+        k_rfrac<C, T><<<dim3(unsigned(blocks)), dim3(kB), 0, s>>>(static_cast<T const*>(pos), n, Q, stream, nstream, f_norm,
                                                                   counts, static_cast<T*>(out_frac),
                                                                   static_cast<T*>(out_eval));
         AGWN_HIP(hipGetLastError());

@@ -107,6 +107,26 @@ def test_torus_subsample(gpu, oracle, dt, pol):
     _check_parity(got, ref, dt)
 
 
+@pytest.mark.parametrize("dt", ["float32", "float64"])
+def test_int64_grid_fp64_form_at_its_limit(gpu, oracle, dt):
+    """int64_grid computes a wave's determinants, dot products and squared lengths in FP64 when every quantized query
+    of the wave lies within 2^25 in each coordinate (mesh coordinates lie within 2^20, so every such value is an
+    integer below 2^53); other waves take the 64-bit integer form. 64 queries just inside the limit (31.9 mesh
+    half-extents from the centre along the max norm, the largest exact products) and 64 just outside (32.1) fill one
+    wave each; both equal the robust restatement: counts bit-exact, fractional to the policy's bar."""
+    v, idx = mg.icosphere_cap_mesh()
+    bnd = oracle.boundary(v, idx)
+    lo, hi = v.min(axis=0), v.max(axis=0)
+    centre, half = 0.5 * (lo + hi), 0.5 * float(np.max(hi - lo))
+    dirs = mg.normal(3, 31, 3 * 128).reshape(128, 3)
+    dirs /= np.abs(dirs).max(axis=1, keepdims=True)
+    radius = np.concatenate([np.full(64, 31.9), np.full(64, 32.1)]) * half
+    p = centre + dirs * radius[:, None]
+    print(f"case robust int64_grid limit {dt}", flush=True)
+    got, ref = _both(gpu, oracle, v, idx, bnd, p, "int64_grid", dt)
+    _check_parity(got, ref, dt)
+
+
 def test_empty_inputs(gpu):
     with gpu.RobustMeshGwn(np.zeros((0, 3)), np.zeros(0, dtype=np.int32), np.zeros((0, 7))) as r:
         p = mg.uniform_points(1, 100, [-1] * 3, [1] * 3)
