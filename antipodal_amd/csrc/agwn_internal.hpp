@@ -241,12 +241,18 @@ __device__ __forceinline__ void query_at(src_array<T> const& s, int64_t i, T& x,
     z = s.pos[3 * i + 2];
 }
 
+// lattice cell (ix, iy, iz) of flat index i
+__device__ __forceinline__ void grid_cell(grid_desc const& g, int64_t i, int64_t& ix, int64_t& iy, int64_t& iz)
+{
+    int64_t const t = div_recip(i, g.nx, g.inx, ix);
+    iz = div_recip(t, g.ny, g.iny, iy);
+}
+
 __device__ __forceinline__ void query_at(src_grid const& s, int64_t i, float& x, float& y, float& z)
 {
 #pragma clang fp contract(off)
-    int64_t ix, iy;
-    int64_t const t = div_recip(i, s.g.nx, s.g.inx, ix);
-    int64_t const iz = div_recip(t, s.g.ny, s.g.iny, iy);
+    int64_t ix, iy, iz;
+    grid_cell(s.g, i, ix, iy, iz);
     float const fx = float(ix), fy = float(iy), fz = float(iz);
     x = ((s.g.start[0] + s.g.dx[0] * fx) + s.g.dy[0] * fy) + s.g.dz[0] * fz;
     y = ((s.g.start[1] + s.g.dx[1] * fx) + s.g.dy[1] * fy) + s.g.dz[1] * fz;
@@ -357,9 +363,16 @@ int frac_f64_ws(Src const& src, int64_t n, double const* seg7, int64_t nseg, dou
 //                  [3] (list length); counters (5 of them) start zeroed
 //   gather_routed  pts[e] = the query of list entry e (the coordinates the exact kernels would compute)
 //   route_apply    labels of list entries from their exact values w; audit disagreements counted in counters[4]
+// lat: for a lattice source, device room for mlp_lattice_table_floats(g) floats (null, or a lattice whose layer 0 does
+// not split by axis: the per-point features); ignored for arrays
 template <class Src>
 int mlp_route(agwn_mlp const* mlp, Src const& src, int64_t n, float delta, uint32_t audit_mod, uint32_t audit_seed,
-              int8_t* labels, int64_t* list, unsigned long long* counters, hipStream_t s);
+              int8_t* labels, int64_t* list, unsigned long long* counters, hipStream_t s, float* lat = nullptr);
+// floats of the layer-0 table of an axis-separable lattice (0: the lattice's layer 0 does not split by axis, or its
+// table would exceed 2^16 rows)
+int64_t mlp_lattice_table_floats(grid_desc const& g);
+// network outputs of the n = nx ny nz lattice points, in the lattice's flat order
+int mlp_predict_grid_dev(agwn_mlp const* mlp, grid_desc const& g, float* out, hipStream_t s);
 template <class Src>
 int gather_routed(Src const& src, int64_t const* list, int64_t m, float* pts, hipStream_t s);
 //   route_order    orders the m list entries of n queries: *by_index = the entries in index order (list or tmp);

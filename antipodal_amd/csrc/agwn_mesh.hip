@@ -22,6 +22,7 @@
 #include <cstring>
 #include <memory>
 #include <mutex>
+#include <type_traits>
 
 namespace agwn
 {
@@ -1316,9 +1317,16 @@ int label_surrogate(mesh_t<float>& mesh, agwn_mlp const* mlp, Src const& src, in
         int64_t* list = list_ws;
         if (!list)
             AGWN_TRY(ws.take_n(size_t(n), &list));
+        float* lat = nullptr; // layer-0 table of an axis-separable lattice
+        if constexpr (std::is_same_v<Src, src_grid>)
+        {
+            int64_t const tf = mlp_lattice_table_floats(src.g);
+            if (tf > 0)
+                AGWN_TRY(ws.take_n(size_t(tf), &lat));
+        }
         AGWN_HIP(hipMemsetAsync(dc, 0, sizeof(cnt), s));
         AGWN_TRY(mlp_route(mlp, src, n, delta, uint32_t(audit_every), uint32_t(audit_seed ^ (audit_seed >> 32)), labels,
-                           list, dc, s));
+                           list, dc, s, lat));
         AGWN_HIP(hipMemcpyAsync(cnt, dc, sizeof(cnt), hipMemcpyDeviceToHost, s));
         AGWN_HIP(hipStreamSynchronize(s));
         int64_t const m = int64_t(cnt[3]);
@@ -1488,6 +1496,15 @@ int agwn_grid_label_surrogate_dev(agwn_grid* g, agwn_mlp const* mlp, float const
     return guarded2([&] {
         return grid_label_dev(g, mlp, start, dx, dy, dz, nx, ny, nz, delta, audit_every, audit_seed, labels, stats,
                               hipStream_t(s));
+    });
+}
+int agwn_mlp_predict_grid_dev(agwn_mlp const* mlp, float const* start, float const* dx, float const* dy,
+                              float const* dz, int nx, int ny, int nz, float* out, agwn_stream s)
+{
+    return guarded2([&]() -> int {
+        grid_call c;
+        AGWN_TRY(grid_setup(start, dx, dy, dz, nx, ny, nz, c));
+        return mlp_predict_grid_dev(mlp, c.gd, out, hipStream_t(s));
     });
 }
 int agwn_label_surrogate_dev(agwn_mesh const* m, agwn_mlp const* mlp, float const* pos, int64_t n, float const* x0,

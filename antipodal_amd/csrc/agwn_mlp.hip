@@ -28,6 +28,7 @@
 #include <memory>
 #include <mutex>
 #include <new>
+#include <type_traits>
 #include <vector>
 
 namespace agwn
@@ -608,9 +609,65 @@ __device__ __forceinline__ uint32_t hash32(uint64_t x)
     return uint32_t(x);
 }
 
-template <class Src, bool ROUTE>
+// Layer 0 of an axis-separable lattice. When every coordinate of the lattice point start + dx ix + dy iy + dz iz varies
+// along one lattice axis at most (each of the three coordinates has at most one nonzero step among dx, dy, dz -- the
+// axis-aligned grids, in any axis order), the first layer's input splits by lattice axis: a coordinate's 64 features
+// depend on one index only, so
+//   z_0(ix, iy, iz) = b_0 + sum_k W_0[k] f_k = T_0[ix] + (T_1[iy] + T_2[iz])
+// with T_s[t] the 256-unit sum over the features of the coordinates that vary along lattice axis s, at index t
+// (coordinates that vary along none go into T_0, the bias into T_2). The table holds nx + ny + nz rows of 256 FP32
+// units (axis s's rows from off[s]); the point's features, the BF16 weights and their exact FP32 products are those
+// of the per-point evaluation, only the order of the FP32 sums differs.
+struct lattice_map
+{
+    int32_t dep[3];  // lattice axis along which coordinate a varies, -1: along none
+    int64_t off[3];  // first row of lattice axis s: 0, nx, nx + ny
+};
+constexpr int64_t kMaxLatticeRows = int64_t(1) << 16; // 64 MiB of table at most
+
+__global__ void __launch_bounds__(kH) k_lattice_table(grid_desc const g, feat_params F, lattice_map L,
+                                                      __bf16 const* __restrict__ wr0, float const* __restrict__ b0,
+                                                      float* __restrict__ T)
+{
+#pragma clang fp contract(off)
+    __shared__ float f[3][2 * kFreq];
+    int64_t const r = blockIdx.x;
+    int const s = r < L.off[1] ? 0 : (r < L.off[2] ? 1 : 2);
+    int64_t const t = r - L.off[s];
+    int const n = threadIdx.x;
+    bool use[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+        use[a] = L.dep[a] == s || (L.dep[a] < 0 && s == 0);
+    if (n < 3 * kFreq)
+    {
+        int const a = n / kFreq, k = n % kFreq;
+        if (use[a])
+        {
+            // the coordinate as query_at forms it: the steps of the other lattice axes are zero for this coordinate,
+            // so their terms are exactly zero at every index
+            float const ft = float(t);
+            float const fi0 = s == 0 ? ft : 0.0f, fi1 = s == 1 ? ft : 0.0f, fi2 = s == 2 ? ft : 0.0f;
+            float const p = ((g.start[a] + g.dx[a] * fi0) + g.dy[a] * fi1) + g.dz[a] * fi2;
+            float const u = (p - F.c[a]) * F.inv_h[a];
+            float const tt = 0.5f * (F.w[k] * u);
+            f[a][2 * k] = float(__bf16(__builtin_amdgcn_sinf(tt)));
+            f[a][2 * k + 1] = float(__bf16(__builtin_amdgcn_cosf(tt)));
+        }
+    }
+    __syncthreads();
+    float acc = s == 2 ? b0[n] : 0.0f;
+    for (int a = 0; a < 3; ++a)
+        if (use[a])
+            for (int q = 0; q < 2 * kFreq; ++q)
+                acc = fmaf(f[a][q], float(wr0[int64_t(2 * kFreq * a + q) * kH + n]), acc);
+    T[r * kH + n] = acc;
+}
+
+template <class Src, bool ROUTE, bool LAT = false>
 __global__ void __launch_bounds__(256, kInferBlocks) k_mlp_infer(Src const src, int64_t n, feat_params F, infer_params P,
-                                                      float* __restrict__ pred, route_params RP)
+                                                      float* __restrict__ pred, route_params RP,
+                                                      float const* __restrict__ lat)
 {
     __shared__ __attribute__((aligned(16))) __bf16 act[kTileP * kLdsStride];
     __shared__ float predw[4][kTileP]; // per wave: the dot over its 64 units
@@ -640,7 +697,8 @@ __global__ void __launch_bounds__(256, kInferBlocks) k_mlp_infer(Src const src, 
     // axis and the frequencies -- is the same across the wave and the frequencies come from scalar registers; lane
     // (li, hl) computes point li of the chunk at frequencies 4 (g % 8) + 2 hl + {0, 1}: 2 pairs, one 8-byte store.
     // Same operations as the training features (feature_pair); points past n get the features of the origin, which
-    // nothing reads.
+    // nothing reads. (Lattice tables: no features; layer 0 is a sum of three table rows.)
+    if constexpr (!LAT)
     {
         float u[kTileJ][3];
 #pragma unroll
@@ -673,8 +731,8 @@ __global__ void __launch_bounds__(256, kInferBlocks) k_mlp_infer(Src const src, 
                 *reinterpret_cast<bf16x4*>(act + (32 * c + li) * kLdsStride + 2 * (32 * a + k0 + 2 * hl)) = v;
             }
         }
+        __syncthreads();
     }
-    __syncthreads();
 
     int const kh = hl * 8;
     int const n0 = wave * 64;
@@ -752,11 +810,8 @@ __global__ void __launch_bounds__(256, kInferBlocks) k_mlp_infer(Src const src, 
             __builtin_amdgcn_sched_barrier(0);
         }
     };
-#pragma unroll 1
-    for (int l = 0; l + 1 < kL; ++l)
-    {
-        layer(l);
-        __syncthreads(); // every wave has read the layer input
+    // hidden layer l's activations -> LDS (the next layer's B operand)
+    auto store_act = [&]() __attribute__((always_inline)) {
 #pragma unroll
         for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -775,6 +830,73 @@ __global__ void __launch_bounds__(256, kInferBlocks) k_mlp_infer(Src const src, 
                 }
             }
         __syncthreads();
+    };
+    if constexpr (LAT)
+    {
+        // layer 0 from the tables: z_0 = T_0[ix] + (T_1[iy] + T_2[iz]) (bias in T_2), activated and stored as
+        // store_act does (the accumulators stay free). Wave w fills the LDS rows of points 32w .. 32w + 31, one point
+        // per step, lane l taking units 4l .. 4l + 3: every table row is read whole (1 KiB per wave load). Within a
+        // wave the 32 points usually share (iy, iz) -- one lattice row -- and then T_1 + T_2 is read once. Points
+        // past n take cell (0, 0, 0), which nothing reads.
+        src_grid const& gs = src;
+        float const* T1 = lat + gs.g.nx * kH;
+        float const* T2 = T1 + gs.g.ny * kH;
+        int32_t cx = 0, cy = 0, cz = 0; // lane l < 32: the cell of point 32 w + l
+        {
+            int64_t const i = base + 32 * wave + li;
+            if (i < n)
+            {
+                int64_t ix, iy, iz;
+                grid_cell(gs.g, i, ix, iy, iz);
+                cx = int32_t(ix);
+                cy = int32_t(iy);
+                cz = int32_t(iz);
+            }
+        }
+        int const y0 = __builtin_amdgcn_readfirstlane(cy), z0 = __builtin_amdgcn_readfirstlane(cz);
+        bool const row = __ballot(cy != y0 || cz != z0) == 0; // uniform across the wave
+        __bf16* dst = act + (32 * wave) * kLdsStride + 4 * lane;
+        auto put = [&](int p, float4 const a, float4 const bc) __attribute__((always_inline)) {
+            bf16x4 hv;
+            hv[0] = __bf16(hsilu(a.x + bc.x));
+            hv[1] = __bf16(hsilu(a.y + bc.y));
+            hv[2] = __bf16(hsilu(a.z + bc.z));
+            hv[3] = __bf16(hsilu(a.w + bc.w));
+            *reinterpret_cast<bf16x4*>(dst + p * kLdsStride) = hv;
+        };
+        auto rowsum = [&](int iy, int iz) __attribute__((always_inline)) {
+            float4 const b = *reinterpret_cast<float4 const*>(T1 + int64_t(iy) * kH + 4 * lane);
+            float4 const c = *reinterpret_cast<float4 const*>(T2 + int64_t(iz) * kH + 4 * lane);
+            return float4{b.x + c.x, b.y + c.y, b.z + c.z, b.w + c.w};
+        };
+        if (row)
+        {
+            float4 const bc = rowsum(y0, z0);
+#pragma unroll 8
+            for (int p = 0; p < 32; ++p)
+            {
+                int const ix = __builtin_amdgcn_readlane(cx, p);
+                put(p, *reinterpret_cast<float4 const*>(lat + int64_t(ix) * kH + 4 * lane), bc);
+            }
+        }
+        else
+        {
+#pragma unroll 4
+            for (int p = 0; p < 32; ++p)
+            {
+                int const ix = __builtin_amdgcn_readlane(cx, p);
+                int const iy = __builtin_amdgcn_readlane(cy, p), iz = __builtin_amdgcn_readlane(cz, p);
+                put(p, *reinterpret_cast<float4 const*>(lat + int64_t(ix) * kH + 4 * lane), rowsum(iy, iz));
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll 1
+    for (int l = LAT ? 1 : 0; l + 1 < kL; ++l)
+    {
+        layer(l);
+        __syncthreads(); // every wave has read the layer input
+        store_act();
     }
     layer(kL - 1);
     {
@@ -1277,14 +1399,53 @@ int mlp_grad(agwn_mlp* h, float const* pts, float const* y, int64_t n, float* gr
     return ok;
 }
 
+// Whether layer 0 of the lattice splits by lattice axis (see k_lattice_table), and how: each coordinate has at most one
+// nonzero step among dx, dy, dz, and the table's nx + ny + nz rows stay within kMaxLatticeRows.
+bool lattice_separable(grid_desc const& g, lattice_map& L)
+{
+    if (g.nx + g.ny + g.nz > kMaxLatticeRows)
+        return false;
+    for (int a = 0; a < 3; ++a)
+    {
+        float const d[3] = {g.dx[a], g.dy[a], g.dz[a]};
+        L.dep[a] = -1;
+        for (int k = 0; k < 3; ++k)
+            if (d[k] != 0.0f)
+            {
+                if (L.dep[a] >= 0)
+                    return false;
+                L.dep[a] = k;
+            }
+    }
+    L.off[0] = 0;
+    L.off[1] = g.nx;
+    L.off[2] = g.nx + g.ny;
+    return true;
+}
+
 template <class Src, bool ROUTE>
-int infer_launch(mlp_impl const& M, Src const& src, int64_t n, float* pred, route_params const& RP, hipStream_t s)
+int infer_launch(mlp_impl const& M, Src const& src, int64_t n, float* pred, route_params const& RP, hipStream_t s,
+                 float* lat = nullptr)
 {
     if (n == 0)
         return ok;
     int64_t const blocks = (n + kTileP - 1) / kTileP;
     AGWN_CHECK(blocks < (int64_t(1) << 31), "too many points (%lld)", (long long)n);
-    k_mlp_infer<Src, ROUTE><<<dim3(unsigned(blocks)), dim3(256), 0, s>>>(src, n, M.F, M.iparams(), pred, RP);
+    if constexpr (std::is_same_v<Src, src_grid>)
+    {
+        lattice_map L{};
+        if (lat && lattice_separable(src.g, L))
+        {
+            int64_t const rows = src.g.nx + src.g.ny + src.g.nz;
+            k_lattice_table<<<dim3(unsigned(rows)), dim3(kH), 0, s>>>(src.g, M.F, L, M.wr[0], M.w + kLay.b[0], lat);
+            AGWN_HIP(hipGetLastError());
+            k_mlp_infer<Src, ROUTE, true><<<dim3(unsigned(blocks)), dim3(256), 0, s>>>(src, n, M.F, M.iparams(), pred,
+                                                                                      RP, lat);
+            AGWN_HIP(hipGetLastError());
+            return ok;
+        }
+    }
+    k_mlp_infer<Src, ROUTE><<<dim3(unsigned(blocks)), dim3(256), 0, s>>>(src, n, M.F, M.iparams(), pred, RP, nullptr);
     AGWN_HIP(hipGetLastError());
     return ok;
 }
@@ -1385,18 +1546,36 @@ int mlp_predict_dev(agwn_mlp const* h, float const* pts, int64_t n, float* out, 
     return infer_launch<src_array<float>, false>(h->impl, src_array<float>{pts}, n, out, route_params{}, s);
 }
 
+int64_t mlp_lattice_table_floats(grid_desc const& g)
+{
+    lattice_map L{};
+    return lattice_separable(g, L) ? (g.nx + g.ny + g.nz) * kH : 0;
+}
+
+int mlp_predict_grid_dev(agwn_mlp const* h, grid_desc const& g, float* out, hipStream_t s)
+{
+    AGWN_CHECK(h != nullptr, "mlp handle is null");
+    AGWN_CHECK(out != nullptr, "null output");
+    int64_t const n = g.nx * g.ny * g.nz;
+    int64_t const tf = mlp_lattice_table_floats(g);
+    dbuf lat;
+    if (tf > 0)
+        AGWN_TRY(lat.alloc(size_t(tf) * sizeof(float), s));
+    return infer_launch<src_grid, false>(h->impl, src_grid{g}, n, out, route_params{}, s, lat.as<float>());
+}
+
 template <class Src>
 int mlp_route(agwn_mlp const* h, Src const& src, int64_t n, float delta, uint32_t audit_mod, uint32_t audit_seed,
-              int8_t* labels, int64_t* list, unsigned long long* counters, hipStream_t s)
+              int8_t* labels, int64_t* list, unsigned long long* counters, hipStream_t s, float* lat)
 {
     AGWN_CHECK(h != nullptr, "mlp handle is null");
     route_params RP{delta, audit_mod, audit_seed, labels, list, counters, h->impl.mask};
-    return infer_launch<Src, true>(h->impl, src, n, nullptr, RP, s);
+    return infer_launch<Src, true>(h->impl, src, n, nullptr, RP, s, lat);
 }
 template int mlp_route<src_array<float>>(agwn_mlp const*, src_array<float> const&, int64_t, float, uint32_t,
-                                         uint32_t, int8_t*, int64_t*, unsigned long long*, hipStream_t);
+                                         uint32_t, int8_t*, int64_t*, unsigned long long*, hipStream_t, float*);
 template int mlp_route<src_grid>(agwn_mlp const*, src_grid const&, int64_t, float, uint32_t, uint32_t, int8_t*,
-                                 int64_t*, unsigned long long*, hipStream_t);
+                                 int64_t*, unsigned long long*, hipStream_t, float*);
 
 int route_apply(int64_t const* list, int64_t m, float const* w, int8_t* labels, unsigned long long* counters,
                 hipStream_t s)

@@ -5,6 +5,7 @@ inference and routing all run in the library's HIP kernels (BF16 MFMA, FP32 accu
     .train(pts, target, ...)        Adam on the mean squared error (device tensors)
     .grad(pts, target)              one forward/backward pass without an update (host arrays; gradient checks)
     .predict(pts)                   fused inference
+    .predict_grid(start, dx, ...)   fused inference over a lattice
     .label(mesh, pts, x0, boundary, delta, ...)           surrogate + exact fallback labels of points
     .label_grid(grid, start, dx, dy, dz, nx, ny, nz, delta, ...)   the same over an eval_grid lattice
   training_samples(...)             the training distribution: points near the surface and uniform in the box
@@ -107,6 +108,19 @@ class GwnSurrogate:
         if out is None:
             out = torch.empty(n, dtype=torch.float32, device=pts.device)
         check(lib.agwn_mlp_predict_dev(self._h, ptr(pts), n, ptr(out), _stream_ptr(stream)))
+        return out
+
+    def predict_grid(self, start, dx, dy, dz, nx: int, ny: int, nz: int, out=None, stream=None):
+        """Network output over an eval_grid lattice, in its ((iz*ny)+iy)*nx+ix layout. On axis-aligned lattices the
+        first layer is summed from per-axis tables (same BF16 products, FP32 sums in another order)."""
+        import torch
+
+        n = nx * ny * nz
+        if out is None:
+            out = torch.empty(n, dtype=torch.float32, device="cuda")
+        vs = [np.ascontiguousarray(np.asarray(a, dtype=np.float32).reshape(3)) for a in (start, dx, dy, dz)]
+        check(lib.agwn_mlp_predict_grid_dev(self._h, *[ptr(a) for a in vs], nx, ny, nz, ptr(out),
+                                            _stream_ptr(stream)))
         return out
 
     def label(self, mesh: HipMesh, pts, x0, boundary, delta: float, audit_every: int = 0, audit_seed: int = 0,

@@ -19,8 +19,9 @@ The headline value is surrogate steps/s x queries; mislabels = labels that diffe
 
 Reported beside it:
   roofline      the dominant kernel of a surrogate step, re-launched alone on the same inputs and timed with HIP
-                events on its stream: the fused inference (492,032 FLOP per point against the 2.5 PFLOP/s BF16 MFMA
-                peak) unless the exact stage takes longer, else the full boundary sum over the routed
+                events on its stream: the fused lattice inference (394,240 FLOP per point: the network's 492,032
+                less layer 0, which on an axis-aligned lattice is the sum of three per-axis table rows; against the
+                2.5 PFLOP/s BF16 MFMA peak) unless the exact stage takes longer, else the full boundary sum over the routed
                 points (routed x segments x 56 FLOP: the 53 adds and multiplies of the reference's pair expression,
                 FMA = 2, plus 2 sqrt and 1 atan2, against the 157.3 TFLOP/s FP32 vector peak)
   cpu_baseline  the reference's own eval_gwnr_mesh_batch_fractional<float> (oracle/_ref, compiled from its headers)
@@ -47,6 +48,10 @@ PEAK_FP32_TFLOPS = 157.3
 PEAK_BF16_TFLOPS = 2500.0
 FLOP_PER_PAIR = 56
 MLP_FLOP_PER_POINT = 2 * (192 * 256 + 3 * 256 * 256 + 256)
+# the lattice kernel's own work per point: layer 0 as two adds of table rows per unit, the three 256 x 256 layers and
+# the output dot; plus, per table row (one lattice index of one axis), a coordinate's 64 features times 256 weights
+MLP_FLOP_PER_POINT_LATTICE = 2 * 256 + 2 * 3 * 256 * 256 + 2 * 256
+LATTICE_TABLE_FLOP_PER_ROW = 2 * 64 * 256
 METRIC = "GWN queries/s, 1M-tri open mesh; surrogate+fallback vs plain, 0 mislabels"
 
 
@@ -456,9 +461,9 @@ def main() -> int:
     P = lattice_torch(start, dx, dy, dz, nres, dev)
     near = net.near_surface(P).bool()
     pred = torch.empty(nq, dtype=torch.float32, device=dev)
-    net.predict(P, out=pred, stream=stream)
+    net.predict_grid(start, dx, dy, dz, nres, nres, nres, out=pred, stream=stream)
     ev0.record(stream)
-    net.predict(P, out=pred, stream=stream)
+    net.predict_grid(start, dx, dy, dz, nres, nres, nres, out=pred, stream=stream)
     ev1.record(stream)
     ev1.synchronize()
     infer_ms = ev0.elapsed_time(ev1)  # the route variant also writes labels and the list: a lower bound
@@ -520,10 +525,12 @@ def main() -> int:
                 "flop_per_launch": len(rp) * nseg * FLOP_PER_PAIR, "launch_points": len(rp)}
         traffic_key = "k_frac_f32_chain"
     else:
-        achieved = nq * MLP_FLOP_PER_POINT / (infer_ms * 1e-3) / 1e12
-        roof = {"bound": "mfma", "kernel": "k_mlp_infer", "achieved": achieved, "peak": PEAK_BF16_TFLOPS,
-                "unit": "TFLOP/s", "frac": achieved / PEAK_BF16_TFLOPS, "kernel_ms": infer_ms,
-                "flop_per_launch": nq * MLP_FLOP_PER_POINT, "launch_points": nq}
+        flop = nq * MLP_FLOP_PER_POINT_LATTICE + 3 * nres * LATTICE_TABLE_FLOP_PER_ROW
+        achieved = flop / (infer_ms * 1e-3) / 1e12
+        roof = {"bound": "mfma", "kernel": "k_mlp_infer (lattice tables)", "achieved": achieved,
+                "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_BF16_TFLOPS, "kernel_ms": infer_ms,
+                "flop_per_launch": flop, "launch_points": nq,
+                "network_equivalent_tflops": nq * MLP_FLOP_PER_POINT / (infer_ms * 1e-3) / 1e12}
         traffic_key = "k_mlp_infer"
     # HBM bytes per launch of that kernel, from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE passes of this
     # workload (profiles/traffic.json, scaled by the copy calibration); null when none is committed

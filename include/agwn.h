@@ -236,6 +236,13 @@ int agwn_mlp_grad(agwn_mlp* mlp, const float* pts, const float* target, int64_t 
                   float* loss);
 /* Fused inference (device arrays): out[i] = network(pts[i]). */
 int agwn_mlp_predict_dev(const agwn_mlp* mlp, const float* pts, int64_t n, float* out, agwn_stream stream);
+/* Fused inference over the lattice of agwn_grid_eval (host start/dx/dy/dz float[3]; out: device float[nx ny nz] in its
+ * layout). When each coordinate steps along one lattice axis at most (axis-aligned lattices, any axis order) and
+ * nx + ny + nz <= 65536, the first layer is a sum of three per-axis tables of the same BF16 products (the points'
+ * features depend on one index each), so outputs may differ from agwn_mlp_predict_dev's on the same points by the
+ * order of FP32 sums in that layer; otherwise it is agwn_mlp_predict_dev's per-point kernel. */
+int agwn_mlp_predict_grid_dev(const agwn_mlp* mlp, const float* start, const float* dx, const float* dy,
+                              const float* dz, int nx, int ny, int nz, float* out, agwn_stream stream);
 /* Near-surface mask of the network's box: voxels of edge `voxel` (<= 0: eps/2) within eps of a triangle of the mesh
  * (host verts/idx), marked conservatively on the GPU. Points in marked voxels, or outside the box, are never labelled
  * by the network. */
