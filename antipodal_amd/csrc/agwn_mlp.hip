@@ -467,7 +467,7 @@ __global__ void __launch_bounds__(kB) k_perm(int64_t* __restrict__ perm, int64_t
 constexpr int kLdsStride = kH + 8; // BF16 per LDS row: 528 B rows keep the 16-lane groups of ds_read_b128 conflict-free
 constexpr int kTileP = 128;         // points per workgroup (a multiple of 32)
 constexpr int kTileJ = kTileP / 32; // 32-point MFMA column blocks per tile
-constexpr int kInferBlocks = (160 * 1024) / (kTileP * kLdsStride * 2 + 16 * kTileP); // workgroups per CU that the LDS holds
+constexpr int kInferBlocks = (160 * 1024) / (kTileP * kLdsStride * 2 + 16 * kTileP + 4 * (kL + 1) * kH); // workgroups per CU that the LDS holds
 static_assert(kTileP % 32 == 0 && kInferBlocks >= 1, "tile shape");
 
 struct infer_params
@@ -671,9 +671,27 @@ __global__ void __launch_bounds__(256, kInferBlocks) k_mlp_infer(Src const src, 
 {
     __shared__ __attribute__((aligned(16))) __bf16 act[kTileP * kLdsStride];
     __shared__ float predw[4][kTileP]; // per wave: the dot over its 64 units
+    __shared__ __attribute__((aligned(16))) float par_s[kL * kH + kH]; // biases b_0 .. b_3, then wo
     int const lane = threadIdx.x & 63;
     int const wave = threadIdx.x >> 6;
+    int const n0 = wave * 64;
     int64_t const base = int64_t(blockIdx.x) * kTileP;
+
+    // the first MFMA layer's first three weight steps, fetched before anything else so that the prologue hides them
+    bf16x8 f0[4], f1[4]; // weight fragments of unit blocks 2w, 2w + 1 for four k steps (see layer)
+    {
+        int const l = LAT ? 1 : 0;
+        int const K = l == 0 ? kIn : kH;
+        __bf16 const* wa0 = P.wf[l] + (int64_t(n0 / 32) * (K / 16) * 64 + lane) * 8;
+#pragma unroll
+        for (int t = 0; t < 3; ++t)
+        {
+            f0[t] = *reinterpret_cast<bf16x8 const*>(wa0 + 512 * t);
+            f1[t] = *reinterpret_cast<bf16x8 const*>(wa0 + int64_t(K / 16) * 512 + 512 * t);
+        }
+    }
+    for (int k = threadIdx.x; k < kL * kH + kH; k += 256)
+        par_s[k] = k < kL * kH ? P.b[k / kH][k % kH] : P.wo[k - kL * kH];
 
     // the routing's near-surface mask word, fetched now and tested after the output layer, so that the MLP hides the
     // load's latency
@@ -735,33 +753,34 @@ __global__ void __launch_bounds__(256, kInferBlocks) k_mlp_infer(Src const src, 
     }
 
     int const kh = hl * 8;
-    int const n0 = wave * 64;
     f32x16 acc[2][kTileJ]; // C[r][c] of the lane's units: unit r = 8(e/4) + 4(l/32) + e%4
-    // layer l's MFMAs: acc = b_l + Wt_l[units of the wave] act^T. The biases go into acc[i][0] and enter every point
-    // block's first MFMA as its C operand (no copies). The k loop is unrolled (static buffer indexes; a layer of
-    // K = 192 leaves the last 4 of 16 steps); step s's weight fragments (f[s % 3]) are fetched from L2 three steps
-    // ahead and its activation operands (b[s & 1], LDS) one step ahead, so the MFMAs of a step rarely wait for either
-    // (two steps ahead measured 4 % slower: L2 latency under this kernel's load exceeds two steps); scheduling
-    // barriers keep the compiler from sinking the fetches back to their uses. ~210 VGPRs: two waves per SIMD.
+    // layer l's MFMAs: acc = b_l + Wt_l[units of the wave] act^T. The biases (from LDS) go into acc[i][0] and enter
+    // every point block's first MFMA as its C operand (no copies). The k loop is unrolled (static buffer indexes; a
+    // layer of K = 192 leaves the last 4 of 16 steps). Step s's weight fragments (f[s % 4]) are fetched from L2 three
+    // steps ahead -- across layers too: a layer's last three steps fetch the next layer's first three, so no layer
+    // starts on an L2 round trip (layers have 12 or 16 steps, multiples of 4, so a step's buffer is the same counted
+    // in either layer) -- and its activation operands (b[s & 1], LDS) one step ahead. Scheduling barriers keep the
+    // compiler from sinking the fetches back to their uses.
+    auto frag = [&](int l) __attribute__((always_inline)) {
+        int const K = l == 0 ? kIn : kH;
+        return P.wf[l] + (int64_t(n0 / 32) * (K / 16) * 64 + lane) * 8; // unit block 2w; 2w + 1 at + (K / 16) 512
+    };
     auto layer = [&](int l) __attribute__((always_inline)) {
         int const K = l == 0 ? kIn : kH;
-        // fragments of unit blocks 2w and 2w + 1 (units n0 .. n0 + 63); k step st at + 512 st
-        __bf16 const* wa0 = P.wf[l] + (int64_t(n0 / 32) * (K / 16) * 64 + lane) * 8;
-        __bf16 const* wa1 = wa0 + int64_t(K / 16) * 512;
+        int const nst = K / 16;
+        __bf16 const* wa0 = frag(l);
+        __bf16 const* wa1 = wa0 + int64_t(nst) * 512;
+        bool const nxt = l + 1 < kL;
+        __bf16 const* na0 = nxt ? frag(l + 1) : wa0;
+        __bf16 const* na1 = na0 + int64_t(kH / 16) * 512;
         __bf16 const* brow = act + li * kLdsStride + kh;
-        bf16x8 f0[3], f1[3], b[2][kTileJ];
-#pragma unroll
-        for (int t = 0; t < 3; ++t)
-        {
-            f0[t] = *reinterpret_cast<bf16x8 const*>(wa0 + 512 * t);
-            f1[t] = *reinterpret_cast<bf16x8 const*>(wa1 + 512 * t);
-        }
+        bf16x8 b[2][kTileJ];
 #pragma unroll
         for (int i = 0; i < 2; ++i)
 #pragma unroll
             for (int q = 0; q < 4; ++q)
             {
-                float4 const bq = *reinterpret_cast<float4 const*>(P.b[l] + n0 + i * 32 + 8 * q + 4 * hl);
+                float4 const bq = *reinterpret_cast<float4 const*>(par_s + l * kH + n0 + i * 32 + 8 * q + 4 * hl);
                 acc[i][0][4 * q] = bq.x;
                 acc[i][0][4 * q + 1] = bq.y;
                 acc[i][0][4 * q + 2] = bq.z;
@@ -777,7 +796,7 @@ __global__ void __launch_bounds__(256, kInferBlocks) k_mlp_infer(Src const src, 
             int const k0 = 16 * st;
             if (k0 >= K)
                 break;
-            int const buf = st & 1, fb = st % 3;
+            int const buf = st & 1, fb = st % 4;
             if (k0 + 16 < K)
             {
 #pragma unroll
@@ -801,11 +820,17 @@ __global__ void __launch_bounds__(256, kInferBlocks) k_mlp_infer(Src const src, 
                         acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b[buf][j], acc[i][j], 0, 0, 0);
                 }
             }
-            int const kn = k0 + 48;
-            if (kn < K)
+            int const s3 = st + 3, f3 = s3 % 4;
+            if (s3 < nst)
             {
-                f0[fb] = *reinterpret_cast<bf16x8 const*>(wa0 + int64_t(kn) * 32);
-                f1[fb] = *reinterpret_cast<bf16x8 const*>(wa1 + int64_t(kn) * 32);
+                f0[f3] = *reinterpret_cast<bf16x8 const*>(wa0 + int64_t(s3) * 512);
+                f1[f3] = *reinterpret_cast<bf16x8 const*>(wa1 + int64_t(s3) * 512);
+            }
+            else if (nxt)
+            {
+                int64_t const t = s3 - nst; // the next layer's step, same buffer
+                f0[f3] = *reinterpret_cast<bf16x8 const*>(na0 + t * 512);
+                f1[f3] = *reinterpret_cast<bf16x8 const*>(na1 + t * 512);
             }
             __builtin_amdgcn_sched_barrier(0);
         }
@@ -909,7 +934,7 @@ __global__ void __launch_bounds__(256, kInferBlocks) k_mlp_infer(Src const src, 
 #pragma unroll
             for (int q = 0; q < 4; ++q)
             {
-                float4 const wq = *reinterpret_cast<float4 const*>(P.wo + n0 + i * 32 + 8 * q + 4 * (lane >> 5));
+                float4 const wq = *reinterpret_cast<float4 const*>(par_s + kL * kH + n0 + i * 32 + 8 * q + 4 * hl);
 #pragma unroll
                 for (int j = 0; j < kTileJ; ++j)
                 {
