@@ -401,3 +401,16 @@ def test_bad_arguments(gpu, c1):
         with gpu.HipMesh(v, idx, dtype=np.float64) as mesh64:
             with pytest.raises(gpu.AgwnError, match="f32 mesh"):
                 net.label(mesh64, pts, mg.example_x0(np.float32), None, 0.1)
+        # the lattice entry validates the lattice and the handle before any launch
+        e = np.eye(3, dtype=np.float32)
+        with pytest.raises(gpu.AgwnError, match="positive"):
+            net.predict_grid(lo, e[0], e[1], e[2], 0, 4, 4)
+        small = torch.empty(8, dtype=torch.float32, device="cuda")
+        with pytest.raises(gpu.AgwnError, match="too large"):
+            net.predict_grid(lo, e[0], e[1], e[2], 1 << 30, 1 << 30, 1 << 2, out=small)
+    from antipodal_amd._lib import lib, ptr
+
+    out = torch.empty(8, dtype=torch.float32, device="cuda")
+    vs = [np.ascontiguousarray(a, dtype=np.float32) for a in (lo, e[0], e[1], e[2])]
+    assert lib.agwn_mlp_predict_grid_dev(None, *[ptr(a) for a in vs], 2, 2, 2, ptr(out), None) != 0
+    assert b"mlp handle is null" in lib.agwn_last_error()
