@@ -22,11 +22,12 @@ from .api import (
     eval_gwnr_mesh_batch_fractional,
     eval_gwnr_mesh_batch_integer,
 )
-from .surrogate import GwnSurrogate, fit_surrogate
+from .surrogate import GwnSurrogate, TrainingSampler, fit_surrogate
 
 __all__ = [
     "AgwnError",
     "GwnSurrogate",
+    "TrainingSampler",
     "HipGridGwn",
     "HipMesh",
     "HipMeshBatch",

@@ -88,6 +88,8 @@ _SIG = {
     "agwn_mlp_predict_grid_dev": (_int, [_p, _p, _p, _p, _p, _int, _int, _int, _p, _p]),
     "agwn_mlp_set_surface": (_int, [_p, _p, _i64, _p, _i64, C.c_float, C.c_float]),
     "agwn_mlp_near_surface_dev": (_int, [_p, _p, _i64, _p, _p]),
+    "agwn_training_samples_dev": (_int, [_p, _p, _p, _i64, _p, _p, _i64, C.c_double, C.c_double, C.c_double,
+                                         C.c_uint64, _p, _p]),
     "agwn_label_surrogate_dev": (_int, [_p, _p, _p, _i64, _p, _p, _i64, C.c_float, _i64, C.c_uint64, _p, _p, _p]),
     "agwn_grid_label_surrogate_dev": (_int, [_p, _p, _p, _p, _p, _p, _int, _int, _int, C.c_float, _i64, C.c_uint64,
                                              _p, _p, _p]),

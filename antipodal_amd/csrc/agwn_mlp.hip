@@ -664,6 +664,77 @@ __global__ void __launch_bounds__(kH) k_lattice_table(grid_desc const g, feat_pa
     T[r * kH + n] = acc;
 }
 
+// Training samples of the surrogate: point i is drawn from counter-based SplitMix64 streams c = 0..5 of (seed, c), its
+// k-th value mix(mix(seed G + c C) + (i + 1) G) (G = 0x9E3779B97F4A7C15, C = 0xD1B54A32D192ED03) turned into a uniform
+// double (z >> 11) 2^-53 -- the streams of meshgen.uniform01(seed, c, n). With probability near_frac (stream 0) the
+// point lies at a signed normal offset from an area-uniform surface point: the triangle is the first whose cumulative
+// doubled area exceeds u1 times the total (cdf: inclusive prefix sums of |(b - a) x (c - a)|), the barycentrics are
+// sqrt(u2) and u3, the offset band s u5^power with s = -1 when u4 < 1/2; otherwise it is uniform in the box
+// (lo + ext (u1, u2, u3)). Double arithmetic without contraction, rounded to float at the end.
+__device__ __forceinline__ uint64_t sm_mix(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__global__ void __launch_bounds__(kB) k_training_samples(float const* __restrict__ v, int32_t const* __restrict__ idx,
+                                                         double const* __restrict__ cdf, int64_t ntri, double3 lo,
+                                                         double3 ext, int64_t n, double near_frac, double band,
+                                                         double power, uint64_t seed, float* __restrict__ out)
+{
+#pragma clang fp contract(off)
+    int64_t const i = int64_t(blockIdx.x) * kB + threadIdx.x;
+    if (i >= n)
+        return;
+    auto uni = [&](uint64_t c) {
+        uint64_t const base = sm_mix(seed * 0x9E3779B97F4A7C15ull + c * 0xD1B54A32D192ED03ull);
+        return double(sm_mix(base + uint64_t(i + 1) * 0x9E3779B97F4A7C15ull) >> 11) * 0x1p-53;
+    };
+    double p[3];
+    if (ntri > 0 && uni(0) < near_frac)
+    {
+        double const t = uni(1) * cdf[ntri - 1];
+        int64_t a0 = 0, a1 = ntri; // first k with cdf[k] > t
+        while (a0 < a1)
+        {
+            int64_t const m = (a0 + a1) >> 1;
+            if (cdf[m] > t)
+                a1 = m;
+            else
+                a0 = m + 1;
+        }
+        int64_t const k = a0 < ntri ? a0 : ntri - 1;
+        double A[3], B[3], C[3];
+        for (int d = 0; d < 3; ++d)
+        {
+            A[d] = v[3 * int64_t(idx[3 * k]) + d];
+            B[d] = v[3 * int64_t(idx[3 * k + 1]) + d];
+            C[d] = v[3 * int64_t(idx[3 * k + 2]) + d];
+        }
+        double const r1 = sqrt(uni(2)), r2 = uni(3);
+        double const w0 = 1.0 - r1, w1 = r1 * (1.0 - r2), w2 = r1 * r2;
+        double const e1[3] = {B[0] - A[0], B[1] - A[1], B[2] - A[2]};
+        double const e2[3] = {C[0] - A[0], C[1] - A[1], C[2] - A[2]};
+        double nr[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+        double const len = sqrt((nr[0] * nr[0] + nr[1] * nr[1]) + nr[2] * nr[2]);
+        if (len > 0.0)
+            for (int d = 0; d < 3; ++d)
+                nr[d] /= len;
+        double const off = (band * (uni(4) < 0.5 ? -1.0 : 1.0)) * pow(uni(5), power);
+        for (int d = 0; d < 3; ++d)
+            p[d] = ((w0 * A[d] + w1 * B[d]) + w2 * C[d]) + off * nr[d];
+    }
+    else
+    {
+        p[0] = lo.x + ext.x * uni(1);
+        p[1] = lo.y + ext.y * uni(2);
+        p[2] = lo.z + ext.z * uni(3);
+    }
+    for (int d = 0; d < 3; ++d)
+        out[3 * i + d] = float(p[d]);
+}
+
 template <class Src, bool ROUTE, bool LAT = false>
 __global__ void __launch_bounds__(256, kInferBlocks) k_mlp_infer(Src const src, int64_t n, feat_params F, infer_params P,
                                                       float* __restrict__ pred, route_params RP,
@@ -1551,6 +1622,29 @@ int mlp_set_surface(agwn_mlp* h, float const* verts, int64_t nv, int32_t const* 
     return ok;
 }
 
+int training_samples_dev(float const* verts, int32_t const* idx, double const* cdf, int64_t ntri, float const* lo,
+                         float const* hi, int64_t n, double near_frac, double band, double power, uint64_t seed,
+                         float* out, hipStream_t s)
+{
+    AGWN_CHECK(n >= 0 && ntri >= 0, "negative count");
+    AGWN_CHECK(lo && hi, "null box");
+    AGWN_CHECK(n == 0 || out, "null output");
+    AGWN_CHECK(ntri == 0 || (verts && idx && cdf), "null mesh arrays");
+    AGWN_CHECK(near_frac >= 0.0 && near_frac <= 1.0 && band >= 0.0 && power > 0.0 && std::isfinite(band) &&
+                   std::isfinite(power),
+               "bad sampling parameters (near_frac %g, band %g, power %g)", near_frac, band, power);
+    for (int a = 0; a < 3; ++a)
+        AGWN_CHECK(std::isfinite(lo[a]) && std::isfinite(hi[a]) && lo[a] <= hi[a], "bad box on axis %d", a);
+    if (n == 0)
+        return ok;
+    double3 const L{double(lo[0]), double(lo[1]), double(lo[2])};
+    double3 const E{double(hi[0]) - double(lo[0]), double(hi[1]) - double(lo[1]), double(hi[2]) - double(lo[2])};
+    k_training_samples<<<dim3(unsigned((n + kB - 1) / kB)), dim3(kB), 0, s>>>(verts, idx, cdf, ntri, L, E, n, near_frac,
+                                                                              band, power, seed, out);
+    AGWN_HIP(hipGetLastError());
+    return ok;
+}
+
 int mlp_near_surface_dev(agwn_mlp const* h, float const* pts, int64_t n, int8_t* out, hipStream_t s)
 {
     AGWN_CHECK(h != nullptr, "mlp handle is null");
@@ -1793,6 +1887,15 @@ int agwn_mlp_set_surface(agwn_mlp* m, float const* verts, int64_t nv, int32_t co
                          float voxel)
 {
     return guarded_mlp([&] { return agwn::mlp_set_surface(m, verts, nv, idx, nidx, eps, voxel); });
+}
+int agwn_training_samples_dev(float const* verts, int32_t const* idx, double const* cdf, int64_t ntri, float const* lo,
+                              float const* hi, int64_t n, double near_frac, double band, double power, uint64_t seed,
+                              float* out, agwn_stream s)
+{
+    return guarded_mlp([&] {
+        return agwn::training_samples_dev(verts, idx, cdf, ntri, lo, hi, n, near_frac, band, power, seed, out,
+                                          hipStream_t(s));
+    });
 }
 int agwn_mlp_near_surface_dev(agwn_mlp const* m, float const* pts, int64_t n, int8_t* out, agwn_stream s)
 {

@@ -8,7 +8,8 @@ inference and routing all run in the library's HIP kernels (BF16 MFMA, FP32 accu
     .predict_grid(start, dx, ...)   fused inference over a lattice
     .label(mesh, pts, x0, boundary, delta, ...)           surrogate + exact fallback labels of points
     .label_grid(grid, start, dx, dy, dz, nx, ny, nz, delta, ...)   the same over an eval_grid lattice
-  training_samples(...)             the training distribution: points near the surface and uniform in the box
+  TrainingSampler(verts, idx, lo, hi)(n, ...)   the training distribution on the device: points near the surface and
+                                    uniform in the box
   fit_surrogate(...)                samples -> exact GWN targets (HipMesh.full) -> training on jittered inputs ->
                                     band calibration on unjittered validation points
 
@@ -176,41 +177,67 @@ def _stats(st: np.ndarray, n: int) -> dict:
             "audit_disagree": int(st[4]), "fallback_fraction": float(st[3]) / max(n, 1)}
 
 
-def training_samples(verts, idx, lo, hi, n: int, near_frac: float = 0.7, band: float = 0.05, seed: int = 0,
-                     power: float = 2.0):
-    """n float32 points: near_frac of them at signed normal offsets t = band * s * u^power (s = +-1, u ~ U(0,1)) from
-    area-uniform surface points (so all within `band` of the surface; power > 1 puts more of them close to it), the
-    rest uniform in [lo, hi]."""
-    rng = np.random.default_rng(seed)
-    v = np.asarray(verts, dtype=np.float64).reshape(-1, 3)
-    tri = np.asarray(idx, dtype=np.int64).reshape(-1, 3)
-    a, b, c = v[tri[:, 0]], v[tri[:, 1]], v[tri[:, 2]]
-    nrm = np.cross(b - a, c - a)
-    area = np.linalg.norm(nrm, axis=1)
-    good = area > 0
-    nrm[good] /= area[good, None]
-    cdf = np.cumsum(area)
-    n_near = int(round(n * near_frac))
-    t = rng.random(n_near) * cdf[-1]
-    k = np.minimum(np.searchsorted(cdf, t, side="right"), len(tri) - 1)
-    r1 = np.sqrt(rng.random(n_near))
-    r2 = rng.random(n_near)
-    p = (1 - r1)[:, None] * a[k] + (r1 * (1 - r2))[:, None] * b[k] + (r1 * r2)[:, None] * c[k]
-    off = band * np.where(rng.random(n_near) < 0.5, -1.0, 1.0) * rng.random(n_near) ** power
-    p += off[:, None] * nrm[k]
-    lo = np.asarray(lo, dtype=np.float64)
-    hi = np.asarray(hi, dtype=np.float64)
-    u = lo + (hi - lo) * rng.random((n - n_near, 3))
-    pts = np.concatenate([p, u]).astype(np.float32)
-    return pts[rng.permutation(n)]
+class TrainingSampler:
+    """The surrogate's training distribution on the device (`agwn_training_samples_dev`): near_frac of the points at
+    signed normal offsets band * s * u^power (s = +-1, u ~ U(0, 1)) from area-uniform surface points, so all within
+    `band` of the surface (power > 1 puts more of them close to it), the rest uniform in [lo, hi]. Point i of a call
+    depends on (seed, i) only (counter-based SplitMix64 streams). The mesh and its cumulative triangle areas (doubled
+    areas |(b - a) x (c - a)| in float64, summed in triangle order) are uploaded once."""
+
+    def __init__(self, verts, idx, lo, hi):
+        import torch
+
+        v = np.asarray(verts, dtype=np.float64).reshape(-1, 3)
+        tri = np.asarray(idx, dtype=np.int64).reshape(-1, 3)
+        a, b, c = v[tri[:, 0]], v[tri[:, 1]], v[tri[:, 2]]
+        area = np.linalg.norm(np.cross(b - a, c - a), axis=1)
+        self._v = torch.from_numpy(np.ascontiguousarray(v, dtype=np.float32)).cuda()
+        self._i = torch.from_numpy(np.ascontiguousarray(tri, dtype=np.int32)).cuda()
+        self._cdf = torch.from_numpy(np.cumsum(area)).cuda()
+        self._ntri = len(tri)
+        self._lo = np.ascontiguousarray(np.asarray(lo, dtype=np.float32).reshape(3))
+        self._hi = np.ascontiguousarray(np.asarray(hi, dtype=np.float32).reshape(3))
+
+    def __call__(self, n: int, near_frac: float = 0.7, band: float = 0.05, seed: int = 0, power: float = 2.0,
+                 out=None, stream=None):
+        import torch
+
+        if out is None:
+            out = torch.empty((n, 3), dtype=torch.float32, device="cuda")
+        check(lib.agwn_training_samples_dev(ptr(self._v), ptr(self._i), ptr(self._cdf), self._ntri, ptr(self._lo),
+                                            ptr(self._hi), n, C.c_double(near_frac), C.c_double(band),
+                                            C.c_double(power), C.c_uint64(seed), ptr(out), _stream_ptr(stream)))
+        return out
 
 
-def calibrate_delta(pred: np.ndarray, exact: np.ndarray, factor: float = 2.0, margin: float = 0.02) -> float:
+def calibrate_delta(pred: np.ndarray, exact: np.ndarray, factor: float = 2.0, margin: float = 0.02,
+                    skip: np.ndarray | None = None) -> float:
     """Band half-width from validation predictions: factor x the largest |pred - 0.5| of a wrongly labelled
-    validation point, plus margin."""
+    validation point (those flagged in `skip` left out), plus margin."""
     wrong = (pred > 0.5) != (exact > 0.5)
+    if skip is not None:
+        wrong &= ~skip
     worst = float(np.max(np.abs(pred[wrong] - 0.5))) if wrong.any() else 0.0
     return factor * worst + margin
+
+
+def exact_disagreements(verts, idx, boundary, x0, pts, w32):
+    """Flags the points (device (m, 3) float32) whose FP32 full GWN w32 (device) differs by more than 1/2 from the
+    FP64 evaluation of the same float32 geometry: the non-watertight ray test of the integer term (the reference's
+    Moller-Trumbore, kept bit for bit) counted a crossing differently in the two precisions, a ray grazing an edge or
+    vertex. Host bool array."""
+    import torch
+
+    if len(pts) == 0:
+        return np.zeros(0, dtype=bool)
+    v64 = np.asarray(verts, dtype=np.float32).astype(np.float64)
+    seg64 = None
+    if boundary is not None and len(boundary):
+        seg64 = torch.from_numpy(np.asarray(boundary, dtype=np.float32).reshape(-1, 7).astype(np.float64)).cuda()
+    w64 = torch.empty(len(pts), dtype=torch.float64, device="cuda")
+    with HipMesh(v64, idx, dtype=np.float64) as m64:
+        m64.full(pts.double().contiguous(), np.asarray(x0, dtype=np.float64), seg64, out=w64)
+    return ((w64 - w32.double()).abs() > 0.5).cpu().numpy()
 
 
 def fit_surrogate(verts, idx, boundary, x0, lo, hi, n_train: int = 1 << 24, n_val: int = 1 << 22,
@@ -244,10 +271,10 @@ def fit_surrogate(verts, idx, boundary, x0, lo, hi, n_train: int = 1 << 24, n_va
     seg = torch.from_numpy(np.ascontiguousarray(np.asarray(boundary, dtype=np.float32).reshape(-1, 7))).cuda()
     # candidates, then drop those the mask routes anyway; oversample so that enough remain
     want = n_train + n_val
+    sampler = TrainingSampler(verts, idx, lo, hi)
     keep_parts, have, rnd = [], 0, 0
     while have < want and rnd < 4:
-        cand = torch.from_numpy(training_samples(verts, idx, lo, hi, int(want * 1.6) + 1024, seed=seed + 7919 * rnd,
-                                                 power=power)).cuda()
+        cand = sampler(int(want * 1.6) + 1024, seed=seed + 7919 * rnd, power=power)
         ok = net.near_surface(cand) == 0
         keep_parts.append(cand[ok])
         have += int(ok.sum())
@@ -279,10 +306,19 @@ def fit_surrogate(verts, idx, boundary, x0, lo, hi, n_train: int = 1 << 24, n_va
     pred = net.predict(va_p).cpu().numpy()
     exact = va_y.cpu().numpy()
     wrong = (pred > 0.5) != (exact > 0.5)
-    delta = calibrate_delta(pred, exact, factor, margin)
+    # The band calibrates the network's error, not the exact path's: a wrongly labelled validation point whose FP32
+    # value is off by an integer (the integer term's ray grazed an edge; the FP64 evaluation counts it otherwise) would
+    # set delta near 1 by itself, so such points are left out of the calibration (and counted).
+    skip = np.zeros(len(pred), dtype=bool)
+    iw = np.nonzero(wrong)[0]
+    if len(iw):
+        iw_t = torch.from_numpy(iw).cuda()
+        skip[iw] = exact_disagreements(verts, idx, boundary, x0, va_p[iw_t], va_y[iw_t])
+    delta = calibrate_delta(pred, exact, factor, margin, skip)
+    counted = wrong & ~skip
     info["val_pred"], info["val_exact"] = pred, exact
-    info.update(val_points=n_val, val_wrong=int(wrong.sum()),
-                val_max_wrong=float(np.max(np.abs(pred[wrong] - 0.5))) if wrong.any() else 0.0, delta=delta,
+    info.update(val_points=n_val, val_wrong=int(wrong.sum()), val_exact_disagree=int(skip.sum()),
+                val_max_wrong=float(np.max(np.abs(pred[counted] - 0.5))) if counted.any() else 0.0, delta=delta,
                 val_band_fraction=float(np.mean(np.abs(pred - 0.5) <= delta)), fit_s=time.perf_counter() - t0)
     if log:
         log("surrogate: " + str({k: v for k, v in info.items() if k not in ("val_pred", "val_exact")}))

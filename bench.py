@@ -610,6 +610,7 @@ def main() -> int:
             "fit_s": fit["fit_s"],
             "loss_first_last": [fit["losses"][0], fit["losses"][-1]],
             "val_wrong": fit["val_wrong"],
+            "val_exact_disagree": fit.get("val_exact_disagree", 0),
             "val_points": fit["val_points"],
         },
         "kernels_ms": {"k_mlp_infer": infer_ms, "exact_stage_on_routed": stage_ms,

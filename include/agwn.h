@@ -248,6 +248,14 @@ int agwn_mlp_predict_grid_dev(const agwn_mlp* mlp, const float* start, const flo
  * by the network. */
 int agwn_mlp_set_surface(agwn_mlp* mlp, const float* verts, int64_t nv, const int32_t* idx, int64_t nidx, float eps,
                          float voxel);
+/* n training positions (device float[3n]) of the surrogate: with probability near_frac a point at a signed offset
+ * band s u^power (s = +-1, u uniform) along the unit normal of an area-uniform point of the mesh (device verts
+ * float[3nv], idx int32[3 ntri]; cdf: device double[ntri], the inclusive prefix sums of |(b - a) x (c - a)| in triangle
+ * order), otherwise uniform in the box [lo, hi] (host float[3] each). Counter-based SplitMix64 streams (seed, 0..5):
+ * point i depends on (seed, i) only. */
+int agwn_training_samples_dev(const float* verts, const int32_t* idx, const double* cdf, int64_t ntri, const float* lo,
+                              const float* hi, int64_t n, double near_frac, double band, double power, uint64_t seed,
+                              float* out, agwn_stream stream);
 /* out[i] = 1 if pts[i] (device) is in the mask or outside the box, else 0 (device int8). */
 int agwn_mlp_near_surface_dev(const agwn_mlp* mlp, const float* pts, int64_t n, int8_t* out, agwn_stream stream);
 /* Surrogate + fallback labelling of points: labels[i] = (GWN(pts[i]) > 0.5). The network labels every point; points
