@@ -248,7 +248,8 @@ __global__ void __launch_bounds__(256)
             }
 }
 
-// out[i] = sum over p of part[p * count + i], in a fixed order (deterministic)
+// out[i] = sum over p of part[p * count + i], in a fixed order (deterministic); the loads of eight partials are issued
+// before their sums
 __global__ void __launch_bounds__(kB) k_sum_parts(float const* __restrict__ part, int64_t parts, int64_t count,
                                                   float* __restrict__ out)
 {
@@ -256,19 +257,38 @@ __global__ void __launch_bounds__(kB) k_sum_parts(float const* __restrict__ part
     if (i >= count)
         return;
     float s = 0.0f;
-    for (int64_t p = 0; p < parts; ++p)
+    int64_t p = 0;
+    for (; p + 8 <= parts; p += 8)
+    {
+        float v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            v[k] = part[(p + k) * count + i];
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            s += v[k];
+    }
+    for (; p < parts; ++p)
         s += part[p * count + i];
     out[i] = s;
 }
 
-// Bias gradient: out[n] = sum_m DT[n][m] (one block per unit n)
+// Bias gradient: out[n] = sum_m DT[n][m] (one block per unit n; rows is a multiple of 128: each thread sums 8
+// consecutive values per 16-byte load, then the block reduces)
 __global__ void __launch_bounds__(kB) k_rowsum(__bf16 const* __restrict__ DT, int64_t rows, float* __restrict__ out)
 {
     __shared__ float red[kB / 64];
     __bf16 const* row = DT + int64_t(blockIdx.x) * rows;
     float s = 0.0f;
-    for (int64_t m = threadIdx.x; m < rows; m += kB)
-        s += float(row[m]);
+    for (int64_t m = 8 * int64_t(threadIdx.x); m < rows; m += 8 * kB)
+    {
+        bf16x8 const v = *reinterpret_cast<bf16x8 const*>(row + m);
+        float t = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            t += float(v[k]);
+        s += t;
+    }
     for (int off = 32; off > 0; off >>= 1)
         s += __shfl_xor(s, off);
     if ((threadIdx.x & 63) == 0)
@@ -278,23 +298,27 @@ __global__ void __launch_bounds__(kB) k_rowsum(__bf16 const* __restrict__ DT, in
         out[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
-constexpr int64_t kObl = 64; // blocks of the output-layer kernel
+constexpr int64_t kOblRows = 128; // rows per block of the output-layer kernel (rows are a multiple of 128)
 
-// Output layer and loss. Wave-per-sample; lane l covers units 4l .. 4l+3.
+// Output layer and loss over kOblRows rows per block. Wave-per-sample; lane l covers units 4l .. 4l+3.
 //   pred = H3 . wo + bo;  g = 2 (pred - y) / nvalid;  dz3 = bf16(g wo hsilu'(Z3));  per-block partials of g H3 (dwo),
 //   g (dbo) and (pred - y)^2.
-// y is read through perm (sample order of the batch). Without D only pred is written.
+// D is written row by row; its transpose DT goes through an LDS tile of the block's rows and is written unit by unit,
+// 8 lanes per 256-byte unit row (whole cache lines). y is read through perm (sample order of the batch). Without D
+// only pred is written.
 __global__ void __launch_bounds__(kB)
     k_out_loss(__bf16 const* __restrict__ H, float const* __restrict__ Z, float const* __restrict__ wo,
                float const* __restrict__ bo_p, float const* __restrict__ y, int64_t const* __restrict__ perm,
                int64_t nvalid, int64_t rows, float* __restrict__ pred, __bf16* __restrict__ D, __bf16* __restrict__ DT,
                float* __restrict__ part_wo, float* __restrict__ part_sc)
 {
+    constexpr int kTs = kOblRows + 8; // LDS row stride of the transposed tile
+    __shared__ __attribute__((aligned(16))) __bf16 dt[kH * kTs];
+    __shared__ float sw[4][kH];
+    __shared__ float ss[4][2];
     int const lane = threadIdx.x & 63;
     int const wave = threadIdx.x >> 6;
-    int64_t const per = (rows + kObl - 1) / kObl;
-    int64_t const r0 = int64_t(blockIdx.x) * per;
-    int64_t const r1 = r0 + per < rows ? r0 + per : rows;
+    int64_t const r0 = int64_t(blockIdx.x) * kOblRows;
     float w[4];
     for (int j = 0; j < 4; ++j)
         w[j] = wo[4 * lane + j];
@@ -302,8 +326,9 @@ __global__ void __launch_bounds__(kB)
     float const inv = 1.0f / float(nvalid);
     float gw[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     float gb = 0.0f, loss = 0.0f;
-    for (int64_t m = r0 + wave; m < r1; m += 4)
+    for (int mr = wave; mr < kOblRows; mr += 4)
     {
+        int64_t const m = r0 + mr;
         bf16x4 const hv = *reinterpret_cast<bf16x4 const*>(H + m * kH + 4 * lane);
         float h[4];
         float s = 0.0f;
@@ -329,17 +354,14 @@ __global__ void __launch_bounds__(kB)
         {
             dv[j] = __bf16(g * w[j] * hsilu_grad(zv[j]));
             gw[j] += g * h[j];
+            dt[(4 * lane + j) * kTs + mr] = dv[j];
         }
         *reinterpret_cast<bf16x4*>(D + m * kH + 4 * lane) = dv;
-        for (int j = 0; j < 4; ++j)
-            DT[int64_t(4 * lane + j) * rows + m] = dv[j];
         gb += g;
         loss += valid ? (p - target) * (p - target) : 0.0f;
     }
     if (!D)
         return;
-    __shared__ float sw[4][kH];
-    __shared__ float ss[4][2];
     for (int j = 0; j < 4; ++j)
         sw[wave][4 * lane + j] = gw[j];
     if (lane == 0)
@@ -348,26 +370,36 @@ __global__ void __launch_bounds__(kB)
         ss[wave][1] = loss;
     }
     __syncthreads();
+    // DT[n][r0 .. r0 + 128): wave w writes units 64 w .. 64 w + 63, four units per store (16 lanes x 16 bytes each)
+#pragma unroll 4
+    for (int it = 0; it < 16; ++it)
+    {
+        int const n = 64 * wave + 4 * it + (lane >> 4);
+        int const c = (lane & 15) * 8;
+        *reinterpret_cast<bf16x8*>(DT + int64_t(n) * rows + r0 + c) = *reinterpret_cast<bf16x8 const*>(dt + n * kTs + c);
+    }
     int const t = threadIdx.x;
     part_wo[int64_t(blockIdx.x) * kH + t] = (sw[0][t] + sw[1][t]) + (sw[2][t] + sw[3][t]);
     if (t < 2)
         part_sc[int64_t(blockIdx.x) * 2 + t] = (ss[0][t] + ss[1][t]) + (ss[2][t] + ss[3][t]);
 }
 
-// Reduces the output-layer partials: g_wo, g_bo, and adds the batch's squared-error sum to loss_acc (double).
+// Reduces the output-layer partials of the kernel's nblk blocks (in block order): g_wo, g_bo, and adds the batch's
+// squared-error sum to loss_acc (double).
 __global__ void __launch_bounds__(kB) k_out_final(float const* __restrict__ part_wo, float const* __restrict__ part_sc,
-                                                  float* __restrict__ g_wo, float* __restrict__ g_bo,
+                                                  int64_t nblk, float* __restrict__ g_wo, float* __restrict__ g_bo,
                                                   double* __restrict__ loss_acc)
 {
     int const t = threadIdx.x;
     float s = 0.0f;
-    for (int64_t p = 0; p < kObl; ++p)
+#pragma unroll 8
+    for (int64_t p = 0; p < nblk; ++p)
         s += part_wo[p * kH + t];
     g_wo[t] = s;
     if (t == 0)
     {
         float gb = 0.0f, l = 0.0f;
-        for (int64_t p = 0; p < kObl; ++p)
+        for (int64_t p = 0; p < nblk; ++p)
         {
             gb += part_sc[2 * p];
             l += part_sc[2 * p + 1];
@@ -1254,8 +1286,8 @@ struct train_ws
             AGWN_TRY(DT[i].alloc(R * kH * sizeof(__bf16), s));
         }
         AGWN_TRY(part.alloc(size_t(parts) * kH * kH * sizeof(float), s));
-        AGWN_TRY(pwo.alloc(size_t(kObl) * kH * sizeof(float), s));
-        AGWN_TRY(psc.alloc(size_t(kObl) * 2 * sizeof(float), s));
+        AGWN_TRY(pwo.alloc(size_t(rows / kOblRows) * kH * sizeof(float), s));
+        AGWN_TRY(psc.alloc(size_t(rows / kOblRows) * 2 * sizeof(float), s));
         AGWN_TRY(perm.alloc(R * sizeof(int64_t), s));
         AGWN_TRY(loss.alloc(sizeof(double), s));
         return ok;
@@ -1282,14 +1314,14 @@ int pass(mlp_impl& M, train_ws& ws, int64_t nvalid, float const* y, int64_t cons
         k_gemm<EPI_FWD><<<dim3(unsigned(rows / 128), kH / 128, 1), blk, 0, s>>>(M.wt[l], K, x, K, kH, rows, K, K, E);
     }
     int const top = (kL - 1) & 1;
-    k_out_loss<<<dim3(unsigned(kObl)), blk, 0, s>>>(
+    k_out_loss<<<dim3(unsigned(rows / kOblRows)), blk, 0, s>>>(
         ws.H[kL - 1].as<__bf16>(), ws.Z[kL - 1].as<float>(), M.w + kLay.wo, M.w + kLay.bo, y, perm, nvalid, rows, pred,
         bwd ? ws.D[top].as<__bf16>() : nullptr, bwd ? ws.DT[top].as<__bf16>() : nullptr, ws.pwo.as<float>(),
         ws.psc.as<float>());
     AGWN_HIP(hipGetLastError());
     if (!bwd)
         return ok;
-    k_out_final<<<dim3(1), blk, 0, s>>>(ws.pwo.as<float>(), ws.psc.as<float>(), M.g + kLay.wo, M.g + kLay.bo,
+    k_out_final<<<dim3(1), blk, 0, s>>>(ws.pwo.as<float>(), ws.psc.as<float>(), rows / kOblRows, M.g + kLay.wo, M.g + kLay.bo,
                                         ws.loss.as<double>());
     for (int l = kL - 1; l >= 0; --l)
     {
