@@ -127,7 +127,8 @@ __global__ void __launch_bounds__(kB) k_features(float const* __restrict__ pts, 
 
 // ---------------------------------------------------------------------------------------------------------------------
 // Training GEMM: C[r][c] = sum_{k in chunk} A[r*lda + k] * B[c*ldb + k], r < R, c < Cn. Block tile 128 x 128 (4 waves,
-// 2 x 2, each 64 x 64 = 2 x 2 MFMA blocks); R, Cn multiples of 64, K chunk a multiple of 16. blockIdx.x tiles c,
+// 2 x 2, each 64 x 64 = 2 x 2 MFMA blocks; operands staged through LDS 256 k at a time); R, Cn multiples of 64, K
+// chunk a multiple of 16. blockIdx.x tiles c,
 // blockIdx.y tiles r, blockIdx.z splits K (partial sums, EPI_PART). The epilogue gets each lane's runs of 4
 // consecutive rows r..r+3 at one column c.
 // ---------------------------------------------------------------------------------------------------------------------
@@ -154,49 +155,80 @@ __global__ void __launch_bounds__(256)
     k_gemm(__bf16 const* __restrict__ A, int64_t lda, __bf16 const* __restrict__ B, int64_t ldb, int64_t R, int64_t Cn,
            int64_t K, int64_t kchunk, epi_args E)
 {
+    // The block's 128 rows of A and 128 rows of B are staged in LDS 256 k at a time (every load of a stage in flight
+    // at once, one round trip), then the waves' MFMAs read their operands from LDS. 528-byte LDS rows keep the 16-lane
+    // groups of ds_read_b128 conflict-free.
+    constexpr int kS = 256 + 8;
+    __shared__ __attribute__((aligned(16))) __bf16 sa[128 * kS];
+    __shared__ __attribute__((aligned(16))) __bf16 sb[128 * kS];
     int const lane = threadIdx.x & 63;
     int const wave = threadIdx.x >> 6;
-    int64_t const r0 = int64_t(blockIdx.y) * 128 + (wave >> 1) * 64;
-    int64_t const c0 = int64_t(blockIdx.x) * 128 + (wave & 1) * 64;
-    if (r0 >= R || c0 >= Cn)
-        return;
+    int64_t const rb = int64_t(blockIdx.y) * 128, cb = int64_t(blockIdx.x) * 128;
+    int64_t const r0 = rb + (wave >> 1) * 64;
+    int64_t const c0 = cb + (wave & 1) * 64;
+    bool const active = r0 < R && c0 < Cn;
     int64_t const kb = int64_t(blockIdx.z) * kchunk;
     int64_t const ke = kb + kchunk < K ? kb + kchunk : K;
     int const li = lane & 31;
     int const kh = (lane >> 5) * 8;
-    __bf16 const* pa[2] = {A + (r0 + li) * lda + kh, A + (r0 + 32 + li) * lda + kh};
-    __bf16 const* pb[2] = {B + (c0 + li) * ldb + kh, B + (c0 + 32 + li) * ldb + kh};
     f32x16 acc[2][2] = {};
-    bf16x8 a[2], b[2];
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int64_t k1 = kb; k1 < ke; k1 += 256)
     {
-        a[i] = *reinterpret_cast<bf16x8 const*>(pa[i] + kb);
-        b[i] = *reinterpret_cast<bf16x8 const*>(pb[i] + kb);
-    }
-    for (int64_t k0 = kb; k0 < ke; k0 += 16)
-    {
-        // prefetch the next k step (the last iteration re-reads its own step)
-        int64_t const kn = k0 + 16 < ke ? k0 + 16 : k0;
-        bf16x8 an[2], bn[2];
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
+        int const kn = int(ke - k1 < 256 ? ke - k1 : 256); // a multiple of 16
+        int const cpr = kn / 8;                           // 16-byte chunks per row
+        int const nch = 128 * cpr;
+        for (int q0 = threadIdx.x; q0 < nch; q0 += 8 * 256)
         {
-            an[i] = *reinterpret_cast<bf16x8 const*>(pa[i] + kn);
-            bn[i] = *reinterpret_cast<bf16x8 const*>(pb[i] + kn);
+            bf16x8 va[8], vb[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+            {
+                int const q = q0 + 256 * u;
+                int const row = q / cpr, ch = q - row * cpr;
+                va[u] = bf16x8{};
+                vb[u] = bf16x8{};
+                if (q < nch && rb + row < R)
+                    va[u] = *reinterpret_cast<bf16x8 const*>(A + (rb + row) * lda + k1 + 8 * ch);
+                if (q < nch && cb + row < Cn)
+                    vb[u] = *reinterpret_cast<bf16x8 const*>(B + (cb + row) * ldb + k1 + 8 * ch);
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+            {
+                int const q = q0 + 256 * u;
+                int const row = q / cpr, ch = q - row * cpr;
+                if (q < nch)
+                {
+                    *reinterpret_cast<bf16x8*>(sa + row * kS + 8 * ch) = va[u];
+                    *reinterpret_cast<bf16x8*>(sb + row * kS + 8 * ch) = vb[u];
+                }
+            }
         }
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
-#pragma unroll
-            for (int j = 0; j < 2; ++j)
-                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
+        __syncthreads();
+        if (active)
         {
-            a[i] = an[i];
-            b[i] = bn[i];
+            __bf16 const* la[2] = {sa + (r0 - rb + li) * kS + kh, sa + (r0 - rb + 32 + li) * kS + kh};
+            __bf16 const* lb[2] = {sb + (c0 - cb + li) * kS + kh, sb + (c0 - cb + 32 + li) * kS + kh};
+            for (int k0 = 0; k0 < kn; k0 += 16)
+            {
+                bf16x8 a[2], b[2];
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+                {
+                    a[i] = *reinterpret_cast<bf16x8 const*>(la[i] + k0);
+                    b[i] = *reinterpret_cast<bf16x8 const*>(lb[i] + k0);
+                }
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int j = 0; j < 2; ++j)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+            }
         }
+        __syncthreads(); // the stage has been read before the next one overwrites it
     }
+    if (!active)
+        return;
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -326,39 +358,62 @@ __global__ void __launch_bounds__(kB)
     float const inv = 1.0f / float(nvalid);
     float gw[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     float gb = 0.0f, loss = 0.0f;
-    for (int mr = wave; mr < kOblRows; mr += 4)
+    // four rows per iteration (rows mr, mr + 4, mr + 8, mr + 12 of the wave's stride-4 share): their loads and
+    // reductions are independent, the accumulations keep the row order
+    constexpr int kU = 4;
+    for (int mr0 = wave; mr0 < kOblRows; mr0 += 4 * kU)
     {
-        int64_t const m = r0 + mr;
-        bf16x4 const hv = *reinterpret_cast<bf16x4 const*>(H + m * kH + 4 * lane);
-        float h[4];
-        float s = 0.0f;
-        for (int j = 0; j < 4; ++j)
+        float h[kU][4], s[kU], zv[kU][4], target[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u)
         {
-            h[j] = float(hv[j]);
-            s += h[j] * w[j];
+            int64_t const m = r0 + mr0 + 4 * u;
+            bf16x4 const hv = *reinterpret_cast<bf16x4 const*>(H + m * kH + 4 * lane);
+            s[u] = 0.0f;
+            for (int j = 0; j < 4; ++j)
+            {
+                h[u][j] = float(hv[j]);
+                s[u] += h[u][j] * w[j];
+            }
+            if (D)
+            {
+                float4 const zz = *reinterpret_cast<float4 const*>(Z + m * kH + 4 * lane);
+                zv[u][0] = zz.x;
+                zv[u][1] = zz.y;
+                zv[u][2] = zz.z;
+                zv[u][3] = zz.w;
+                target[u] = m < nvalid ? y[perm ? perm[m] : m] : 0.0f;
+            }
         }
+#pragma unroll
         for (int off = 32; off > 0; off >>= 1)
-            s += __shfl_xor(s, off);
-        float const p = s + bo;
-        bool const valid = m < nvalid;
-        if (pred && lane == 0 && valid)
-            pred[m] = p;
-        if (!D)
-            continue;
-        float const target = valid ? y[perm ? perm[m] : m] : p;
-        float const g = valid ? 2.0f * (p - target) * inv : 0.0f;
-        float4 const zz = *reinterpret_cast<float4 const*>(Z + m * kH + 4 * lane);
-        float const zv[4] = {zz.x, zz.y, zz.z, zz.w};
-        bf16x4 dv;
-        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int u = 0; u < kU; ++u)
+                s[u] += __shfl_xor(s[u], off);
+#pragma unroll
+        for (int u = 0; u < kU; ++u)
         {
-            dv[j] = __bf16(g * w[j] * hsilu_grad(zv[j]));
-            gw[j] += g * h[j];
-            dt[(4 * lane + j) * kTs + mr] = dv[j];
+            int const mr = mr0 + 4 * u;
+            int64_t const m = r0 + mr;
+            float const p = s[u] + bo;
+            bool const valid = m < nvalid;
+            if (pred && lane == 0 && valid)
+                pred[m] = p;
+            if (!D)
+                continue;
+            float const tg = valid ? target[u] : p;
+            float const g = valid ? 2.0f * (p - tg) * inv : 0.0f;
+            bf16x4 dv;
+            for (int j = 0; j < 4; ++j)
+            {
+                dv[j] = __bf16(g * w[j] * hsilu_grad(zv[u][j]));
+                gw[j] += g * h[u][j];
+                dt[(4 * lane + j) * kTs + mr] = dv[j];
+            }
+            *reinterpret_cast<bf16x4*>(D + m * kH + 4 * lane) = dv;
+            gb += g;
+            loss += valid ? (p - tg) * (p - tg) : 0.0f;
         }
-        *reinterpret_cast<bf16x4*>(D + m * kH + 4 * lane) = dv;
-        gb += g;
-        loss += valid ? (p - target) * (p - target) : 0.0f;
     }
     if (!D)
         return;
@@ -392,21 +447,40 @@ __global__ void __launch_bounds__(kB) k_out_final(float const* __restrict__ part
 {
     int const t = threadIdx.x;
     float s = 0.0f;
-#pragma unroll 8
-    for (int64_t p = 0; p < nblk; ++p)
+    int64_t p = 0;
+    for (; p + 8 <= nblk; p += 8)
+    {
+        float v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            v[k] = part_wo[(p + k) * kH + t];
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            s += v[k];
+    }
+    for (; p < nblk; ++p)
         s += part_wo[p * kH + t];
     g_wo[t] = s;
-    if (t == 0)
+    if (t < 64)
     {
+        // lane t sums blocks t, t + 64, ... in order; the lanes' sums are then combined in a fixed tree
         float gb = 0.0f, l = 0.0f;
-        for (int64_t p = 0; p < nblk; ++p)
+        for (int64_t q = t; q < nblk; q += 64)
         {
-            gb += part_sc[2 * p];
-            l += part_sc[2 * p + 1];
+            gb += part_sc[2 * q];
+            l += part_sc[2 * q + 1];
         }
-        *g_bo = gb;
-        if (loss_acc)
-            *loss_acc += double(l);
+        for (int off = 32; off > 0; off >>= 1)
+        {
+            gb += __shfl_xor(gb, off);
+            l += __shfl_xor(l, off);
+        }
+        if (t == 0)
+        {
+            *g_bo = gb;
+            if (loss_acc)
+                *loss_acc += double(l);
+        }
     }
 }
 
