@@ -155,9 +155,9 @@ __global__ void __launch_bounds__(256)
     k_gemm(__bf16 const* __restrict__ A, int64_t lda, __bf16 const* __restrict__ B, int64_t ldb, int64_t R, int64_t Cn,
            int64_t K, int64_t kchunk, epi_args E)
 {
-    // The block's 128 rows of A and 128 rows of B are staged in LDS 256 k at a time (every load of a stage in flight
-    // at once, one round trip), then the waves' MFMAs read their operands from LDS. 528-byte LDS rows keep the 16-lane
-    // groups of ds_read_b128 conflict-free.
+    // The block's 128 rows of A and 128 rows of B are staged in LDS 256 k at a time (each thread keeps sixteen 16-byte
+    // loads in flight: two round trips per stage), then the waves' MFMAs read their operands from LDS. 528-byte LDS
+    // rows keep the 16-lane groups of ds_read_b128 conflict-free.
     constexpr int kS = 256 + 8;
     __shared__ __attribute__((aligned(16))) __bf16 sa[128 * kS];
     __shared__ __attribute__((aligned(16))) __bf16 sb[128 * kS];
@@ -336,7 +336,7 @@ constexpr int64_t kOblRows = 128; // rows per block of the output-layer kernel (
 //   pred = H3 . wo + bo;  g = 2 (pred - y) / nvalid;  dz3 = bf16(g wo hsilu'(Z3));  per-block partials of g H3 (dwo),
 //   g (dbo) and (pred - y)^2.
 // D is written row by row; its transpose DT goes through an LDS tile of the block's rows and is written unit by unit,
-// 8 lanes per 256-byte unit row (whole cache lines). y is read through perm (sample order of the batch). Without D
+// 16 lanes per 256-byte unit row (whole cache lines). y is read through perm (sample order of the batch). Without D
 // only pred is written.
 __global__ void __launch_bounds__(kB)
     k_out_loss(__bf16 const* __restrict__ H, float const* __restrict__ Z, float const* __restrict__ wo,
