@@ -99,6 +99,25 @@ def test_adam_first_step(gpu, c1):
     assert np.mean(err > 2e-2 * lr) < 2e-3
 
 
+def test_gradient_of_a_large_batch(gpu, c1):
+    """A batch of 131,072 samples (the weight-gradient GEMM's batch chunks are then 2048 long: eight LDS stages each)
+    made of 32 copies of 4096 samples has the mean-squared-error gradient of those 4096 (one stage per chunk):
+    within 1e-4 of the gradient's norm per parameter block (float32 sums in another grouping)."""
+    v, idx, lo, hi = c1
+    rng = np.random.default_rng(4)
+    n = 4096
+    pts = (lo + (hi - lo) * rng.random((n, 3))).astype(np.float32)
+    y = rng.random(n).astype(np.float32)
+    with gpu.GwnSurrogate(lo, hi, seed=9) as net:
+        g1, p1, l1 = net.grad(pts, y)
+        g32, p32, l32 = net.grad(np.tile(pts, (32, 1)), np.tile(y, 32))
+    assert np.array_equal(p32.reshape(32, n)[7], p1)  # the forward pass does not depend on the batch
+    errs = block_errors(g32.astype(np.float64), g1.astype(np.float64))
+    print("large-batch gradient block errors:", {k: f"{e:.1e}" for k, e in errs.items()})
+    assert max(errs.values()) <= 1e-4, errs
+    assert abs(l32 - l1) <= 1e-5 * l1
+
+
 @pytest.fixture(scope="module")
 def trained_c1(gpu, c1):
     """A network fitted to the small open mesh (icosphere cap): 2^18 training and 2^16 validation samples, top
