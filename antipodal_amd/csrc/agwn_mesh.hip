@@ -313,7 +313,82 @@ __device__ __forceinline__ int count_one(T px, T py, T pz, T dx, T dy, T dz, bvh
 // traversal would, with the same arithmetic, so the counts are the same; nodes are read once per wave (scalar loads)
 // and the control flow is uniform, which pays when a wave's rays are close and parallel (lattice rows, spatially
 // ordered queries). The stack (node, mask) lives in the wave's LDS slots; the larger subtree is pushed, as in
-// count_one, so it stays within kMaxStack.
+// count_one, so it stays within kMaxStack. packet_walk is the walk itself: it calls leaves(first, count) in each lane
+// whose ray hits the box of the leaf (bucket) [first, first + count).
+template <bool Buckets, class Leaves>
+__device__ __forceinline__ void packet_walk(float ox, float oy, float oz, bool valid, float ix, float iy, float iz,
+                                            bvh_node<float> const* __restrict__ nodes, int L, int32_t* stk_node,
+                                            uint64_t* stk_mask, int lane, Leaves&& leaves)
+{
+    uint64_t m = __ballot(valid); // lanes whose rays reach `node` (uniform)
+    int32_t node = 0;
+    int sp = 0;
+    while (m != 0)
+    {
+        bvh_node<float> const nd = nodes[node];
+        bool const act = (m >> lane) & 1ull;
+        bool const h0 = act && slab(nd.lo0, nd.hi0, ox, oy, oz, ix, iy, iz);
+        bool const h1 = act && slab(nd.lo1, nd.hi1, ox, oy, oz, ix, iy, iz);
+        uint64_t const m0 = __ballot(h0), m1 = __ballot(h1);
+        int32_t next = -1;
+        uint64_t nm = 0;
+        int32_t first = 0, count = 0;
+        if (m0)
+        {
+            if (bucket_span<Buckets>(nd.c0, nd.sz & 0xFFu, false, L, first, count))
+            {
+                if (h0)
+                    leaves(first, count);
+            }
+            else
+            {
+                next = nd.c0;
+                nm = m0;
+            }
+        }
+        if (m1)
+        {
+            if (bucket_span<Buckets>(nd.c1, nd.sz >> 8, true, L, first, count))
+            {
+                if (h1)
+                    leaves(first, count);
+            }
+            else if (next >= 0)
+            {
+                // both children internal: go down the one with fewer leaves, push the other
+                if (lane == 0)
+                {
+                    stk_node[sp] = nd.big ? nd.c1 : nd.c0;
+                    stk_mask[sp] = nd.big ? m1 : m0;
+                }
+                ++sp;
+                next = nd.big ? nd.c0 : nd.c1;
+                nm = nd.big ? m0 : m1;
+            }
+            else
+            {
+                next = nd.c1;
+                nm = m1;
+            }
+        }
+        if (next >= 0)
+        {
+            node = next;
+            m = nm;
+        }
+        else if (sp > 0)
+        {
+            --sp;
+            __builtin_amdgcn_wave_barrier();
+            node = __builtin_amdgcn_readfirstlane(stk_node[sp]);
+            m = stk_mask[sp];
+            __builtin_amdgcn_wave_barrier();
+        }
+        else
+            m = 0;
+    }
+}
+
 template <bool Buckets, class T>
 __device__ __forceinline__ int count_packet(T px, T py, T pz, bool valid, T dx, T dy, T dz,
                                             bvh_node<float> const* __restrict__ nodes,
@@ -322,86 +397,17 @@ __device__ __forceinline__ int count_packet(T px, T py, T pz, bool valid, T dx, 
                                             int lane)
 {
     int sum = 0;
-    auto leaves = [&](int32_t first, int32_t count) __attribute__((always_inline)) {
-        for (int32_t k = first; k < first + count; ++k)
-        {
-            leaf_rec<T> const r = rec[k];
-            if (r.live)
-                sum += mt_test(r, px, py, pz, dx, dy, dz);
-        }
-    };
     if (use_bvh)
-    {
-        float const ox = float(px), oy = float(py), oz = float(pz);
-        float const ix = inv_dir(double(dx)), iy = inv_dir(double(dy)), iz = inv_dir(double(dz));
-        uint64_t m = __ballot(valid); // lanes whose rays reach `node` (uniform)
-        int32_t node = 0;
-        int sp = 0;
-        while (m != 0)
-        {
-            bvh_node<float> const nd = nodes[node];
-            bool const act = (m >> lane) & 1ull;
-            bool const h0 = act && slab(nd.lo0, nd.hi0, ox, oy, oz, ix, iy, iz);
-            bool const h1 = act && slab(nd.lo1, nd.hi1, ox, oy, oz, ix, iy, iz);
-            uint64_t const m0 = __ballot(h0), m1 = __ballot(h1);
-            int32_t next = -1;
-            uint64_t nm = 0;
-            int32_t first = 0, count = 0;
-            if (m0)
-            {
-                if (bucket_span<Buckets>(nd.c0, nd.sz & 0xFFu, false, L, first, count))
-                {
-                    if (h0)
-                        leaves(first, count);
-                }
-                else
-                {
-                    next = nd.c0;
-                    nm = m0;
-                }
-            }
-            if (m1)
-            {
-                if (bucket_span<Buckets>(nd.c1, nd.sz >> 8, true, L, first, count))
-                {
-                    if (h1)
-                        leaves(first, count);
-                }
-                else if (next >= 0)
-                {
-                    // both children internal: go down the one with fewer leaves, push the other
-                    if (lane == 0)
-                    {
-                        stk_node[sp] = nd.big ? nd.c1 : nd.c0;
-                        stk_mask[sp] = nd.big ? m1 : m0;
-                    }
-                    ++sp;
-                    next = nd.big ? nd.c0 : nd.c1;
-                    nm = nd.big ? m0 : m1;
-                }
-                else
-                {
-                    next = nd.c1;
-                    nm = m1;
-                }
-            }
-            if (next >= 0)
-            {
-                node = next;
-                m = nm;
-            }
-            else if (sp > 0)
-            {
-                --sp;
-                __builtin_amdgcn_wave_barrier();
-                node = __builtin_amdgcn_readfirstlane(stk_node[sp]);
-                m = stk_mask[sp];
-                __builtin_amdgcn_wave_barrier();
-            }
-            else
-                m = 0;
-        }
-    }
+        packet_walk<Buckets>(float(px), float(py), float(pz), valid, inv_dir(double(dx)), inv_dir(double(dy)),
+                             inv_dir(double(dz)), nodes, L, stk_node, stk_mask, lane,
+                             [&](int32_t first, int32_t count) __attribute__((always_inline)) {
+                                 for (int32_t k = first; k < first + count; ++k)
+                                 {
+                                     leaf_rec<T> const r = rec[k];
+                                     if (r.live)
+                                         sum += mt_test(r, px, py, pz, dx, dy, dz);
+                                 }
+                             });
     if (valid)
         for (int32_t b = 0; b < nb; ++b)
             sum += mt_test(rec[brute[b]], px, py, pz, dx, dy, dz);
@@ -491,6 +497,168 @@ __global__ void __launch_bounds__(kTB)
         out[i] = count_one<false>(px, py, pz, dx, dy, dz, nodes, rec, brute, *nbrute_p, use_bvh, 1, stack,
                                   int(threadIdx.x));
     }
+}
+
+// Integer terms of lattice points by rows. When d = +-e_k and the lattice is axis-aligned with its first axis along
+// e_k, the points of a lattice row differ only in coordinate k, and for such a d the replica's u and v do not depend on
+// that coordinate (h_k = d_j e2_l - d_l e2_j = +-0 and d . q = +-q_k, with q_k free of s_k): the points of a row pass
+// u and v on the same triangles and differ only in t. A live triangle that the replica counts for a point p has its
+// crossing, which shares p's other two coordinates, inside its leaf box (the box holds every point where the replica
+// can report a crossing -- the property the traversal relies on). So the triangles any point of a row counts are among
+// those whose leaf box spans the row's two coordinates and which pass u and v there: each live triangle is binned into
+// those rows (k_row_bin), and each point tests its row's triangles, plus the exhaustive list, with its own coordinates
+// -- the same tests as its traversal, so the same count. A row binned more than kRowHits triangles is counted point by
+// point.
+constexpr int kRowHits = 64;
+constexpr int64_t kMaxRows = int64_t(1) << 20; // lattice rows (ny nz) up to which the row counts are used
+
+// Axis-aligned lattice in row form: coordinate k (the rows' axis) of point (ix, ., .) is xk(ix); coordinate a of row
+// iy is ya(iy) and coordinate b of row iz is zb(iz), each the float expression of query_at with the zero terms dropped
+// (adding a zero does not change a nonzero value, and the test's comparisons do not see the sign of a zero).
+struct row_frame
+{
+    int k, a, b;
+    int fwd; // d = +e_k
+    float sa, ha, sb, hb;
+    int64_t ny, nz;
+};
+
+// first index i of [0, n) with lo <= s + h i <= hi in float (none: n); the values are monotone in i, so the indices
+// that satisfy the test are contiguous, and the walk from the estimate to their start is exact whatever its error
+__device__ __forceinline__ int64_t first_in(float s, float h, int64_t n, float lo, float hi)
+{
+#pragma clang fp contract(off)
+    auto val = [&](int64_t i) { return s + h * float(i); };
+    auto before = [&](int64_t i) { return h > 0.0f ? val(i) < lo : val(i) > hi; };
+    double const t = (double(h > 0.0f ? lo : hi) - double(s)) / double(h);
+    int64_t e = t <= 0.0 ? 0 : (t >= double(n - 1) ? n - 1 : int64_t(t));
+    while (e > 0 && !before(e - 1))
+        --e;
+    while (e < n && before(e))
+        ++e;
+    return e;
+}
+
+// The replica's u and v tests (intersector.hh:81-93): 1 when both pass. The same operations as mt_test.
+template <class T>
+__device__ __forceinline__ bool mt_uv(leaf_rec<T> const& r, T px, T py, T pz, T dx, T dy, T dz)
+{
+#pragma clang fp contract(off)
+    T const sx = px - r.ax, sy = py - r.ay, sz = pz - r.az;
+    T const u = r.inv * (sx * r.hx + sy * r.hy + sz * r.hz);
+    if (u < T(0) || u > T(1))
+        return false;
+    T const qx = sy * r.e1z - sz * r.e1y;
+    T const qy = sz * r.e1x - sx * r.e1z;
+    T const qz = sx * r.e1y - sy * r.e1x;
+    T const v = r.inv * (dx * qx + dy * qy + dz * qz);
+    return !(v < T(0) || u + v > T(1));
+}
+
+// Bins each live triangle into the rows its leaf box spans where it passes u and v: hits[row][slot] = (triangle, far
+// side of its box along d: hi_k for d = +e_k, lo_k for -e_k), nh[row] = the number binned (zeroed before; more than kRowHits: the list is incomplete and unused). xk0: coordinate k of the rows'
+// first points (any value gives the same u and v).
+__global__ void __launch_bounds__(kB)
+    k_row_bin(row_frame F, float xk0, int64_t ntri, float dx, float dy, float dz, leaf_rec<float> const* __restrict__ rec,
+              float const* __restrict__ lo, float const* __restrict__ hi, int2* __restrict__ hits,
+              int32_t* __restrict__ nh)
+{
+#pragma clang fp contract(off)
+    int64_t const t = int64_t(blockIdx.x) * kB + threadIdx.x;
+    if (t >= ntri)
+        return;
+    leaf_rec<float> const r = rec[t];
+    if (!r.live)
+        return;
+    float const la = lo[3 * t + F.a], ha_ = hi[3 * t + F.a], lb = lo[3 * t + F.b], hb_ = hi[3 * t + F.b];
+    int64_t const iy0 = first_in(F.sa, F.ha, F.ny, la, ha_);
+    int64_t const iz0 = first_in(F.sb, F.hb, F.nz, lb, hb_);
+    for (int64_t iz = iz0; iz < F.nz; ++iz)
+    {
+        float const zb = F.sb + F.hb * float(iz);
+        if (!(lb <= zb && zb <= hb_))
+            break;
+        for (int64_t iy = iy0; iy < F.ny; ++iy)
+        {
+            float const ya = F.sa + F.ha * float(iy);
+            if (!(la <= ya && ya <= ha_))
+                break;
+            float const px = F.k == 0 ? xk0 : (F.a == 0 ? ya : zb);
+            float const py = F.k == 1 ? xk0 : (F.a == 1 ? ya : zb);
+            float const pz = F.k == 2 ? xk0 : (F.a == 2 ? ya : zb);
+            if (mt_uv(r, px, py, pz, dx, dy, dz))
+            {
+                int64_t const row = iz * F.ny + iy;
+                int const slot = atomicAdd(&nh[row], 1);
+                if (slot < kRowHits)
+                    hits[row * kRowHits + slot] = make_int2(int32_t(t), __float_as_int(F.fwd ? hi[3 * t + F.k]
+                                                                                              : lo[3 * t + F.k]));
+            }
+        }
+    }
+}
+
+// Integer terms from the rows' triangles and the exhaustive list, lane = point: all nx ny nz lattice points (ALL:
+// entry e = query e) or m routed entries (positions pts, index-ordered entries li, query index li >> 1). A row's hit
+// list holds (triangle, far side of its leaf box along d); a triangle whose box lies wholly behind the point is
+// skipped without its record (its test fails: a crossing the test reports lies in the box, ahead of the point). A
+// row with an incomplete list takes the packet traversal.
+template <bool ALL>
+__global__ void __launch_bounds__(kTB)
+    k_row_count(grid_desc g, int k, float const* __restrict__ pts, int64_t const* __restrict__ li, int64_t m,
+                float dx, float dy, float dz, bvh_node<float> const* __restrict__ nodes,
+                leaf_rec<float> const* __restrict__ rec, int32_t const* __restrict__ brute,
+                int32_t const* __restrict__ nbrute_p, int2 const* __restrict__ hits, int32_t const* __restrict__ nh,
+                int32_t* __restrict__ out)
+{
+    __shared__ int32_t stk_node[kTB / 64][kMaxStack];
+    __shared__ uint64_t stk_mask[kTB / 64][kMaxStack];
+    int64_t const e = int64_t(blockIdx.x) * kTB + threadIdx.x;
+    bool const valid = e < m;
+    float px = 0.0f, py = 0.0f, pz = 0.0f;
+    int64_t row = 0;
+    int n = 0;
+    if (valid)
+    {
+        int64_t ix;
+        if constexpr (ALL)
+        {
+            query_at(src_grid{g}, e, px, py, pz);
+            row = div_recip(e, g.nx, g.inx, ix);
+        }
+        else
+        {
+            px = pts[3 * e];
+            py = pts[3 * e + 1];
+            pz = pts[3 * e + 2];
+            row = div_recip(li[e] >> 1, g.nx, g.inx, ix);
+        }
+        n = nh[row];
+    }
+    int32_t const nb = *nbrute_p;
+    int c = 0;
+    bool const over = n > kRowHits;
+    if (__ballot(valid && over))
+        c = count_packet<false>(px, py, pz, valid && over, dx, dy, dz, nodes, rec, brute, nb, 1, 1,
+                                stk_node[threadIdx.x >> 6], stk_mask[threadIdx.x >> 6], int(threadIdx.x & 63));
+    if (valid && !over)
+    {
+        float const pk = k == 0 ? px : (k == 1 ? py : pz);
+        bool const fwd = (k == 0 ? dx : (k == 1 ? dy : dz)) > 0.0f;
+        int2 const* const h = hits + row * kRowHits;
+        for (int j = 0; j < n; ++j)
+        {
+            int2 const hv = h[j];
+            float const far = __int_as_float(hv.y);
+            if (fwd ? far < pk : far > pk)
+                continue;
+            c += mt_test(rec[hv.x], px, py, pz, dx, dy, dz);
+        }
+        for (int32_t b = 0; b < nb; ++b)
+            c += mt_test(rec[brute[b]], px, py, pz, dx, dy, dz);
+    }
+    if (valid)
+        out[e] = c;
 }
 
 // One prepared mesh of a batch, for the persistent counting kernel.
@@ -833,6 +1001,68 @@ struct mesh_t final : mesh_impl
         AGWN_HIP(hipGetLastError());
         // keep the state alive until the kernel has consumed it (it may be replaced by another thread's call)
         AGWN_HIP(hipStreamSynchronize(s));
+        return ok;
+    }
+
+    // Integer terms by rows (k_row_bin) when d = +-e_k and lattice g is axis-aligned with its rows along e_k; *done =
+    // false (nothing launched) otherwise. li null: all m = nx ny nz lattice points, in the lattice's order; else m
+    // routed points (positions pts; entries li in index order, query index li >> 1). All of norm <= qmax. Temporaries
+    // come from ws, or are allocated on s when ws is null.
+    int count_rows(grid_desc const& g, int64_t const* li, float const* pts, int64_t m, double qmax, float const d[3],
+                   int32_t* out, hipStream_t s, arena* ws, bool& done)
+    {
+        static_assert(std::is_same_v<T, float>, "row counts are FP32");
+        done = false;
+        int64_t const rows = g.ny * g.nz;
+        if (m == 0 || ntri < 2 || rows > kMaxRows)
+            return ok;
+        // the axis of each lattice step's one nonzero component (-1: none or several)
+        auto axis = [](float const* v) {
+            int ax = -1, nz = 0;
+            for (int c = 0; c < 3; ++c)
+                if (v[c] != 0.0f)
+                {
+                    ax = c;
+                    ++nz;
+                }
+            return nz == 1 ? ax : -1;
+        };
+        int const k = axis(g.dx), a = axis(g.dy), b = axis(g.dz);
+        if (k < 0 || a < 0 || b < 0 || k == a || k == b || a == b || axis(d) != k || std::fabs(d[k]) != 1.0f)
+            return ok;
+        row_frame const F{k, a, b, d[k] > 0.0f ? 1 : 0, g.start[a], g.dy[a], g.start[b], g.dz[b], g.ny, g.nz};
+        std::shared_ptr<dir_state<T>> st;
+        AGWN_TRY(prepare(d, qmax, s, st));
+        int32_t* nh = nullptr;
+        int2* hits = nullptr;
+        dbuf bn, bh;
+        if (ws)
+        {
+            AGWN_TRY(ws->take_n(size_t(rows), &nh));
+            AGWN_TRY(ws->take_n(size_t(rows) * kRowHits, &hits));
+        }
+        else
+        {
+            AGWN_TRY(bn.alloc(size_t(rows) * sizeof(int32_t), s));
+            AGWN_TRY(bh.alloc(size_t(rows) * kRowHits * sizeof(int2), s));
+            nh = bn.as<int32_t>();
+            hits = bh.as<int2>();
+        }
+        AGWN_HIP(hipMemsetAsync(nh, 0, size_t(rows) * sizeof(int32_t), s));
+        k_row_bin<<<dim3(unsigned((ntri + kB - 1) / kB)), dim3(kB), 0, s>>>(F, g.start[k], ntri, d[0], d[1], d[2],
+                                                                         st->rec, st->lo, st->hi, hits, nh);
+        AGWN_HIP(hipGetLastError());
+        int64_t const blocks = (m + kTB - 1) / kTB;
+        AGWN_CHECK(blocks < (int64_t(1) << 31), "too many queries (%lld)", (long long)m);
+        if (li)
+            k_row_count<false><<<dim3(unsigned(blocks)), dim3(kTB), 0, s>>>(
+                g, k, pts, li, m, d[0], d[1], d[2], st->nodes, st->rec, st->brute, st->nbrute, hits, nh, out);
+        else
+            k_row_count<true><<<dim3(unsigned(blocks)), dim3(kTB), 0, s>>>(
+                g, k, nullptr, nullptr, m, d[0], d[1], d[2], st->nodes, st->rec, st->brute, st->nbrute, hits, nh, out);
+        AGWN_HIP(hipGetLastError());
+        AGWN_HIP(hipStreamSynchronize(s)); // the state outlives its kernels
+        done = true;
         return ok;
     }
 };
@@ -1214,7 +1444,10 @@ int grid_eval_dev(agwn_grid* g, float const* start, float const* dx, float const
     AGWN_CHECK(g->mesh != nullptr, "set_scene was not called");
     dbuf cnt;
     AGWN_TRY(cnt.alloc(size_t(c.n) * sizeof(int32_t), s));
-    AGWN_TRY(g->mesh->count_src(src_grid{c.gd}, c.n, c.qmax, c.x0f, cnt.as<int32_t>(), s, true));
+    bool rows = false;
+    AGWN_TRY(g->mesh->count_rows(c.gd, nullptr, nullptr, c.n, c.qmax, c.x0f, cnt.as<int32_t>(), s, nullptr, rows));
+    if (!rows)
+        AGWN_TRY(g->mesh->count_src(src_grid{c.gd}, c.n, c.qmax, c.x0f, cnt.as<int32_t>(), s, true));
     return frac_f32_planned(src_grid{c.gd}, c.n, *g->plan, g->seg7, g->nseg, c.x0d, cnt.as<int32_t>(), out, s);
 }
 
@@ -1285,6 +1518,22 @@ int label_exact_levels(float const* pts, int32_t const* cnt, int64_t m, frac_pla
     k_scatter_values<<<dim3(gb), dim3(kB), 0, s>>>(und, nu, uw, w);
     AGWN_HIP(hipGetLastError());
     return ok;
+}
+
+// Integer terms of the m routed points (positions pts, entries li in index order): by lattice rows where the lattice
+// and the direction allow it (mesh_t::count_rows), else by the packet traversal over the points.
+template <class Src>
+int count_routed(mesh_t<float>& mesh, Src const& src, int64_t const* li, float const* pts, int64_t m, double qmax,
+                 float const x0f[3], int32_t* c, hipStream_t s, arena& ws)
+{
+    if constexpr (std::is_same_v<Src, src_grid>)
+    {
+        bool done = false;
+        AGWN_TRY(mesh.count_rows(src.g, li, pts, m, qmax, x0f, c, s, &ws, done));
+        if (done)
+            return ok;
+    }
+    return mesh.count_src(src_array<float>{pts}, m, qmax, x0f, c, s, true);
 }
 
 // Surrogate labelling with the exact FP32 path as fallback (see mlp_route): network labels for every query, then the
@@ -1360,7 +1609,7 @@ int label_surrogate(mesh_t<float>& mesh, agwn_mlp const* mlp, Src const& src, in
                 AGWN_TRY(ws.take_n(size_t(m), &c_i));
                 AGWN_TRY(ws.take_n(size_t(m), &lm));
                 AGWN_TRY(gather_routed(src, Li, m, pts_i, s));
-                AGWN_TRY(mesh.count_src(src_array<float>{pts_i}, m, qmax, x0f, c_i, s, true));
+                AGWN_TRY(count_routed(mesh, src, Li, pts_i, m, qmax, x0f, c_i, s, ws));
                 k_permute_routed<<<dim3(unsigned((m + kB - 1) / kB)), dim3(kB), 0, s>>>(perm, m, Li, pts_i, c_i,
                                                                                         lm, pts, c);
                 AGWN_HIP(hipGetLastError());
@@ -1369,7 +1618,7 @@ int label_surrogate(mesh_t<float>& mesh, agwn_mlp const* mlp, Src const& src, in
             else
             {
                 AGWN_TRY(gather_routed(src, Li, m, pts, s));
-                AGWN_TRY(mesh.count_src(src_array<float>{pts}, m, qmax, x0f, c, s, true));
+                AGWN_TRY(count_routed(mesh, src, Li, pts, m, qmax, x0f, c, s, ws));
             }
             frac_plan local;
             frac_plan const* pl = plan;

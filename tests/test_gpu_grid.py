@@ -126,6 +126,70 @@ def test_torus_grid_matches_batch(gpu, oracle):
     assert np.max(np.abs(got[cells] - base)) < 5e-4
 
 
+def _sheets(n: int = 40):
+    """n parallel squares x = const, y and z in [-0.5, 0.5] (two triangles each, split along y = z): an open surface
+    that a lattice row along x through the squares crosses n times."""
+    f = np.float32
+    xs = np.linspace(-0.9, 0.9, n)
+    corners = np.array([[0, -0.5, -0.5], [0, 0.5, -0.5], [0, 0.5, 0.5], [0, -0.5, 0.5]])
+    v = np.concatenate([corners + [x, 0, 0] for x in xs]).astype(f)
+    q = 4 * np.arange(n)[:, None]
+    idx = np.concatenate([q + [0, 1, 2], q + [0, 2, 3]], axis=1).reshape(-1).astype(np.int32)
+    return v, idx
+
+
+@pytest.mark.parametrize("case", ["plus_x", "minus_x", "rows_along_y", "rows_along_minus_z", "one_point_rows",
+                                  "sheets"])
+def test_grid_rows_along_the_ray(gpu, oracle, case):
+    """Lattices whose rows run along the integration direction (x0 = +-e_k: the lattice's first axis is its shortest
+    and steps along one world axis), where the integer term is counted per row: one ray per lattice row collects the
+    triangles its points can cross and each point tests those. Rows along +x, -x, +y and -z, rows of one point, and a
+    stack of 40 squares that every row through it crosses 40 times (more triangles than a row's list holds: those
+    rows' points are counted one by one). Every cell equals the batch path on the same points with the same x0 bit for
+    bit, and the integer terms equal NaiveIntersector<float>'s."""
+    f = np.float32
+    if case == "sheets":
+        v, idx = _sheets()
+    else:
+        verts, idx = mg.holed_torus_mesh(nu=96, nv=40, holes=20, hole_radius=0.1)
+        v = verts.astype(f)
+    bnd = gpu.build_boundary_segments(v, idx)
+    h, hz = f(3.0 / 15), f(1.0 / 17)
+    ex, ey, ez = np.eye(3, dtype=f)
+    nx, ny, nz = 16, 20, 18
+    start = np.array([-1.5, -1.5, -0.5], dtype=f)
+    dx, dy, dz = ex * h, ey * f(3.0 / 19), ez * hz
+    if case == "minus_x":
+        start, dx = np.array([1.5, -1.5, -0.5], dtype=f), ex * -h
+    elif case == "rows_along_y":
+        dx, dy = ey * h, ex * f(3.0 / 19)
+    elif case == "rows_along_minus_z":
+        start = np.array([-1.5, -1.5, 0.5], dtype=f)
+        dx, dz = ez * -hz, ex * f(3.0 / 19)
+    elif case == "one_point_rows":
+        nx = 1
+    elif case == "sheets":
+        start = np.array([-1.2, -0.8, -0.8], dtype=f)
+        dx, dy, dz = ex * f(2.4 / 15), ey * f(1.6 / 19), ez * f(1.6 / 19)
+        nz = 20
+    assert nx <= min(ny, nz)  # the first axis is the integration axis
+    with gpu.HipGridGwn() as g:
+        g.set_scene(v, idx, bnd)
+        got = g.eval_grid(start, dx, dy, dz, nx, ny, nz)
+    pts = lattice(start, dx, dy, dz, nx, ny, nz)
+    x0 = dx / f(np.sqrt(f(dx[0] * dx[0] + dx[1] * dx[1]) + f(dx[2] * dx[2])))
+    assert np.count_nonzero(x0) == 1 and np.abs(x0).max() == 1.0
+    with gpu.HipMesh(v, idx) as mesh:
+        batch = gpu.eval_gwnr_mesh_batch(mesh, pts, x0, bnd)
+        cnt = gpu.eval_gwnr_mesh_batch_integer(mesh, pts, x0)
+    ref_cnt = oracle.naive_count(v, idx, pts, x0, np.float32)
+    assert np.array_equal(cnt, ref_cnt)
+    assert np.array_equal(got, batch)
+    assert np.count_nonzero(ref_cnt) > 0
+    if case == "sheets":
+        assert np.abs(ref_cnt).max() >= 33  # rows whose triangles overflow a row's list
+
+
 def test_grid_rejects_bad_arguments(gpu):
     with gpu.HipGridGwn() as g:
         with pytest.raises(gpu.AgwnError):
