@@ -1236,11 +1236,13 @@ bool loops_of_stream(std::vector<float4> const& p, std::vector<loop_host>& loops
 }
 
 // The loop tree of the decision: the loops in the Morton order of their centres (10 bits per axis over the box of the
-// centres), halved recursively. A node's ball is the smallest one holding its children's balls (so it holds every fan
-// below it); its vector area and fan area are the children's sums and its first moment, about its centre,
-// sum_child M_child + (c_child - c) (x) a_child. The far-field expansion and bound hold for any surface inside the
-// ball, so a node is tested and estimated like one loop. Nodes in depth-first order, the root first, each with the
-// index just past its subtree (skip).
+// centres), halved recursively. A node's ball is centred where the smallest ball holding its children's balls is, with
+// the radius of the farthest loop point below it (no larger than that ball's: every point lies in a child's ball). It
+// holds every fan below, as each fan lies in the convex hull of its loop's points. Its vector area and fan area are the
+// children's sums and its first moment, about its centre, sum_child M_child + (c_child - c) (x) a_child. The far-field
+// expansion and bound hold for any surface inside the ball, so a node is tested and estimated like one loop. Nodes in
+// depth-first order, the root first, each with the index just past its subtree (skip). pts: the decision stream (the
+// loops' points).
 struct tree_node
 {
     loop_host L;
@@ -1248,7 +1250,7 @@ struct tree_node
     int32_t skip; // the first node after this one's subtree
 };
 
-void loop_tree(std::vector<loop_host> const& loops, std::vector<tree_node>& nodes)
+void loop_tree(std::vector<loop_host> const& loops, std::vector<float4> const& pts, std::vector<tree_node>& nodes)
 {
     size_t const n = loops.size();
     nodes.clear();
@@ -1339,6 +1341,18 @@ void loop_tree(std::vector<loop_host> const& loops, std::vector<tree_node>& node
             }
         }
         C.inertia = std::max(C.inertia, 0.0) + 1e-12 * slack;
+        double far2 = 0.0; // the farthest loop point below from the centre (squared)
+        for (size_t k = b; k < e; ++k)
+        {
+            loop_host const& L = loops[size_t(key[k].second)];
+            for (int64_t j = L.start; j < L.start + L.count; ++j)
+            {
+                float4 const& q = pts[size_t(j)];
+                double const u[3] = {double(q.x) - C.c[0], double(q.y) - C.c[1], double(q.z) - C.c[2]};
+                far2 = std::max(far2, u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+            }
+        }
+        C.rho = std::min(C.rho, std::sqrt(far2) * (1.0 + 1e-12));
         nodes[size_t(me)].L = C;
         nodes[size_t(me)].loop = -1;
         nodes[size_t(me)].skip = int32_t(nodes.size());
@@ -1355,6 +1369,7 @@ int frac_plan_build(float const* h, int64_t nseg, frac_plan& plan, hipStream_t s
     plan.loops.clear();
     plan.loops_ok = false;
     plan.dec_npts = 0;
+    plan.dec_host.clear();
     plan.walk_starts.clear();
     plan.sexp = boundary_scale_exp(h, nseg, &plan.far);
     std::vector<float4> stream_pts;
@@ -1382,10 +1397,11 @@ int frac_plan_build(float const* h, int64_t nseg, frac_plan& plan, hipStream_t s
     else
     {
         plan.dec_npts = int64_t(dec.size());
+        plan.dec_host = dec;
         AGWN_TRY(plan.dec.alloc(std::max<size_t>(dec.size(), 1) * sizeof(float4), stream));
         if (!dec.empty())
-            AGWN_HIP(hipMemcpyAsync(plan.dec.p, dec.data(), dec.size() * sizeof(float4), hipMemcpyHostToDevice,
-                                    stream));
+            AGWN_HIP(hipMemcpyAsync(plan.dec.p, plan.dec_host.data(), dec.size() * sizeof(float4),
+                                    hipMemcpyHostToDevice, stream));
     }
     AGWN_TRY(plan.buf.alloc(std::max<size_t>(stream_pts.size(), 1) * sizeof(float4), stream));
     if (!stream_pts.empty())
@@ -1601,7 +1617,7 @@ int frac_decide(float const* pos, int64_t m, int32_t const* counts, frac_plan co
         if (!plan.dec_state)
         {
             auto st = std::make_shared<decide_state>();
-            loop_tree(plan.loops, st->nodes);
+            loop_tree(plan.loops, plan.dec_host, st->nodes);
             plan.dec_state = st;
         }
         auto* st = static_cast<decide_state*>(plan.dec_state.get());

@@ -296,6 +296,7 @@ struct frac_plan
     std::vector<loop_host> loops; // closed loops (long walks split into arcs + a skeleton), in the decision stream
     bool loops_ok = false;
     dbuf dec;                     // decision stream: the loops' points, each loop contiguous (flag 0 at its start)
+    std::vector<float4> dec_host; // its host copy (the loop tree's node balls are fitted to the points)
     int64_t dec_npts = 0;
     // the decision's loop tree and, per pole, its device tables: built by the plan's first decision for that pole
     // (under dec_mu) and kept with the plan, read-only afterwards (agwn_frac.hip)

@@ -520,18 +520,19 @@ struct row_frame
     int k, a, b;
     int fwd; // d = +e_k
     float sa, ha, sb, hb;
+    float ia, ib; // 1 / ha, 1 / hb (estimates only)
     int64_t ny, nz;
 };
 
 // first index i of [0, n) with lo <= s + h i <= hi in float (none: n); the values are monotone in i, so the indices
 // that satisfy the test are contiguous, and the walk from the estimate to their start is exact whatever its error
-__device__ __forceinline__ int64_t first_in(float s, float h, int64_t n, float lo, float hi)
+__device__ __forceinline__ int64_t first_in(float s, float h, float inv_h, int64_t n, float lo, float hi)
 {
 #pragma clang fp contract(off)
     auto val = [&](int64_t i) { return s + h * float(i); };
     auto before = [&](int64_t i) { return h > 0.0f ? val(i) < lo : val(i) > hi; };
-    double const t = (double(h > 0.0f ? lo : hi) - double(s)) / double(h);
-    int64_t e = t <= 0.0 ? 0 : (t >= double(n - 1) ? n - 1 : int64_t(t));
+    float const t = ((h > 0.0f ? lo : hi) - s) * inv_h;
+    int64_t e = !(t > 0.0f) ? 0 : (t >= float(n - 1) ? n - 1 : int64_t(t));
     while (e > 0 && !before(e - 1))
         --e;
     while (e < n && before(e))
@@ -571,8 +572,8 @@ __global__ void __launch_bounds__(kB)
     if (!r.live)
         return;
     float const la = lo[3 * t + F.a], ha_ = hi[3 * t + F.a], lb = lo[3 * t + F.b], hb_ = hi[3 * t + F.b];
-    int64_t const iy0 = first_in(F.sa, F.ha, F.ny, la, ha_);
-    int64_t const iz0 = first_in(F.sb, F.hb, F.nz, lb, hb_);
+    int64_t const iy0 = first_in(F.sa, F.ha, F.ia, F.ny, la, ha_);
+    int64_t const iz0 = first_in(F.sb, F.hb, F.ib, F.nz, lb, hb_);
     for (int64_t iz = iz0; iz < F.nz; ++iz)
     {
         float const zb = F.sb + F.hb * float(iz);
@@ -1030,7 +1031,8 @@ struct mesh_t final : mesh_impl
         int const k = axis(g.dx), a = axis(g.dy), b = axis(g.dz);
         if (k < 0 || a < 0 || b < 0 || k == a || k == b || a == b || axis(d) != k || std::fabs(d[k]) != 1.0f)
             return ok;
-        row_frame const F{k, a, b, d[k] > 0.0f ? 1 : 0, g.start[a], g.dy[a], g.start[b], g.dz[b], g.ny, g.nz};
+        row_frame const F{k, a, b, d[k] > 0.0f ? 1 : 0, g.start[a], g.dy[a], g.start[b], g.dz[b], 1.0f / g.dy[a],
+                          1.0f / g.dz[b], g.ny, g.nz};
         std::shared_ptr<dir_state<T>> st;
         AGWN_TRY(prepare(d, qmax, s, st));
         int32_t* nh = nullptr;
