@@ -540,7 +540,7 @@ __device__ __forceinline__ int64_t first_in(float s, float h, float inv_h, int64
     return e;
 }
 
-// The replica's u and v tests (intersector.hh:81-93): 1 when both pass. The same operations as mt_test.
+// The replica's u and v tests (intersector.hh:81-93): true when both pass. The same operations as mt_test.
 template <class T>
 __device__ __forceinline__ bool mt_uv(leaf_rec<T> const& r, T px, T py, T pz, T dx, T dy, T dz)
 {
@@ -557,12 +557,13 @@ __device__ __forceinline__ bool mt_uv(leaf_rec<T> const& r, T px, T py, T pz, T 
 }
 
 // Bins each live triangle into the rows its leaf box spans where it passes u and v: hits[row][slot] = (triangle, far
-// side of its box along d: hi_k for d = +e_k, lo_k for -e_k), nh[row] = the number binned (zeroed before; more than kRowHits: the list is incomplete and unused). xk0: coordinate k of the rows'
-// first points (any value gives the same u and v).
+// side of its box along d: hi_k for d = +e_k, lo_k for -e_k), nh[row] = the number binned (cleared before; above
+// kRowHits the list is incomplete and unused). xk0: coordinate k of the rows' first points (any value gives the same u
+// and v).
 __global__ void __launch_bounds__(kB)
-    k_row_bin(row_frame F, float xk0, int64_t ntri, float dx, float dy, float dz, leaf_rec<float> const* __restrict__ rec,
-              float const* __restrict__ lo, float const* __restrict__ hi, int2* __restrict__ hits,
-              int32_t* __restrict__ nh)
+    k_row_bin(row_frame F, float xk0, int64_t ntri, float dx, float dy, float dz,
+              leaf_rec<float> const* __restrict__ rec, float const* __restrict__ lo, float const* __restrict__ hi,
+              int2* __restrict__ hits, int32_t* __restrict__ nh)
 {
 #pragma clang fp contract(off)
     int64_t const t = int64_t(blockIdx.x) * kB + threadIdx.x;
