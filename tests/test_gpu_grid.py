@@ -126,7 +126,7 @@ def test_torus_grid_matches_batch(gpu, oracle):
     assert np.max(np.abs(got[cells] - base)) < 5e-4
 
 
-def _sheets(n: int = 40):
+def _sheets(n: int):
     """n parallel squares x = const, y and z in [-0.5, 0.5] (two triangles each, split along y = z): an open surface
     that a lattice row along x through the squares crosses n times."""
     f = np.float32
@@ -139,17 +139,18 @@ def _sheets(n: int = 40):
 
 
 @pytest.mark.parametrize("case", ["plus_x", "minus_x", "rows_along_y", "rows_along_minus_z", "one_point_rows",
-                                  "sheets"])
+                                  "descending_y_z", "sheets"])
 def test_grid_rows_along_the_ray(gpu, oracle, case):
     """Lattices whose rows run along the integration direction (x0 = +-e_k: the lattice's first axis is its shortest
-    and steps along one world axis), where the integer term is counted per row: one ray per lattice row collects the
-    triangles its points can cross and each point tests those. Rows along +x, -x, +y and -z, rows of one point, and a
-    stack of 40 squares that every row through it crosses 40 times (more triangles than a row's list holds: those
-    rows' points are counted one by one). Every cell equals the batch path on the same points with the same x0 bit for
-    bit, and the integer terms equal NaiveIntersector<float>'s."""
+    and steps along one world axis), where the integer term is counted per row: each triangle is binned into the rows
+    that its leaf box spans and where it passes the u and v tests, and each point tests its row's triangles. Rows along
+    +x, -x, +y and -z, rows of one point, rows whose other two coordinates descend with their indices, and a stack of
+    80 squares that every row through it crosses 80 times (more triangles than a row's list holds: those rows' points
+    take the traversal). Every cell equals the batch path on the same points with the same x0 bit for bit, and the
+    integer terms equal NaiveIntersector<float>'s."""
     f = np.float32
     if case == "sheets":
-        v, idx = _sheets()
+        v, idx = _sheets(80)
     else:
         verts, idx = mg.holed_torus_mesh(nu=96, nv=40, holes=20, hole_radius=0.1)
         v = verts.astype(f)
@@ -168,6 +169,9 @@ def test_grid_rows_along_the_ray(gpu, oracle, case):
         dx, dz = ez * -hz, ex * f(3.0 / 19)
     elif case == "one_point_rows":
         nx = 1
+    elif case == "descending_y_z":  # the rows' other two coordinates fall with their indices
+        start = np.array([-1.5, 1.5, 0.5], dtype=f)
+        dy, dz = ey * f(-3.0 / 19), ez * -hz
     elif case == "sheets":
         start = np.array([-1.2, -0.8, -0.8], dtype=f)
         dx, dy, dz = ex * f(2.4 / 15), ey * f(1.6 / 19), ez * f(1.6 / 19)
@@ -187,7 +191,7 @@ def test_grid_rows_along_the_ray(gpu, oracle, case):
     assert np.array_equal(got, batch)
     assert np.count_nonzero(ref_cnt) > 0
     if case == "sheets":
-        assert np.abs(ref_cnt).max() >= 33  # rows whose triangles overflow a row's list
+        assert np.abs(ref_cnt).max() >= 65  # rows whose triangles overflow a row's list (64)
 
 
 def test_grid_rejects_bad_arguments(gpu):
