@@ -127,6 +127,28 @@ def test_int64_grid_fp64_form_at_its_limit(gpu, oracle, dt):
     _check_parity(got, ref, dt)
 
 
+@pytest.mark.parametrize("dt,pol", rc.CONFIGS)
+def test_stacked_squares(gpu, oracle, dt, pol):
+    """80 parallel unit squares x = const (two triangles each, sharing the diagonal y = z): rays along the robust axis
+    cross up to 80 of them, and the lattice queries with y = z meet the shared diagonals exactly, where only the exact
+    signs and the symbolic perturbation decide which triangle counts. Counts bit-exact, fractional to the bar."""
+    n = 80
+    xs = np.linspace(-0.9, 0.9, n)
+    corners = np.array([[0, -0.5, -0.5], [0, 0.5, -0.5], [0, 0.5, 0.5], [0, -0.5, 0.5]])
+    v = np.concatenate([corners + [x, 0, 0] for x in xs])
+    q = 4 * np.arange(n)[:, None]
+    idx = np.concatenate([q + [0, 1, 2], q + [0, 2, 3]], axis=1).reshape(-1).astype(np.int32)
+    bnd = oracle.boundary(v, idx)
+    t = np.linspace(-0.8, 0.8, 9)
+    yy, zz = np.meshgrid(t, t, indexing="ij")
+    xq = np.array([-1.2, -0.45, 0.0, 0.37, 1.2])
+    p = np.array([[x, y, z] for x in xq for y, z in zip(yy.ravel(), zz.ravel())])
+    print(f"case robust stacked squares {dt} {pol}", flush=True)
+    got, ref = _both(gpu, oracle, v, idx, bnd, p, pol, dt)
+    _check_parity(got, ref, dt)
+    assert np.abs(ref[1]).max() >= 40
+
+
 def test_empty_inputs(gpu):
     with gpu.RobustMeshGwn(np.zeros((0, 3)), np.zeros(0, dtype=np.int32), np.zeros((0, 7))) as r:
         p = mg.uniform_points(1, 100, [-1] * 3, [1] * 3)
