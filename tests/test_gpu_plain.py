@@ -179,6 +179,32 @@ def test_sorted_ray_counts_equal_per_lane_counts(gpu, oracle, dt):
     assert len(np.unique(big)) > 2  # inside, outside and deeper crossings all occur
 
 
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_stacked_squares_counts(gpu, oracle, dt):
+    """80 parallel unit squares x = const, each two triangles sharing the diagonal y = z: rays along +-x from lattice
+    queries cross up to 80 of them, and those with y = z run through the shared diagonals, where Moeller-Trumbore's
+    closed tests count both triangles or neither (the reference's caveat). Per-lane calls (small) and sorted packet
+    calls (2^16 queries and more, the queries repeated) both equal the FP32/FP64 restatement bit for bit."""
+    n = 80
+    xs = np.linspace(-0.9, 0.9, n)
+    corners = np.array([[0, -0.5, -0.5], [0, 0.5, -0.5], [0, 0.5, 0.5], [0, -0.5, 0.5]])
+    v = np.concatenate([corners + [x, 0, 0] for x in xs]).astype(dt)
+    q = 4 * np.arange(n)[:, None]
+    idx = np.concatenate([q + [0, 1, 2], q + [0, 2, 3]], axis=1).reshape(-1).astype(np.int32)
+    t = np.linspace(-0.8, 0.8, 17)
+    yy, zz = np.meshgrid(t, t, indexing="ij")
+    pos = np.array([[x, y, z] for x in np.linspace(-1.2, 1.2, 13) for y, z in zip(yy.ravel(), zz.ravel())], dtype=dt)
+    print(f"case stacked squares {np.dtype(dt).name}", flush=True)
+    for x0 in (np.array([1.0, 0.0, 0.0], dtype=dt), np.array([-1.0, 0.0, 0.0], dtype=dt)):
+        ref = oracle.naive_count(v, idx, pos, x0, dt)
+        with gpu.HipMesh(v, idx) as mesh:
+            small = gpu.eval_gwnr_mesh_batch_integer(mesh, pos, x0)
+            big = gpu.eval_gwnr_mesh_batch_integer(mesh, np.tile(pos, (20, 1)), x0)
+        assert np.array_equal(small, ref)
+        assert np.array_equal(big, np.tile(ref, 20))
+        assert np.abs(ref).max() >= 80
+
+
 def test_empty_and_degenerate_inputs(gpu):
     x0 = mg.example_x0(np.float64)
     # no queries
