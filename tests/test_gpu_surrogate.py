@@ -345,6 +345,36 @@ def test_grid_labels_over_many_poles(gpu, c1, trained_c1):
                 assert torch.equal(lab, first)
 
 
+def test_grid_labels_on_mirrored_lattices(gpu, c1, trained_c1):
+    """Lattices whose first axis steps along -x, or along +y with the other axes descending: x0 is then -e_x or +e_y
+    and the routed points' integer terms come from the lattice-row lists with rays running against or across the
+    other lattices' direction. Every point routed (delta = +inf): the labels equal eval_grid's on the same lattice."""
+    import torch
+
+    v, idx, lo, hi = c1
+    net, delta, info, bnd, x0 = trained_c1
+    f = np.float32
+    ex, ey, ez = np.eye(3, dtype=f)
+    ext = (hi - lo).astype(f)
+    with gpu.HipGridGwn() as grid:
+        grid.set_scene(v.astype(np.float32), idx, bnd)
+        for name, nx, ny, nz in (("minus_x", 40, 44, 48), ("plus_y", 36, 41, 45)):
+            if name == "minus_x":
+                start = np.array([hi[0], lo[1], lo[2]], dtype=f)
+                dx, dy, dz = ex * f(-ext[0] / (nx - 1)), ey * f(ext[1] / (ny - 1)), ez * f(ext[2] / (nz - 1))
+            else:
+                start = np.array([hi[0], lo[1], hi[2]], dtype=f)
+                dx, dy, dz = ey * f(ext[1] / (nx - 1)), ex * f(-ext[0] / (ny - 1)), ez * f(-ext[2] / (nz - 1))
+            print(f"lattice {name} {nx} x {ny} x {nz}", flush=True)
+            w = torch.empty(nx * ny * nz, dtype=torch.float32, device="cuda")
+            grid.eval_grid(start, dx, dy, dz, nx, ny, nz, out=w)
+            exact = (w > 0.5).to(torch.int8)
+            lab, st = net.label_grid(grid, start, dx, dy, dz, nx, ny, nz, float("inf"))
+            assert st["routed"] == nx * ny * nz
+            assert torch.equal(lab, exact), name
+            assert 0 < int(exact.sum()) < nx * ny * nz
+
+
 def test_point_labels_and_audit(gpu, c1, trained_c1):
     import torch
 
