@@ -499,8 +499,8 @@ __global__ void __launch_bounds__(kTB)
     }
 }
 
-// Integer terms of lattice points by rows. When d = +-e_k and the lattice is axis-aligned with its first axis along
-// e_k, the points of a lattice row differ only in coordinate k, and for such a d the replica's u and v do not depend on
+// Integer terms of lattice points by rows. When d = +-e_k and the lattice is axis-aligned with one of its axes (L, the
+// rows' axis) along e_k, the points of a lattice row differ only in coordinate k, and for such a d the replica's u and v do not depend on
 // that coordinate (h_k = d_j e2_l - d_l e2_j = +-0 and d . q = +-q_k, with q_k free of s_k): the points of a row pass
 // u and v on the same triangles and differ only in t. A live triangle that the replica counts for a point p has its
 // crossing, which shares p's other two coordinates, inside its leaf box (the box holds every point where the replica
@@ -512,17 +512,28 @@ __global__ void __launch_bounds__(kTB)
 constexpr int kRowHits = 64;
 constexpr int64_t kMaxRows = int64_t(1) << 20; // lattice rows (ny nz) up to which the row counts are used
 
-// Axis-aligned lattice in row form: coordinate k (the rows' axis) of point (ix, ., .) is xk(ix); coordinate a of row
-// iy is ya(iy) and coordinate b of row iz is zb(iz), each the float expression of query_at with the zero terms dropped
-// (adding a zero does not change a nonzero value, and the test's comparisons do not see the sign of a zero).
+// Axis-aligned lattice in row form. The rows run along lattice axis L (world axis k); the other two lattice axes P < Q
+// step along world axes a and b, and row (iP, iQ) is number iQ nP + iP. Coordinate a of row (iP, .) is sa + ha iP and
+// coordinate b of row (., iQ) is sb + hb iQ, the float expressions of query_at with the zero terms dropped (adding a
+// zero does not change a nonzero value, and the test's comparisons do not see the sign of a zero).
 struct row_frame
 {
     int k, a, b;
     int fwd; // d = +e_k
     float sa, ha, sb, hb;
     float ia, ib; // 1 / ha, 1 / hb (estimates only)
-    int64_t ny, nz;
+    int64_t ny, nz; // nP, nQ
 };
+
+// the row of lattice point i, rows along lattice axis L
+__device__ __forceinline__ int64_t lattice_row(grid_desc const& g, int L, int64_t i)
+{
+    int64_t ix, iy, iz;
+    if (L == 0)
+        return div_recip(i, g.nx, g.inx, ix); // iz ny + iy
+    grid_cell(g, i, ix, iy, iz);
+    return (L == 1 ? iz : iy) * g.nx + ix;
+}
 
 // first index i of [0, n) with lo <= s + h i <= hi in float (none: n); the values are monotone in i, so the indices
 // that satisfy the test are contiguous, and the walk from the estimate to their start is exact whatever its error
@@ -607,7 +618,7 @@ __global__ void __launch_bounds__(kB)
 // row with an incomplete list takes the packet traversal.
 template <bool ALL>
 __global__ void __launch_bounds__(kTB)
-    k_row_count(grid_desc g, int k, float const* __restrict__ pts, int64_t const* __restrict__ li, int64_t m,
+    k_row_count(grid_desc g, int L, int k, float const* __restrict__ pts, int64_t const* __restrict__ li, int64_t m,
                 float dx, float dy, float dz, bvh_node<float> const* __restrict__ nodes,
                 leaf_rec<float> const* __restrict__ rec, int32_t const* __restrict__ brute,
                 int32_t const* __restrict__ nbrute_p, int2 const* __restrict__ hits, int32_t const* __restrict__ nh,
@@ -622,18 +633,17 @@ __global__ void __launch_bounds__(kTB)
     int n = 0;
     if (valid)
     {
-        int64_t ix;
         if constexpr (ALL)
         {
             query_at(src_grid{g}, e, px, py, pz);
-            row = div_recip(e, g.nx, g.inx, ix);
+            row = lattice_row(g, L, e);
         }
         else
         {
             px = pts[3 * e];
             py = pts[3 * e + 1];
             pz = pts[3 * e + 2];
-            row = div_recip(li[e] >> 1, g.nx, g.inx, ix);
+            row = lattice_row(g, L, li[e] >> 1);
         }
         n = nh[row];
     }
@@ -1006,7 +1016,7 @@ struct mesh_t final : mesh_impl
         return ok;
     }
 
-    // Integer terms by rows (k_row_bin) when d = +-e_k and lattice g is axis-aligned with its rows along e_k; *done =
+    // Integer terms by rows (k_row_bin) when d = +-e_k and lattice g is axis-aligned with one axis along e_k; *done =
     // false (nothing launched) otherwise. li null: all m = nx ny nz lattice points, in the lattice's order; else m
     // routed points (positions pts; entries li in index order, query index li >> 1). All of norm <= qmax. Temporaries
     // come from ws, or are allocated on s when ws is null.
@@ -1015,8 +1025,7 @@ struct mesh_t final : mesh_impl
     {
         static_assert(std::is_same_v<T, float>, "row counts are FP32");
         done = false;
-        int64_t const rows = g.ny * g.nz;
-        if (m == 0 || ntri < 2 || rows > kMaxRows)
+        if (m == 0 || ntri < 2)
             return ok;
         // the axis of each lattice step's one nonzero component (-1: none or several)
         auto axis = [](float const* v) {
@@ -1029,11 +1038,21 @@ struct mesh_t final : mesh_impl
                 }
             return nz == 1 ? ax : -1;
         };
-        int const k = axis(g.dx), a = axis(g.dy), b = axis(g.dz);
-        if (k < 0 || a < 0 || b < 0 || k == a || k == b || a == b || axis(d) != k || std::fabs(d[k]) != 1.0f)
+        float const* step[3] = {g.dx, g.dy, g.dz};
+        int64_t const cnt[3] = {g.nx, g.ny, g.nz};
+        int const ax[3] = {axis(g.dx), axis(g.dy), axis(g.dz)};
+        int const k = axis(d);
+        if (k < 0 || std::fabs(d[k]) != 1.0f || ax[0] < 0 || ax[1] < 0 || ax[2] < 0 || ax[0] == ax[1] ||
+            ax[0] == ax[2] || ax[1] == ax[2])
             return ok;
-        row_frame const F{k, a, b, d[k] > 0.0f ? 1 : 0, g.start[a], g.dy[a], g.start[b], g.dz[b], 1.0f / g.dy[a],
-                          1.0f / g.dz[b], g.ny, g.nz};
+        int const L = ax[0] == k ? 0 : (ax[1] == k ? 1 : 2); // the lattice axis along the ray
+        int const P = L == 0 ? 1 : 0, Q = L == 2 ? 1 : 2;
+        int64_t const rows = cnt[P] * cnt[Q];
+        if (rows > kMaxRows)
+            return ok;
+        int const a = ax[P], b = ax[Q];
+        row_frame const F{k, a, b, d[k] > 0.0f ? 1 : 0, g.start[a], step[P][a], g.start[b], step[Q][b],
+                          1.0f / step[P][a], 1.0f / step[Q][b], cnt[P], cnt[Q]};
         std::shared_ptr<dir_state<T>> st;
         AGWN_TRY(prepare(d, qmax, s, st));
         int32_t* nh = nullptr;
@@ -1059,10 +1078,11 @@ struct mesh_t final : mesh_impl
         AGWN_CHECK(blocks < (int64_t(1) << 31), "too many queries (%lld)", (long long)m);
         if (li)
             k_row_count<false><<<dim3(unsigned(blocks)), dim3(kTB), 0, s>>>(
-                g, k, pts, li, m, d[0], d[1], d[2], st->nodes, st->rec, st->brute, st->nbrute, hits, nh, out);
+                g, L, k, pts, li, m, d[0], d[1], d[2], st->nodes, st->rec, st->brute, st->nbrute, hits, nh, out);
         else
-            k_row_count<true><<<dim3(unsigned(blocks)), dim3(kTB), 0, s>>>(
-                g, k, nullptr, nullptr, m, d[0], d[1], d[2], st->nodes, st->rec, st->brute, st->nbrute, hits, nh, out);
+            k_row_count<true><<<dim3(unsigned(blocks)), dim3(kTB), 0, s>>>(g, L, k, nullptr, nullptr, m, d[0], d[1], d[2],
+                                                                           st->nodes, st->rec, st->brute, st->nbrute,
+                                                                           hits, nh, out);
         AGWN_HIP(hipGetLastError());
         AGWN_HIP(hipStreamSynchronize(s)); // the state outlives its kernels
         done = true;

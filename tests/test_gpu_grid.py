@@ -139,15 +139,15 @@ def _sheets(n: int):
 
 
 @pytest.mark.parametrize("case", ["plus_x", "minus_x", "rows_along_y", "rows_along_minus_z", "one_point_rows",
-                                  "descending_y_z", "sheets"])
+                                  "descending_y_z", "second_axis", "third_axis", "sheets"])
 def test_grid_rows_along_the_ray(gpu, oracle, case):
-    """Lattices whose rows run along the integration direction (x0 = +-e_k: the lattice's first axis is its shortest
-    and steps along one world axis), where the integer term is counted per row: each triangle is binned into the rows
-    that its leaf box spans and where it passes the u and v tests, and each point tests its row's triangles. Rows along
-    +x, -x, +y and -z, rows of one point, rows whose other two coordinates descend with their indices, and a stack of
-    80 squares that every row through it crosses 80 times (more triangles than a row's list holds: those rows' points
-    take the traversal). Every cell equals the batch path on the same points with the same x0 bit for bit, and the
-    integer terms equal NaiveIntersector<float>'s."""
+    """Axis-aligned lattices, whose rows along the integration axis (the lattice axis with the fewest cells, x0 = +-e_k)
+    take the row-list integer term: each triangle is binned into the rows that its leaf box spans and where it passes
+    the u and v tests, and each point tests its row's triangles. Rows along +x, -x, +y and -z, rows of one point, rows
+    whose other two coordinates descend with their indices, rows along the lattice's second and third axes, and a
+    stack of 80 squares that every row through it crosses 80 times (more triangles than a row's list holds: those
+    rows' points take the traversal). Every cell equals the batch path on the same points with the same x0 bit for
+    bit, and the integer terms equal NaiveIntersector<float>'s."""
     f = np.float32
     if case == "sheets":
         v, idx = _sheets(80)
@@ -172,16 +172,23 @@ def test_grid_rows_along_the_ray(gpu, oracle, case):
     elif case == "descending_y_z":  # the rows' other two coordinates fall with their indices
         start = np.array([-1.5, 1.5, 0.5], dtype=f)
         dy, dz = ey * f(-3.0 / 19), ez * -hz
+    elif case == "second_axis":  # the lattice's y axis has the fewest cells and steps along world x
+        nx, ny, nz = 20, 12, 18
+        dx, dy = ey * f(3.0 / 19), ex * f(3.0 / 11)
+    elif case == "third_axis":  # the lattice's z axis has the fewest cells and steps along world -z
+        nx, ny, nz = 20, 18, 10
+        start = np.array([-1.5, -1.5, 0.5], dtype=f)
+        dx, dy, dz = ex * f(3.0 / 19), ey * f(3.0 / 17), ez * f(-1.0 / 9)
     elif case == "sheets":
         start = np.array([-1.2, -0.8, -0.8], dtype=f)
         dx, dy, dz = ex * f(2.4 / 15), ey * f(1.6 / 19), ez * f(1.6 / 19)
         nz = 20
-    assert nx <= min(ny, nz)  # the first axis is the integration axis
     with gpu.HipGridGwn() as g:
         g.set_scene(v, idx, bnd)
         got = g.eval_grid(start, dx, dy, dz, nx, ny, nz)
     pts = lattice(start, dx, dy, dz, nx, ny, nz)
-    x0 = dx / f(np.sqrt(f(dx[0] * dx[0] + dx[1] * dx[1]) + f(dx[2] * dx[2])))
+    a = [dx, dy, dz][int(np.argmin([nx, ny, nz]))]  # the evaluator's integration axis: fewest cells, ties to the first
+    x0 = a / f(np.sqrt(f(a[0] * a[0] + a[1] * a[1]) + f(a[2] * a[2])))
     assert np.count_nonzero(x0) == 1 and np.abs(x0).max() == 1.0
     with gpu.HipMesh(v, idx) as mesh:
         batch = gpu.eval_gwnr_mesh_batch(mesh, pts, x0, bnd)

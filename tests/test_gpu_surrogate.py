@@ -346,9 +346,10 @@ def test_grid_labels_over_many_poles(gpu, c1, trained_c1):
 
 
 def test_grid_labels_on_mirrored_lattices(gpu, c1, trained_c1):
-    """Lattices whose first axis steps along -x, or along +y with the other axes descending: x0 is then -e_x or +e_y
-    and the routed points' integer terms come from the lattice-row lists with rays running against or across the
-    other lattices' direction. Every point routed (delta = +inf): the labels equal eval_grid's on the same lattice."""
+    """Lattices whose first axis (their integration axis) steps along -x, or along +y with the other axes descending,
+    and one whose third axis does (along -z): x0 is then -e_x, +e_y or -e_z, and the routed points' integer terms come
+    from the lattice-row lists with rays against or across the other lattices' direction. Every point routed (delta =
+    +inf): the labels equal eval_grid's on the same lattice."""
     import torch
 
     v, idx, lo, hi = c1
@@ -358,10 +359,13 @@ def test_grid_labels_on_mirrored_lattices(gpu, c1, trained_c1):
     ext = (hi - lo).astype(f)
     with gpu.HipGridGwn() as grid:
         grid.set_scene(v.astype(np.float32), idx, bnd)
-        for name, nx, ny, nz in (("minus_x", 40, 44, 48), ("plus_y", 36, 41, 45)):
+        for name, nx, ny, nz in (("minus_x", 40, 44, 48), ("plus_y", 36, 41, 45), ("minus_z", 42, 38, 30)):
             if name == "minus_x":
                 start = np.array([hi[0], lo[1], lo[2]], dtype=f)
                 dx, dy, dz = ex * f(-ext[0] / (nx - 1)), ey * f(ext[1] / (ny - 1)), ez * f(ext[2] / (nz - 1))
+            elif name == "minus_z":
+                start = np.array([lo[0], lo[1], hi[2]], dtype=f)
+                dx, dy, dz = ex * f(ext[0] / (nx - 1)), ey * f(ext[1] / (ny - 1)), ez * f(-ext[2] / (nz - 1))
             else:
                 start = np.array([hi[0], lo[1], hi[2]], dtype=f)
                 dx, dy, dz = ey * f(ext[1] / (nx - 1)), ex * f(-ext[0] / (ny - 1)), ez * f(-ext[2] / (nz - 1))
